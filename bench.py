@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the shallow-water hot path on one MI355X.
+
+Workload (default): the forward drop-in at scale -- doubly periodic planar
+hexagonal mesh nx = ny = 512 (262,144 cells, 786,432 edges), dc = 5 km,
+64 layers, 4 tracers, FP64, RK4, Gaussian double vortex in geostrophic
+balance, momentum del2 viscosity 1e3 m^2/s, tracer del4 1e9 m^4/s, the step
+captured in a HIP graph.  A step is one full RK4 step (4 tendency
+evaluations + stage updates); the metric counts cell-layer-steps/s.
+
+Prints one JSON line.  `roofline` is measured on the dominant kernel (the
+fused tendency + stage-update kernel) with HIP events on its launch stream;
+`cpu_baseline` times the CPU oracle (OpenMP, all host cores of the job) on a
+bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from omega_playground_amd import swe  # noqa: E402
+from omega_playground_amd import workloads as wl  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_case(cfg_name: str, ordering: str = "hilbert"):
+    cfg = dict(wl.CONFIGS[cfg_name])
+    mesh = swe.PlanarHexagonalMesh(cfg["nx"], cfg["nx"], cfg["dc"], cfg["nlayers"], ordering=ordering)
+    params = swe.ShallowWaterParams(f0=cfg["f0"], grav=cfg["grav"], visc_del2=cfg["visc_del2"],
+                                    eddy_diff4=cfg["eddy_diff4"], ntracers=cfg["ntracers"])
+    model = swe.ShallowWaterModel(mesh, params)
+    state0 = wl.make_state(mesh, cfg, seed=0)
+    return cfg, mesh, params, model, state0
+
+
+def alg_bytes_per_cell_layer_step(ntr: int, stepper: str) -> float:
+    """B_alg: compulsory state traffic per cell-layer-step, S = 8*(1+3+ntr)
+    bytes (one h, three vn, ntr tracers per cell); RK4 16*S, LSRK5 18*S."""
+    S = 8.0 * (4 + ntr)
+    return (16.0 if stepper == "rk4" else 18.0) * S
+
+
+def cpu_baseline(cfg, sample_steps: int, threads: int):
+    from oracle import pyoracle as po
+
+    po.set_num_threads(threads)
+    om = po.Mesh(cfg["nx"], cfg["nx"], cfg["dc"], cfg["nlayers"])
+    omodel = po.Model(om, f0=cfg["f0"], grav=cfg["grav"], visc_del2=cfg["visc_del2"],
+                      eddy_diff4=cfg["eddy_diff4"], ntracers=cfg["ntracers"])
+    st = po.Stepper(omodel, cfg["stepper"])
+    state = tuple(np.ascontiguousarray(a) for a in wl.make_state(om, cfg, seed=0))
+    dt = wl.dt_rule(cfg["dc"], cfg["grav"])
+    st.run(0.0, dt, 1, state)  # warm-up step (reference protocol)
+    t0 = time.perf_counter()
+    st.run(dt, dt, sample_steps, state)
+    el = time.perf_counter() - t0
+    work = om.ncells * om.nlayers * sample_steps
+    return {
+        "value": work / el,
+        "unit": "cell-layer-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{sample_steps} RK4 steps of the same nx={cfg['nx']} workload after one "
+                  f"warm-up step ({el:.2f} s), oracle C++/OpenMP, {threads} threads",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--ordering", default="hilbert")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--launch", default="", help="rows,lpt,swizzle")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    cfg, mesh, params, model, state0 = build_case(args.config, args.ordering)
+    if args.launch:
+        r, l, s = (int(x) for x in args.launch.split(","))
+        model.set_launch(r, l, s)
+    ntr, L, nc = cfg["ntracers"], cfg["nlayers"], mesh.ncells
+    dt = wl.dt_rule(cfg["dc"], cfg["grav"])
+    state = swe.ShallowWaterState(mesh, ntr)
+    state.upload(*state0)
+    stepper = swe.RK4Stepper(model) if cfg["stepper"] == "rk4" else swe.LSRKStepper(model, 5)
+    stepper.set_graph(not args.no_graph)
+    log(f"[bench] {args.config}: nx={cfg['nx']} cells={nc} L={L} ntr={ntr} dt={dt:.4f}s "
+        f"launch={model.get_launch()} graph={not args.no_graph}")
+
+    # warm-up
+    stepper.run(0.0, dt, args.warmup, state)
+    swe.synchronize()
+    # timed region: K steps, device-synchronised on both sides
+    t0 = time.perf_counter()
+    stepper.run(args.warmup * dt, dt, args.steps, state)
+    swe.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / max(args.steps, 1)
+    work = nc * L * args.steps
+    value = work / elapsed
+    h, vn, tr = state.download()
+    finite = bool(np.isfinite(h).all() and np.isfinite(vn).all() and np.isfinite(tr).all())
+    log(f"[bench] {args.steps} steps in {elapsed:.4f}s = {ms_per_step:.3f} ms/step, "
+        f"{value:.4e} cell-layer-steps/s, finite={finite}")
+
+    # per-kernel durations with HIP events on the launch stream (eager steps)
+    ksteps = max(2, min(10, args.steps))
+    stepper.set_graph(False)
+    swe.timing_enable(True)
+    swe.timing_reset()
+    stepper.run(0.0, dt, ksteps, state)
+    swe.synchronize()
+    tend_ms, tend_n = swe.timing_read("tend")
+    diag_ms, diag_n = swe.timing_read("diag")
+    swe.timing_enable(False)
+    avg_tend_ms = tend_ms / max(tend_n, 1)
+    avg_diag_ms = diag_ms / max(diag_n, 1)
+    B = alg_bytes_per_cell_layer_step(ntr, cfg["stepper"])
+    # the fused tendency + update kernel carries the stage's state traffic:
+    # B_alg per step spread over the step's 4 (RK4) launches
+    nlaunch = 4 if cfg["stepper"] == "rk4" else 5
+    alg_bytes_launch = B * nc * L / nlaunch
+    achieved = alg_bytes_launch / (avg_tend_ms * 1e-3) / 1e9
+    step_achieved = B * value / 1e9
+    log(f"[bench] tend kernel avg {avg_tend_ms:.4f} ms x{tend_n}, diag avg {avg_diag_ms:.4f} ms "
+        f"x{diag_n}; B_alg {B:.0f} B/cell-layer-step")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        threads = min(threads, 16)
+        log(f"[bench] CPU baseline: {args.cpu_steps} steps, {threads} threads")
+        cpu = cpu_baseline(cfg, args.cpu_steps, threads)
+        log(f"[bench] CPU {cpu['value']:.4e} cell-layer-steps/s")
+
+    out = {
+        "metric": "cell-layer-steps/s FP64 fwd+adjoint at 1 GPU; adjoint/fwd ratio; batched speedup",
+        "value": value,
+        "unit": "cell-layer-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"forward RK4, planar hex nx={cfg['nx']} ({nc} cells), dc="
+                               f"{cfg['dc'] / 1e3:g} km, {L} layers, {ntr} tracers, Gaussian double "
+                               f"vortex, del2 visc {cfg['visc_del2']:g}, tracer del4 "
+                               f"{cfg['eddy_diff4']:g}, hipGraph step",
+                   "ncells": nc, "nlayers": L, "ntracers": ntr, "stepper": cfg["stepper"],
+                   "graph": not args.no_graph, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_tend (fused tendency + RK4 stage update)",
+                     "kernel_avg_ms": avg_tend_ms,
+                     "alg_bytes_per_launch": alg_bytes_launch},
+        "step_roofline": {"B_alg_per_cell_layer_step": B, "achieved_GBs": step_achieved,
+                          "frac": step_achieved / HBM_PEAK_GBS, "diag_kernel_avg_ms": avg_diag_ms},
+        "finite": finite,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,280 @@
+/*
+ * omega_swe.h -- C ABI of the MI355X shallow-water hot path.
+ *
+ * Drop-in boundary for mwarusz/omega_playground's mesh / state / tendency /
+ * time-stepper API.  Every entry point names the reference interface it
+ * replaces (file:line in omega_playground).  The C++ classes in
+ * omega_playground_amd/cpp/omega (namespace omega) call these functions;
+ * Python reaches them through ctypes.
+ *
+ * Conventions
+ *  - Every function returns OMEGA_OK (0) or an error code; the message of the
+ *    last failure on the calling thread is omega_last_error().  Nothing
+ *    throws across this boundary.
+ *  - Host arrays handed in or out use the reference's original entity
+ *    numbering and LayoutRight shapes (level fastest):
+ *      h[ncells][nlayers], vn[nedges][nlayers], tr[ntracers][ncells][nlayers].
+ *  - Device arrays (omega_state_device_ptrs, the additional-tendency hook,
+ *    omega_mesh_device_array) use the library's device numbering: cells in
+ *    space-filling-curve order, edges and vertices grouped by owning cell.
+ *    omega_mesh_permutation() maps between the two.
+ *  - Launching calls take a hipStream_t (passed as void*; NULL is the null
+ *    stream) and are asynchronous with respect to the host unless stated.
+ *  - All arithmetic is IEEE binary64; no kernel uses atomics, so results are
+ *    bit-identical from run to run.
+ */
+#ifndef OMEGA_SWE_H
+#define OMEGA_SWE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OMEGA_OK 0
+#define OMEGA_ERR_ARG 1      /* invalid argument or size */
+#define OMEGA_ERR_HIP 2      /* HIP runtime failure */
+#define OMEGA_ERR_STATE 3    /* object used in an invalid state */
+#define OMEGA_ERR_NOTFOUND 4 /* unknown array name */
+#define OMEGA_ERR_CALLBACK 5 /* user hook returned non-zero */
+
+typedef struct omega_mesh_s *omega_mesh_t;
+typedef struct omega_model_s *omega_model_t;
+typedef struct omega_state_s *omega_state_t;
+typedef struct omega_stepper_s *omega_stepper_t;
+typedef void *omega_stream_t; /* hipStream_t */
+
+/* Message of the last error on this thread ("" if none). */
+const char *omega_last_error(void);
+/* Library version string. */
+const char *omega_version(void);
+
+/* ----------------------------------------------------------------------
+ * Mesh
+ * -------------------------------------------------------------------- */
+
+/* Cell ordering on the device. */
+#define OMEGA_ORDER_ORIGINAL 0 /* reference numbering */
+#define OMEGA_ORDER_HILBERT 1  /* cells along a Hilbert curve (default) */
+#define OMEGA_ORDER_MORTON 2   /* cells along a Morton (Z) curve */
+
+/* Replaces PlanarHexagonalMesh(nx, ny, dc, nlayers) + MPASMesh::finalize_mesh
+ * (src/mesh/planar_hexagonal_mesh.hpp:17-18,
+ *  src/mesh/planar_hexagonal_mesh.cpp:8-398, src/mesh/mpas_mesh.cpp:5-57).
+ * Builds the doubly periodic hexagonal mesh on the host in the reference's
+ * numbering and uploads it reordered.  nlayers is used as given (no padding
+ * to a SIMD width). */
+int omega_mesh_create_planar_hex(int nx, int ny, double dc, int nlayers, int ordering,
+                                 omega_mesh_t *out);
+
+/* Replaces constructing an MPASMesh from its arrays followed by
+ * MPASMesh::finalize_mesh (src/mesh/mpas_mesh.hpp:7-71,
+ * src/mesh/mpas_mesh.cpp:5-57).  Arrays are host, original numbering.
+ * edge_sign_on_cell / kite_index_on_cell / edge_sign_on_vertex are derived as
+ * finalize_mesh does.  edge_mask is 1 below max_level_edge_top (NULL: all
+ * levels); mesh_scaling_del2/4 NULL means 1. */
+typedef struct {
+  int ncells, nedges, nvertices, nlayers;
+  int max_edges;                 /* row length of *_on_cell arrays (>= 3) */
+  int max_edges2;                /* row length of edges_on_edge / weights_on_edge */
+  const int *nedges_on_cell;     /* [ncells] */
+  const int *edges_on_cell;      /* [ncells][max_edges] */
+  const int *vertices_on_cell;   /* [ncells][max_edges] */
+  const int *nedges_on_edge;     /* [nedges] */
+  const int *edges_on_edge;      /* [nedges][max_edges2] */
+  const int *cells_on_edge;      /* [nedges][2] */
+  const int *vertices_on_edge;   /* [nedges][2] */
+  const int *edges_on_vertex;    /* [nvertices][3] */
+  const int *cells_on_vertex;    /* [nvertices][3] */
+  const double *area_cell;       /* [ncells] */
+  const double *dc_edge;         /* [nedges] */
+  const double *dv_edge;         /* [nedges] */
+  const double *weights_on_edge; /* [nedges][max_edges2] */
+  const double *area_triangle;   /* [nvertices] */
+  const double *kiteareas_on_vertex; /* [nvertices][3] */
+  const double *mesh_scaling_del2;   /* [nedges] or NULL */
+  const double *mesh_scaling_del4;   /* [nedges] or NULL */
+  const int *max_level_edge_top;     /* [nedges] or NULL */
+  /* optional geometry (kept for host queries; may be NULL) */
+  const double *x_cell, *y_cell, *x_edge, *y_edge, *angle_edge, *x_vertex, *y_vertex;
+} omega_mpas_mesh_desc;
+
+int omega_mesh_create_mpas(const omega_mpas_mesh_desc *desc, int ordering, omega_mesh_t *out);
+
+int omega_mesh_destroy(omega_mesh_t mesh);
+
+/* counts[0..5] = ncells, nedges, nvertices, nlayers, nx, ny (nx = ny = 0 for
+ * a general MPAS mesh); reals[0..2] = dc, period_x, period_y. */
+int omega_mesh_info(omega_mesh_t mesh, int *counts, double *reals);
+
+/* Host copy of an MPASMesh field in the original numbering, by the
+ * reference's field name without the m_ prefix (e.g. "edges_on_cell",
+ * "x_cell", "weights_on_edge"; mpas_mesh.hpp:7-71).  dtype: 0 int32,
+ * 1 float64.  The pointer stays valid for the mesh's lifetime. */
+int omega_mesh_host_array(omega_mesh_t mesh, const char *name, const void **ptr, int *dtype,
+                          int64_t *n0, int64_t *n1);
+
+/* Device copy of a geometry field in device numbering ("x_cell", "y_cell",
+ * "x_edge", "y_edge", "angle_edge", "x_vertex", "y_vertex", "area_cell"). */
+int omega_mesh_device_array(omega_mesh_t mesh, const char *name, const double **dptr,
+                            int64_t *n);
+
+/* new2old[i] = original index of device entity i.  kind: 0 cells,
+ * 1 edges, 2 vertices. */
+int omega_mesh_permutation(omega_mesh_t mesh, int kind, int *new2old);
+
+/* ----------------------------------------------------------------------
+ * State (ShallowWaterState, src/model/shallow_water_state.hpp:9-16)
+ * -------------------------------------------------------------------- */
+
+/* Replaces ShallowWaterState(const MPASMesh*, Int ntracers)
+ * (src/model/shallow_water_state.cpp:7-14).  Device arrays are zeroed, as
+ * Kokkos views are. */
+int omega_state_create(omega_mesh_t mesh, int ntracers, omega_state_t *out);
+int omega_state_destroy(omega_state_t state);
+
+/* Host <-> device copies in the original numbering (the drop-in for
+ * Kokkos::deep_copy between host mirrors and the state views).  NULL skips a
+ * field.  Download synchronises the stream before returning. */
+int omega_state_upload(omega_state_t state, const double *h, const double *vn, const double *tr,
+                       omega_stream_t stream);
+int omega_state_download(omega_state_t state, double *h, double *vn, double *tr,
+                         omega_stream_t stream);
+
+/* Current device buffers (device numbering).  Low-storage RK alternates two
+ * buffers per field, so re-query after stepping. */
+int omega_state_device_ptrs(omega_state_t state, double **h, double **vn, double **tr);
+
+/* deep_copy(dst, src) of all fields (same mesh and tracer count). */
+int omega_state_copy(omega_state_t dst, omega_state_t src, omega_stream_t stream);
+
+/* ----------------------------------------------------------------------
+ * Model (ShallowWaterModel, src/model/shallow_water_model.hpp:12-51)
+ * -------------------------------------------------------------------- */
+
+/* Field order of ShallowWaterParams (src/model/shallow_water_params.hpp:7-18). */
+typedef struct {
+  double f0;
+  double grav;
+  double drag_coeff; /* carried, unused as in the reference */
+  double visc_del2;
+  double visc_del4;
+  double eddy_diff2;
+  double eddy_diff4;
+  int ntracers;
+  int disable_h_tendency;
+  int disable_vn_tendency;
+} omega_params;
+
+/* Replaces ShallowWaterModel(MPASMesh*, const ShallowWaterParams&)
+ * (src/model/shallow_water_model.cpp:5-41): terms are enabled from the
+ * parameters exactly as there; f_vertex = f_edge = f0. */
+int omega_model_create(omega_mesh_t mesh, const omega_params *params, omega_model_t *out);
+int omega_model_destroy(omega_model_t model);
+
+/* Overwrites m_f_vertex (host array, original numbering). */
+int omega_model_set_f_vertex(omega_model_t model, const double *f_vertex);
+
+/* Replaces the virtual ShallowWaterModel::additional_tendency
+ * (src/model/shallow_water_model.hpp:42-44, called at
+ * shallow_water_model.cpp:344-345).  The hook runs on the same stream after
+ * the built-in terms of every tendency evaluation and must only enqueue
+ * work on that stream.  Pointers are device buffers in device numbering.
+ * A non-zero return aborts the call with OMEGA_ERR_CALLBACK.  fn = NULL
+ * removes the hook. */
+typedef int (*omega_additional_tendency_fn)(void *user, double *h_tend, double *vn_tend,
+                                            const double *h, const double *vn, double t,
+                                            omega_stream_t stream);
+int omega_model_set_additional_tendency(omega_model_t model, omega_additional_tendency_fn fn,
+                                        void *user);
+
+#define OMEGA_ADD_REPLACE 0   /* AddMode::replace */
+#define OMEGA_ADD_INCREMENT 1 /* AddMode::increment */
+
+/* Replaces ShallowWaterModel::compute_tendency(tend, state, t, add_mode)
+ * (src/model/shallow_water_model.cpp:313-347). */
+int omega_model_compute_tendency(omega_model_t model, omega_state_t tend, omega_state_t state,
+                                 double t, int add_mode, omega_stream_t stream);
+
+/* Term switches for tests (bit 0 PV flux, 1 KE gradient, 2 SSH gradient,
+ * 3 momentum del2, 4 momentum del4, 5 tracer del2, 6 tracer del4); the
+ * default is what the constructor enabled.  No reference counterpart: the
+ * reference toggles the same m_enabled flags. */
+int omega_model_set_terms(omega_model_t model, int mask);
+int omega_model_get_terms(omega_model_t model, int *mask);
+
+/* Launch shape: rows (entities) per block, levels per thread (1 or 2) and
+ * whether blocks are swizzled so each XCD walks a contiguous stretch of the
+ * cell curve.  0 keeps the current value. */
+int omega_model_set_launch(omega_model_t model, int rows_per_block, int levels_per_thread,
+                           int xcd_swizzle);
+int omega_model_get_launch(omega_model_t model, int *rows_per_block, int *levels_per_thread,
+                           int *xcd_swizzle);
+
+/* ----------------------------------------------------------------------
+ * Time steppers (src/time_integration/time_stepper.hpp:8-38)
+ * -------------------------------------------------------------------- */
+
+#define OMEGA_STEPPER_RK4 0
+#define OMEGA_STEPPER_LSRK 1
+
+/* Replaces RK4Stepper(model) (src/time_integration/rk4.cpp:8-25) and
+ * LSRKStepper(model, nstages) (src/time_integration/lsrk.cpp:5-34). */
+int omega_stepper_create(omega_model_t model, int kind, int nstages, omega_stepper_t *out);
+int omega_stepper_destroy(omega_stepper_t stepper);
+
+/* Replaces TimeStepper::do_step(t, dt, state)
+ * (src/time_integration/rk4.cpp:28-104, src/time_integration/lsrk.cpp:36-89).
+ * Without an additional-tendency hook each stage is one diagnostics launch
+ * and one fused tendency + stage-update launch. */
+int omega_stepper_do_step(omega_stepper_t stepper, double t, double dt, omega_state_t state,
+                          omega_stream_t stream);
+
+/* nsteps calls of do_step with t = t0 + (i + t_offset_steps) * dt.  With
+ * graphs enabled the step is captured once per (dt, state, buffer parity)
+ * and replayed. */
+int omega_stepper_run(omega_stepper_t stepper, double t0, double dt, int nsteps,
+                      int t_offset_steps, omega_state_t state, omega_stream_t stream);
+int omega_stepper_set_graph(omega_stepper_t stepper, int enable);
+
+/* ----------------------------------------------------------------------
+ * Diagnostics (src/model/shallow_water_diagnostics.cpp:5-89)
+ * Deterministic two-level reductions; results are written to *out after
+ * the stream is synchronised.
+ * -------------------------------------------------------------------- */
+int omega_energy_integral(omega_model_t model, omega_state_t state, double *out,
+                          omega_stream_t stream);
+int omega_mass_integral(omega_model_t model, omega_state_t state, double *out,
+                        omega_stream_t stream);
+int omega_circulation_integral(omega_model_t model, omega_state_t state, double *out,
+                               omega_stream_t stream);
+
+/* ----------------------------------------------------------------------
+ * Utilities
+ * -------------------------------------------------------------------- */
+
+/* dst[i][k] += alpha * src[i][k] for a device array dst[n][ncols] and a host
+ * array src given in the ORIGINAL numbering of entity kind (0 cells, 1
+ * edges, 2 vertices) of the mesh; used by host-side additional-tendency
+ * hooks.  ncols may be 1 (broadcast over the levels) or nlayers. */
+int omega_add_host_field(omega_mesh_t mesh, int kind, double *dst, const double *src, int ncols,
+                         double alpha, omega_stream_t stream);
+
+/* Waits for all work on the stream (NULL: the whole device). */
+int omega_synchronize(omega_stream_t stream);
+
+/* Per-kernel timing with HIP events recorded on the launch stream around
+ * every launch of the library's kernels.  Off by default. */
+int omega_timing_enable(int enable);
+int omega_timing_reset(void);
+/* name: "diag", "tend", "vdel4", "update", "reduce"; returns summed
+ * milliseconds and launch count since the last reset (synchronises). */
+int omega_timing_read(const char *name, double *total_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OMEGA_SWE_H */

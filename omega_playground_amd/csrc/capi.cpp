@@ -1,0 +1,1045 @@
+// C ABI of the shallow-water library: object lifetimes, stage orchestration,
+// graph capture, diagnostics and timing.
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <typeinfo>
+
+#include "internal.hpp"
+
+using namespace omega_hip;
+
+// ---------------------------------------------------------------------------
+// Opaque objects
+// ---------------------------------------------------------------------------
+struct omega_mesh_s {
+  DeviceMesh dm;
+  DBuf<Real> staging; // device scratch for omega_add_host_field
+};
+
+struct omega_state_s {
+  omega_mesh_s *mesh = nullptr;
+  int ntr = 0;
+  size_t n[3] = {0, 0, 0}; // element counts of h, vn, tr
+  DBuf<Real> buf[2][3];    // [buffer][field]
+  int cur = 0;             // buffer holding the state
+  Real *ptr(int f) { return buf[cur][f].p; }
+  Real *alt(int f) {
+    if (buf[1 - cur][f].n != n[f]) buf[1 - cur][f].alloc(n[f]);
+    return buf[1 - cur][f].p;
+  }
+};
+
+struct omega_model_s {
+  omega_mesh_s *mesh = nullptr;
+  omega_params params{};
+  int terms = 0;
+  Launch lc;
+  DBuf<Real> f_v, cv2, cv4, ct2, ct4;
+  DBuf<Real> rvort_v, inv_thick_v, ke_c, div_c, trdel2_c, vdel2_e, vdel2_rv_v, vdel2_div_c;
+  DBuf<Real> partial, result;
+  omega_additional_tendency_fn add_fn = nullptr;
+  void *add_user = nullptr;
+
+  ModelCoef coef() const {
+    return ModelCoef{params.grav, f_v.p, cv2.p, cv4.p, ct2.p, ct4.p};
+  }
+  DiagOut diag() {
+    return DiagOut{rvort_v.p, inv_thick_v.p, ke_c.p, div_c.p, trdel2_c.p,
+                   vdel2_e.p, vdel2_rv_v.p, vdel2_div_c.p};
+  }
+};
+
+struct GraphKey {
+  double dt;
+  const void *state;
+  int cur;
+  bool operator<(const GraphKey &o) const {
+    if (dt != o.dt) return dt < o.dt;
+    if (state != o.state) return state < o.state;
+    return cur < o.cur;
+  }
+};
+
+struct omega_stepper_s {
+  omega_model_s *model = nullptr;
+  int kind = OMEGA_STEPPER_RK4;
+  int nstages = 4;
+  std::vector<Real> rka, rkb, rkc;
+  // fused RK4: accumulator and two provisional states; unfused RK4: tend,
+  // provisional and old states; LSRK: tend
+  DBuf<Real> acc[3], p1[3], p2[3], tend[3], provis[3], old[3];
+  bool use_graph = false;
+  std::map<GraphKey, hipGraphExec_t> graphs;
+  hipStream_t cap_stream = nullptr; // private stream the step is captured on
+  ~omega_stepper_s() {
+    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Error plumbing
+// ---------------------------------------------------------------------------
+namespace {
+thread_local std::string g_last_error;
+
+template <class F> int guarded(F &&f) {
+  try {
+    f();
+    return OMEGA_OK;
+  } catch (const Error &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return OMEGA_ERR_STATE;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return OMEGA_ERR_STATE;
+  }
+}
+
+hipStream_t as_stream(omega_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ---------------------------------------------------------------------------
+// Timing
+// ---------------------------------------------------------------------------
+struct TimingTable {
+  std::mutex mu;
+  bool enabled = false;
+  std::vector<std::string> names;
+  std::vector<double> total_ms;
+  std::vector<int64_t> count;
+  struct Pending {
+    int slot;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  int slot_of(const char *name) {
+    for (size_t i = 0; i < names.size(); ++i)
+      if (names[i] == name) return (int)i;
+    names.emplace_back(name);
+    total_ms.push_back(0);
+    count.push_back(0);
+    return (int)names.size() - 1;
+  }
+  hipEvent_t get() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    OMEGA_HIP_CHECK(hipEventCreate(&e));
+    return e;
+  }
+  void drain() {
+    for (auto &p : pending) {
+      OMEGA_HIP_CHECK(hipEventSynchronize(p.b));
+      float ms = 0;
+      OMEGA_HIP_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+      total_ms[p.slot] += ms;
+      count[p.slot] += 1;
+      pool.push_back(p.a);
+      pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+};
+TimingTable &timing() {
+  static TimingTable t;
+  return t;
+}
+
+bool stream_capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
+  return st == hipStreamCaptureStatusActive;
+}
+
+} // namespace
+
+namespace omega_hip {
+TimerScope::TimerScope(const char *name, hipStream_t st) : slot(-1), s(st) {
+  TimingTable &T = timing();
+  if (!T.enabled || stream_capturing(st)) return;
+  std::lock_guard<std::mutex> g(T.mu);
+  slot = T.slot_of(name);
+  hipEvent_t a = T.get(), b = T.get();
+  OMEGA_HIP_CHECK(hipEventRecord(a, st));
+  T.pending.push_back({slot, a, b});
+}
+TimerScope::~TimerScope() {
+  if (slot < 0) return;
+  TimingTable &T = timing();
+  std::lock_guard<std::mutex> g(T.mu);
+  for (auto it = T.pending.rbegin(); it != T.pending.rend(); ++it)
+    if (it->slot == slot) {
+      (void)hipEventRecord(it->b, s);
+      break;
+    }
+  if (T.pending.size() > 4096) {
+    try {
+      T.drain();
+    } catch (...) {
+    }
+  }
+}
+} // namespace omega_hip
+
+// ---------------------------------------------------------------------------
+// Model helpers
+// ---------------------------------------------------------------------------
+namespace {
+
+void model_build_coefficients(omega_model_s *M) {
+  const DeviceMesh &dm = M->mesh->dm;
+  const HostMesh &hm = dm.host;
+  const int NC = hm.ncells, NE = hm.nedges, NV = hm.nvertices, ME = hm.max_edges;
+  const size_t L = hm.nlayers, ntr = M->params.ntracers;
+  std::vector<Real> f(NV, M->params.f0);
+  M->f_v.upload(f);
+  std::vector<Real> cv2(NE), cv4(NE);
+  for (int e = 0; e < NE; ++e) {
+    // mask*visc*scaling with mask = 1 (vel_diff_edge.hpp:83-84,
+    // vel_hyperdiff_edge.hpp:234-235)
+    cv2[e] = M->params.visc_del2 * dm.s2_h[e];
+    cv4[e] = M->params.visc_del4 * dm.s4_h[e];
+  }
+  M->cv2.upload(cv2);
+  M->cv4.upload(cv4);
+  std::vector<Real> ct2((size_t)NC * ME, 0.0), ct4((size_t)NC * ME, 0.0);
+  for (int i = 0; i < NC; ++i) {
+    const int c = dm.cell_n2o[i];
+    const double inv_area = 1. / hm.area_cell[c];
+    for (int j = 0; j < dm.noc_h[i]; ++j) {
+      const int eo = hm.edges_on_cell[(size_t)c * ME + j];
+      const int en = dm.edge_o2n[eo];
+      const double sign = hm.edge_sign_on_cell[(size_t)c * ME + j];
+      // leading factors of tracer_diff_cell.hpp:82-84 and
+      // tracer_hyperdiff_cell.hpp:152-154
+      ct2[(size_t)i * ME + j] = M->params.eddy_diff2 * inv_area * hm.dv_edge[eo] * sign;
+      ct4[(size_t)i * ME + j] =
+          M->params.eddy_diff4 * inv_area * hm.dv_edge[eo] * sign * dm.s4_h[en];
+    }
+  }
+  M->ct2.upload(ct2);
+  M->ct4.upload(ct4);
+  M->rvort_v.alloc((size_t)NV * L);
+  M->inv_thick_v.alloc((size_t)NV * L);
+  M->ke_c.alloc((size_t)NC * L);
+  M->div_c.alloc((size_t)NC * L);
+  if (ntr > 0) M->trdel2_c.alloc(ntr * NC * L);
+  M->partial.alloc((size_t)reduce_blocks(std::max(NC, NV)) + 1);
+  M->result.alloc(4);
+}
+
+void ensure_vdel4(omega_model_s *M) {
+  const HostMesh &hm = M->mesh->dm.host;
+  const size_t L = hm.nlayers;
+  if (!M->vdel2_e.p) {
+    M->vdel2_e.alloc((size_t)hm.nedges * L);
+    M->vdel2_rv_v.alloc((size_t)hm.nvertices * L);
+    M->vdel2_div_c.alloc((size_t)hm.ncells * L);
+  }
+}
+
+// One tendency evaluation (+ optional fused stage update).
+void run_stage(omega_model_s *M, StageArgs a, hipStream_t s) {
+  const MeshView mv = M->mesh->dm.view();
+  const ModelCoef mc = M->coef();
+  if (M->terms & T_VDEL4) ensure_vdel4(M);
+  DiagOut d = M->diag();
+  a.terms = M->terms;
+  a.do_h = !M->params.disable_h_tendency;
+  a.do_vn = !M->params.disable_vn_tendency;
+  a.ntr = M->params.ntracers;
+  // diagnostics are needed by the vn tendency and by tracer del4 only
+  const bool need_diag = a.do_vn || ((M->terms & T_TDEL4) && a.ntr > 0);
+  if (need_diag) {
+    int dterms = a.do_vn ? M->terms : (M->terms & T_TDEL4);
+    launch_diag(mv, M->lc, mc, dterms, a.ntr, a.h.in, a.vn.in, a.tr.in, d, s);
+    if (a.do_vn && (M->terms & T_VDEL4)) launch_vdel4(mv, M->lc, d, s);
+  }
+  launch_tend(mv, M->lc, mc, a, d, s);
+}
+
+FieldIO fio(const Real *in, Real *tend, Real *acc, const Real *yold, Real *out) {
+  return FieldIO{in, tend, acc, yold, out};
+}
+
+void call_hook(omega_model_s *M, Real *ht, Real *vt, const Real *h, const Real *vn, double t,
+               hipStream_t s) {
+  if (!M->add_fn) return;
+  const int rc = M->add_fn(M->add_user, ht, vt, h, vn, t, s);
+  if (rc != 0) throw Error(OMEGA_ERR_CALLBACK, "additional tendency hook returned " + std::to_string(rc));
+}
+
+void check_state(omega_model_s *M, omega_state_s *S, const char *what) {
+  OMEGA_REQUIRE(S != nullptr, std::string(what) + ": null state");
+  OMEGA_REQUIRE(S->mesh == M->mesh, std::string(what) + ": state belongs to another mesh");
+  OMEGA_REQUIRE(S->ntr >= M->params.ntracers,
+                std::string(what) + ": state has fewer tracers than the model");
+}
+
+void ensure3(DBuf<Real> (&b)[3], omega_state_s *S) {
+  for (int f = 0; f < 3; ++f)
+    if (b[f].n != S->n[f]) b[f].alloc(S->n[f]);
+}
+
+// compute_tendency (shallow_water_model.cpp:313-347), writing tend buffers.
+void compute_tendency_ptrs(omega_model_s *M, Real *ht, Real *vt, Real *tt, const Real *h,
+                           const Real *vn, const Real *tr, double t, int add_mode,
+                           hipStream_t s) {
+  StageArgs a{};
+  a.epi = EPI_STORE;
+  a.base_from_tend = add_mode == OMEGA_ADD_INCREMENT;
+  a.a = 1.0;
+  a.h = fio(h, ht, nullptr, nullptr, nullptr);
+  a.vn = fio(vn, vt, nullptr, nullptr, nullptr);
+  a.tr = fio(tr, tt, nullptr, nullptr, nullptr);
+  run_stage(M, a, s);
+  call_hook(M, ht, vt, h, vn, t, s);
+}
+
+// ---------------------------------------------------------------------------
+// Steppers
+// ---------------------------------------------------------------------------
+
+// RK4 with a fused tendency + update per stage (rk4.cpp:28-104 reorganised:
+// the accumulator lives in acc until the last stage writes the state).
+void rk4_fused_step(omega_stepper_s *St, omega_state_s *S, double dt, hipStream_t s) {
+  omega_model_s *M = St->model;
+  ensure3(St->acc, S);
+  ensure3(St->p1, S);
+  ensure3(St->p2, S);
+  Real *Y[3] = {S->ptr(0), S->ptr(1), S->ptr(2)};
+  Real *A[3] = {St->acc[0].p, St->acc[1].p, St->acc[2].p};
+  Real *P1[3] = {St->p1[0].p, St->p1[1].p, St->p1[2].p};
+  Real *P2[3] = {St->p2[0].p, St->p2[1].p, St->p2[2].p};
+  for (int stage = 0; stage < 4; ++stage) {
+    StageArgs a{};
+    a.base_from_tend = 0;
+    a.db = dt * St->rkb[stage];
+    a.da = stage < 3 ? dt * St->rka[stage] : 0.0;
+    Real **in = stage == 0 ? Y : (stage == 2 ? P2 : P1);
+    Real **out = stage == 0 ? P1 : (stage == 1 ? P2 : (stage == 2 ? P1 : Y));
+    a.epi = stage == 0 ? EPI_RK4_FIRST : (stage == 3 ? EPI_RK4_LAST : EPI_RK4_MID);
+    a.h = fio(in[0], nullptr, A[0], Y[0], out[0]);
+    a.vn = fio(in[1], nullptr, A[1], Y[1], out[1]);
+    a.tr = fio(in[2], nullptr, A[2], Y[2], out[2]);
+    run_stage(M, a, s);
+  }
+}
+
+// RK4 exactly as rk4.cpp:28-104 (used when an additional-tendency hook is set).
+void rk4_unfused_step(omega_stepper_s *St, omega_state_s *S, double t, double dt, hipStream_t s) {
+  omega_model_s *M = St->model;
+  ensure3(St->tend, S);
+  ensure3(St->provis, S);
+  ensure3(St->old, S);
+  for (int f = 0; f < 3; ++f)
+    if (S->n[f])
+      OMEGA_HIP_CHECK(hipMemcpyAsync(St->old[f].p, S->ptr(f), S->n[f] * sizeof(Real),
+                                     hipMemcpyDeviceToDevice, s));
+  compute_tendency_ptrs(M, St->tend[0].p, St->tend[1].p, St->tend[2].p, S->ptr(0), S->ptr(1),
+                        S->ptr(2), t, OMEGA_ADD_REPLACE, s);
+  for (int stage = 0; stage < 4; ++stage) {
+    const double db = dt * St->rkb[stage];
+    for (int f = 0; f < 3; ++f) launch_axpy(S->ptr(f), db, St->tend[f].p, S->n[f], s);
+    if (stage < 3) {
+      const double stagetime = t + St->rkc[stage] * dt;
+      const double da = dt * St->rka[stage];
+      for (int f = 0; f < 3; ++f)
+        launch_xpay(St->provis[f].p, St->old[f].p, da, St->tend[f].p, S->n[f], s);
+      compute_tendency_ptrs(M, St->tend[0].p, St->tend[1].p, St->tend[2].p, St->provis[0].p,
+                            St->provis[1].p, St->provis[2].p, stagetime, OMEGA_ADD_REPLACE, s);
+    }
+  }
+}
+
+// Low-storage RK with the state alternating between two buffers per stage
+// (lsrk.cpp:36-89 fused: tend = a*tend + F(y); y' = y + dt*b*tend).
+void lsrk_fused_step(omega_stepper_s *St, omega_state_s *S, double dt, hipStream_t s) {
+  omega_model_s *M = St->model;
+  for (int stage = 0; stage < St->nstages; ++stage) {
+    StageArgs a{};
+    a.epi = EPI_LSRK;
+    a.a = St->rka[stage];
+    a.base_from_tend = St->rka[stage] != 0.0;
+    a.db = dt * St->rkb[stage];
+    Real *in[3] = {S->ptr(0), S->ptr(1), S->ptr(2)};
+    Real *out[3] = {S->alt(0), S->alt(1), S->alt(2)};
+    a.h = fio(in[0], St->tend[0].p, nullptr, nullptr, out[0]);
+    a.vn = fio(in[1], St->tend[1].p, nullptr, nullptr, out[1]);
+    a.tr = fio(in[2], St->tend[2].p, nullptr, nullptr, out[2]);
+    run_stage(M, a, s);
+    S->cur = 1 - S->cur;
+  }
+}
+
+// LSRK exactly as lsrk.cpp:36-89 (used when an additional-tendency hook is set).
+void lsrk_unfused_step(omega_stepper_s *St, omega_state_s *S, double t, double dt, hipStream_t s) {
+  omega_model_s *M = St->model;
+  for (int stage = 0; stage < St->nstages; ++stage) {
+    for (int f = 0; f < 3; ++f) launch_scale(St->tend[f].p, St->rka[stage], S->n[f], s);
+    const double stagetime = t + St->rkc[stage] * dt;
+    compute_tendency_ptrs(M, St->tend[0].p, St->tend[1].p, St->tend[2].p, S->ptr(0), S->ptr(1),
+                          S->ptr(2), stagetime, OMEGA_ADD_INCREMENT, s);
+    const double db = dt * St->rkb[stage];
+    for (int f = 0; f < 3; ++f) launch_axpy(S->ptr(f), db, St->tend[f].p, S->n[f], s);
+  }
+}
+
+void step_once(omega_stepper_s *St, omega_state_s *S, double t, double dt, hipStream_t s) {
+  const bool fused = St->model->add_fn == nullptr;
+  if (St->kind == OMEGA_STEPPER_RK4) {
+    if (fused) rk4_fused_step(St, S, dt, s);
+    else rk4_unfused_step(St, S, t, dt, s);
+  } else {
+    if (fused) lsrk_fused_step(St, S, dt, s);
+    else lsrk_unfused_step(St, S, t, dt, s);
+  }
+}
+
+void do_step_maybe_graph(omega_stepper_s *St, omega_state_s *S, double t, double dt,
+                         hipStream_t s) {
+  const bool graphable = St->use_graph && St->model->add_fn == nullptr && !stream_capturing(s);
+  if (!graphable) {
+    step_once(St, S, t, dt, s);
+    return;
+  }
+  if (St->kind == OMEGA_STEPPER_LSRK) {
+    // make sure both buffers exist before capture
+    for (int f = 0; f < 3; ++f) (void)S->alt(f);
+  }
+  const GraphKey key{dt, S, S->cur};
+  auto it = St->graphs.find(key);
+  if (it == St->graphs.end()) {
+    // every buffer the step touches is allocated before the capture starts
+    if (St->kind == OMEGA_STEPPER_RK4) {
+      ensure3(St->acc, S);
+      ensure3(St->p1, S);
+      ensure3(St->p2, S);
+    }
+    if (St->model->terms & T_VDEL4) ensure_vdel4(St->model);
+    // capture on a private stream (the null stream cannot be captured); the
+    // graph is then launched on the caller's stream
+    if (!St->cap_stream)
+      OMEGA_HIP_CHECK(hipStreamCreateWithFlags(&St->cap_stream, hipStreamNonBlocking));
+    hipStream_t cs = St->cap_stream;
+    const int cur0 = S->cur;
+    hipGraph_t g;
+    OMEGA_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    try {
+      step_once(St, S, t, dt, cs);
+    } catch (...) {
+      hipGraph_t dummy;
+      (void)hipStreamEndCapture(cs, &dummy);
+      S->cur = cur0;
+      throw;
+    }
+    OMEGA_HIP_CHECK(hipStreamEndCapture(cs, &g));
+    hipGraphExec_t ex;
+    OMEGA_HIP_CHECK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+    OMEGA_HIP_CHECK(hipGraphDestroy(g));
+    S->cur = cur0; // capture recorded the flips; replay applies them below
+    it = St->graphs.emplace(key, ex).first;
+  }
+  OMEGA_HIP_CHECK(hipGraphLaunch(it->second, s));
+  if (St->kind == OMEGA_STEPPER_LSRK && (St->nstages % 2 == 1)) S->cur = 1 - S->cur;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *omega_last_error(void) { return g_last_error.c_str(); }
+const char *omega_version(void) { return "omega-mi355x 0.1 (gfx950, fp64)"; }
+
+int omega_mesh_create_planar_hex(int nx, int ny, double dc, int nlayers, int ordering,
+                                 omega_mesh_t *out) {
+  return guarded([&] {
+    OMEGA_REQUIRE(out, "out is null");
+    auto m = std::make_unique<omega_mesh_s>();
+    build_planar_hex(m->dm.host, nx, ny, dc, nlayers);
+    m->dm.build_device(ordering);
+    *out = m.release();
+  });
+}
+
+int omega_mesh_create_mpas(const omega_mpas_mesh_desc *d, int ordering, omega_mesh_t *out) {
+  return guarded([&] {
+    OMEGA_REQUIRE(d && out, "null argument");
+    OMEGA_REQUIRE(d->ncells > 0 && d->nedges > 0 && d->nvertices > 0 && d->nlayers > 0,
+                  "mesh sizes must be positive");
+    OMEGA_REQUIRE(d->max_edges >= 3 && d->max_edges2 >= 1, "bad row lengths");
+    OMEGA_REQUIRE(d->nedges_on_cell && d->edges_on_cell && d->vertices_on_cell &&
+                      d->nedges_on_edge && d->edges_on_edge && d->cells_on_edge &&
+                      d->vertices_on_edge && d->edges_on_vertex && d->cells_on_vertex &&
+                      d->area_cell && d->dc_edge && d->dv_edge && d->weights_on_edge &&
+                      d->area_triangle && d->kiteareas_on_vertex,
+                  "required mesh array missing");
+    auto m = std::make_unique<omega_mesh_s>();
+    HostMesh &h = m->dm.host;
+    h.ncells = d->ncells;
+    h.nedges = d->nedges;
+    h.nvertices = d->nvertices;
+    h.nlayers = d->nlayers;
+    h.max_edges = d->max_edges;
+    h.max_edges2 = d->max_edges2;
+    h.resize();
+    const size_t NC = h.ncells, NE = h.nedges, NV = h.nvertices;
+    auto cpi = [](std::vector<int> &dst, const int *src, size_t n) { dst.assign(src, src + n); };
+    auto cpd = [](std::vector<double> &dst, const double *src, size_t n) {
+      if (src) dst.assign(src, src + n);
+    };
+    cpi(h.nedges_on_cell, d->nedges_on_cell, NC);
+    cpi(h.edges_on_cell, d->edges_on_cell, NC * h.max_edges);
+    cpi(h.vertices_on_cell, d->vertices_on_cell, NC * h.max_edges);
+    cpi(h.nedges_on_edge, d->nedges_on_edge, NE);
+    cpi(h.edges_on_edge, d->edges_on_edge, NE * h.max_edges2);
+    cpi(h.cells_on_edge, d->cells_on_edge, NE * 2);
+    cpi(h.vertices_on_edge, d->vertices_on_edge, NE * 2);
+    cpi(h.edges_on_vertex, d->edges_on_vertex, NV * 3);
+    cpi(h.cells_on_vertex, d->cells_on_vertex, NV * 3);
+    for (size_t c = 0; c < NC; ++c) {
+      OMEGA_REQUIRE(h.nedges_on_cell[c] >= 1 && h.nedges_on_cell[c] <= h.max_edges,
+                    "nedges_on_cell out of range");
+    }
+    for (size_t e = 0; e < NE; ++e) {
+      OMEGA_REQUIRE(h.nedges_on_edge[e] >= 0 && h.nedges_on_edge[e] <= h.max_edges2,
+                    "nedges_on_edge out of range");
+      for (int s = 0; s < 2; ++s) {
+        OMEGA_REQUIRE(h.cells_on_edge[2 * e + s] >= 0 && h.cells_on_edge[2 * e + s] < (int)NC,
+                      "cells_on_edge out of range");
+        OMEGA_REQUIRE(h.vertices_on_edge[2 * e + s] >= 0 &&
+                          h.vertices_on_edge[2 * e + s] < (int)NV,
+                      "vertices_on_edge out of range");
+      }
+      for (int j = 0; j < h.nedges_on_edge[e]; ++j)
+        OMEGA_REQUIRE(h.edges_on_edge[e * h.max_edges2 + j] >= 0 &&
+                          h.edges_on_edge[e * h.max_edges2 + j] < (int)NE,
+                      "edges_on_edge out of range");
+    }
+    for (size_t v = 0; v < NV * 3; ++v) {
+      OMEGA_REQUIRE(h.edges_on_vertex[v] >= 0 && h.edges_on_vertex[v] < (int)NE,
+                    "edges_on_vertex out of range");
+      OMEGA_REQUIRE(h.cells_on_vertex[v] >= 0 && h.cells_on_vertex[v] < (int)NC,
+                    "cells_on_vertex out of range");
+    }
+    cpd(h.area_cell, d->area_cell, NC);
+    cpd(h.dc_edge, d->dc_edge, NE);
+    cpd(h.dv_edge, d->dv_edge, NE);
+    cpd(h.weights_on_edge, d->weights_on_edge, NE * h.max_edges2);
+    cpd(h.area_triangle, d->area_triangle, NV);
+    cpd(h.kiteareas_on_vertex, d->kiteareas_on_vertex, NV * 3);
+    std::fill(h.mesh_scaling_del2.begin(), h.mesh_scaling_del2.end(), 1.0);
+    std::fill(h.mesh_scaling_del4.begin(), h.mesh_scaling_del4.end(), 1.0);
+    cpd(h.mesh_scaling_del2, d->mesh_scaling_del2, NE);
+    cpd(h.mesh_scaling_del4, d->mesh_scaling_del4, NE);
+    std::fill(h.max_level_edge_top.begin(), h.max_level_edge_top.end(), h.nlayers);
+    if (d->max_level_edge_top) cpi(h.max_level_edge_top, d->max_level_edge_top, NE);
+    cpd(h.x_cell, d->x_cell, NC);
+    cpd(h.y_cell, d->y_cell, NC);
+    cpd(h.x_edge, d->x_edge, NE);
+    cpd(h.y_edge, d->y_edge, NE);
+    cpd(h.angle_edge, d->angle_edge, NE);
+    cpd(h.x_vertex, d->x_vertex, NV);
+    cpd(h.y_vertex, d->y_vertex, NV);
+    h.derive_signs_and_levels();
+    m->dm.build_device(ordering);
+    *out = m.release();
+  });
+}
+
+int omega_mesh_destroy(omega_mesh_t mesh) {
+  return guarded([&] { delete mesh; });
+}
+
+int omega_mesh_info(omega_mesh_t mesh, int *counts, double *reals) {
+  return guarded([&] {
+    OMEGA_REQUIRE(mesh, "null mesh");
+    const HostMesh &h = mesh->dm.host;
+    if (counts) {
+      counts[0] = h.ncells; counts[1] = h.nedges; counts[2] = h.nvertices;
+      counts[3] = h.nlayers; counts[4] = h.nx; counts[5] = h.ny;
+    }
+    if (reals) {
+      reals[0] = h.dc; reals[1] = h.period_x; reals[2] = h.period_y;
+    }
+  });
+}
+
+int omega_mesh_host_array(omega_mesh_t mesh, const char *name, const void **ptr, int *dtype,
+                          int64_t *n0, int64_t *n1) {
+  return guarded([&] {
+    OMEGA_REQUIRE(mesh && name && ptr && dtype && n0 && n1, "null argument");
+    HostMesh &h = mesh->dm.host;
+    const std::string nm(name);
+    const int64_t NC = h.ncells, NE = h.nedges, NV = h.nvertices, ME = h.max_edges,
+                  ME2 = h.max_edges2;
+    bool found = false;
+    auto setI = [&](const char *nmx, std::vector<int> &v, int64_t a, int64_t b) {
+      if (!found && nm == nmx) { *ptr = v.data(); *dtype = 0; *n0 = a; *n1 = b; found = true; }
+    };
+    auto setD = [&](const char *nmx, std::vector<double> &v, int64_t a, int64_t b) {
+      if (!found && nm == nmx) { *ptr = v.data(); *dtype = 1; *n0 = a; *n1 = b; found = true; }
+    };
+    setI("nedges_on_cell", h.nedges_on_cell, NC, 1);
+    setI("edges_on_cell", h.edges_on_cell, NC, ME);
+    setI("cells_on_cell", h.cells_on_cell, NC, ME);
+    setI("vertices_on_cell", h.vertices_on_cell, NC, ME);
+    setI("kite_index_on_cell", h.kite_index_on_cell, NC, ME);
+    setI("max_level_cell", h.max_level_cell, NC, 1);
+    setD("area_cell", h.area_cell, NC, 1);
+    setD("lat_cell", h.lat_cell, NC, 1);
+    setD("lon_cell", h.lon_cell, NC, 1);
+    setD("x_cell", h.x_cell, NC, 1);
+    setD("y_cell", h.y_cell, NC, 1);
+    setD("z_cell", h.z_cell, NC, 1);
+    setD("mesh_density", h.mesh_density, NC, 1);
+    setD("edge_sign_on_cell", h.edge_sign_on_cell, NC, ME);
+    setI("nedges_on_edge", h.nedges_on_edge, NE, 1);
+    setI("edges_on_edge", h.edges_on_edge, NE, ME2);
+    setI("cells_on_edge", h.cells_on_edge, NE, 2);
+    setI("vertices_on_edge", h.vertices_on_edge, NE, 2);
+    setI("max_level_edge_bot", h.max_level_edge_bot, NE, 1);
+    setI("max_level_edge_top", h.max_level_edge_top, NE, 1);
+    setD("dc_edge", h.dc_edge, NE, 1);
+    setD("dv_edge", h.dv_edge, NE, 1);
+    setD("angle_edge", h.angle_edge, NE, 1);
+    setD("lat_edge", h.lat_edge, NE, 1);
+    setD("lon_edge", h.lon_edge, NE, 1);
+    setD("x_edge", h.x_edge, NE, 1);
+    setD("y_edge", h.y_edge, NE, 1);
+    setD("z_edge", h.z_edge, NE, 1);
+    setD("weights_on_edge", h.weights_on_edge, NE, ME2);
+    setD("mesh_scaling_del2", h.mesh_scaling_del2, NE, 1);
+    setD("mesh_scaling_del4", h.mesh_scaling_del4, NE, 1);
+    setI("edges_on_vertex", h.edges_on_vertex, NV, 3);
+    setI("cells_on_vertex", h.cells_on_vertex, NV, 3);
+    setI("max_level_vertex_bot", h.max_level_vertex_bot, NV, 1);
+    setI("max_level_vertex_top", h.max_level_vertex_top, NV, 1);
+    setD("area_triangle", h.area_triangle, NV, 1);
+    setD("lat_vertex", h.lat_vertex, NV, 1);
+    setD("lon_vertex", h.lon_vertex, NV, 1);
+    setD("x_vertex", h.x_vertex, NV, 1);
+    setD("y_vertex", h.y_vertex, NV, 1);
+    setD("z_vertex", h.z_vertex, NV, 1);
+    setD("kiteareas_on_vertex", h.kiteareas_on_vertex, NV, 3);
+    setD("edge_sign_on_vertex", h.edge_sign_on_vertex, NV, 3);
+    if (!found) throw Error(OMEGA_ERR_NOTFOUND, "unknown mesh array: " + nm);
+  });
+}
+
+int omega_mesh_device_array(omega_mesh_t mesh, const char *name, const double **dptr,
+                            int64_t *n) {
+  return guarded([&] {
+    OMEGA_REQUIRE(mesh && name && dptr && n, "null argument");
+    DeviceMesh &dm = mesh->dm;
+    const std::string nm(name);
+    DBuf<Real> *b = nullptr;
+    if (nm == "x_cell") b = &dm.x_cell;
+    else if (nm == "y_cell") b = &dm.y_cell;
+    else if (nm == "x_edge") b = &dm.x_edge;
+    else if (nm == "y_edge") b = &dm.y_edge;
+    else if (nm == "angle_edge") b = &dm.angle_edge;
+    else if (nm == "x_vertex") b = &dm.x_vertex;
+    else if (nm == "y_vertex") b = &dm.y_vertex;
+    else if (nm == "area_cell") b = &dm.area_cell;
+    if (!b) throw Error(OMEGA_ERR_NOTFOUND, "unknown device array: " + nm);
+    *dptr = b->p;
+    *n = (int64_t)b->n;
+  });
+}
+
+int omega_mesh_permutation(omega_mesh_t mesh, int kind, int *new2old) {
+  return guarded([&] {
+    OMEGA_REQUIRE(mesh && new2old, "null argument");
+    const DeviceMesh &dm = mesh->dm;
+    const std::vector<int> *p = kind == 0 ? &dm.cell_n2o : kind == 1 ? &dm.edge_n2o
+                                : kind == 2 ? &dm.vert_n2o : nullptr;
+    OMEGA_REQUIRE(p, "kind must be 0, 1 or 2");
+    std::copy(p->begin(), p->end(), new2old);
+  });
+}
+
+// --------------------------------------------------------------------------
+int omega_state_create(omega_mesh_t mesh, int ntracers, omega_state_t *out) {
+  return guarded([&] {
+    OMEGA_REQUIRE(mesh && out, "null argument");
+    OMEGA_REQUIRE(ntracers >= 0, "ntracers must be >= 0");
+    mesh->dm.upload();
+    auto s = std::make_unique<omega_state_s>();
+    const HostMesh &h = mesh->dm.host;
+    s->mesh = mesh;
+    s->ntr = ntracers;
+    s->n[0] = (size_t)h.ncells * h.nlayers;
+    s->n[1] = (size_t)h.nedges * h.nlayers;
+    s->n[2] = (size_t)ntracers * h.ncells * h.nlayers;
+    for (int f = 0; f < 3; ++f) s->buf[0][f].alloc(s->n[f]);
+    *out = s.release();
+  });
+}
+
+int omega_state_destroy(omega_state_t state) {
+  return guarded([&] { delete state; });
+}
+
+namespace {
+// host (original numbering) <-> device (device numbering) row permutation
+void permute_rows(const double *src, double *dst, const std::vector<int> &n2o, size_t L,
+                  bool to_device) {
+  const size_t n = n2o.size();
+  for (size_t i = 0; i < n; ++i) {
+    const size_t o = (size_t)n2o[i];
+    if (to_device) std::memcpy(dst + i * L, src + o * L, L * sizeof(double));
+    else std::memcpy(dst + o * L, src + i * L, L * sizeof(double));
+  }
+}
+} // namespace
+
+int omega_state_upload(omega_state_t S, const double *h, const double *vn, const double *tr,
+                       omega_stream_t stream) {
+  return guarded([&] {
+    OMEGA_REQUIRE(S, "null state");
+    const DeviceMesh &dm = S->mesh->dm;
+    const size_t L = dm.host.nlayers;
+    hipStream_t s = as_stream(stream);
+    std::vector<double> tmp;
+    auto put = [&](int f, const double *src, const std::vector<int> &n2o, int reps) {
+      if (!src || !S->n[f]) return;
+      tmp.resize(S->n[f]);
+      const size_t per = n2o.size() * L;
+      for (int r = 0; r < reps; ++r) permute_rows(src + r * per, tmp.data() + r * per, n2o, L, true);
+      OMEGA_HIP_CHECK(hipMemcpyAsync(S->ptr(f), tmp.data(), S->n[f] * sizeof(double),
+                                     hipMemcpyHostToDevice, s));
+      OMEGA_HIP_CHECK(hipStreamSynchronize(s));
+    };
+    put(0, h, dm.cell_n2o, 1);
+    put(1, vn, dm.edge_n2o, 1);
+    put(2, tr, dm.cell_n2o, S->ntr);
+  });
+}
+
+int omega_state_download(omega_state_t S, double *h, double *vn, double *tr,
+                         omega_stream_t stream) {
+  return guarded([&] {
+    OMEGA_REQUIRE(S, "null state");
+    const DeviceMesh &dm = S->mesh->dm;
+    const size_t L = dm.host.nlayers;
+    hipStream_t s = as_stream(stream);
+    std::vector<double> tmp;
+    auto get = [&](int f, double *dst, const std::vector<int> &n2o, int reps) {
+      if (!dst || !S->n[f]) return;
+      tmp.resize(S->n[f]);
+      OMEGA_HIP_CHECK(hipMemcpyAsync(tmp.data(), S->ptr(f), S->n[f] * sizeof(double),
+                                     hipMemcpyDeviceToHost, s));
+      OMEGA_HIP_CHECK(hipStreamSynchronize(s));
+      const size_t per = n2o.size() * L;
+      for (int r = 0; r < reps; ++r) permute_rows(tmp.data() + r * per, dst + r * per, n2o, L, false);
+    };
+    get(0, h, dm.cell_n2o, 1);
+    get(1, vn, dm.edge_n2o, 1);
+    get(2, tr, dm.cell_n2o, S->ntr);
+  });
+}
+
+int omega_state_device_ptrs(omega_state_t S, double **h, double **vn, double **tr) {
+  return guarded([&] {
+    OMEGA_REQUIRE(S, "null state");
+    if (h) *h = S->ptr(0);
+    if (vn) *vn = S->ptr(1);
+    if (tr) *tr = S->ptr(2);
+  });
+}
+
+int omega_state_copy(omega_state_t dst, omega_state_t src, omega_stream_t stream) {
+  return guarded([&] {
+    OMEGA_REQUIRE(dst && src, "null state");
+    OMEGA_REQUIRE(dst->mesh == src->mesh && dst->ntr == src->ntr, "state shapes differ");
+    for (int f = 0; f < 3; ++f)
+      if (src->n[f])
+        OMEGA_HIP_CHECK(hipMemcpyAsync(dst->ptr(f), src->ptr(f), src->n[f] * sizeof(double),
+                                       hipMemcpyDeviceToDevice, as_stream(stream)));
+  });
+}
+
+// --------------------------------------------------------------------------
+int omega_model_create(omega_mesh_t mesh, const omega_params *p, omega_model_t *out) {
+  return guarded([&] {
+    OMEGA_REQUIRE(mesh && p && out, "null argument");
+    OMEGA_REQUIRE(p->ntracers >= 0, "ntracers must be >= 0");
+    mesh->dm.upload();
+    auto M = std::make_unique<omega_model_s>();
+    M->mesh = mesh;
+    M->params = *p;
+    // shallow_water_model.cpp:17-35
+    M->terms = T_PV | T_KE | T_SSH;
+    if (p->visc_del2 > 0) M->terms |= T_VDEL2;
+    if (p->visc_del4 > 0) M->terms |= T_VDEL4;
+    if (p->eddy_diff2 > 0) M->terms |= T_TDEL2;
+    if (p->eddy_diff4 > 0) M->terms |= T_TDEL4;
+    const int L = mesh->dm.host.nlayers;
+    M->lc.lpt = (L % 2 == 0 && L >= 2) ? 2 : 1;
+    M->lc.rows = 0;
+    M->lc.swizzle = 1;
+    model_build_coefficients(M.get());
+    if (M->terms & T_VDEL4) ensure_vdel4(M.get());
+    *out = M.release();
+  });
+}
+
+int omega_model_destroy(omega_model_t model) {
+  return guarded([&] { delete model; });
+}
+
+int omega_model_set_f_vertex(omega_model_t M, const double *f) {
+  return guarded([&] {
+    OMEGA_REQUIRE(M && f, "null argument");
+    const DeviceMesh &dm = M->mesh->dm;
+    std::vector<Real> tmp(dm.host.nvertices);
+    for (int i = 0; i < dm.host.nvertices; ++i) tmp[i] = f[dm.vert_n2o[i]];
+    M->f_v.upload(tmp);
+  });
+}
+
+int omega_model_set_additional_tendency(omega_model_t M, omega_additional_tendency_fn fn,
+                                        void *user) {
+  return guarded([&] {
+    OMEGA_REQUIRE(M, "null model");
+    M->add_fn = fn;
+    M->add_user = user;
+  });
+}
+
+int omega_model_compute_tendency(omega_model_t M, omega_state_t tend, omega_state_t state,
+                                 double t, int add_mode, omega_stream_t stream) {
+  return guarded([&] {
+    OMEGA_REQUIRE(M, "null model");
+    check_state(M, state, "compute_tendency(state)");
+    check_state(M, tend, "compute_tendency(tend)");
+    OMEGA_REQUIRE(add_mode == OMEGA_ADD_REPLACE || add_mode == OMEGA_ADD_INCREMENT,
+                  "bad add_mode");
+    compute_tendency_ptrs(M, tend->ptr(0), tend->ptr(1), tend->ptr(2), state->ptr(0),
+                          state->ptr(1), state->ptr(2), t, add_mode, as_stream(stream));
+  });
+}
+
+int omega_model_set_terms(omega_model_t M, int mask) {
+  return guarded([&] {
+    OMEGA_REQUIRE(M, "null model");
+    OMEGA_REQUIRE((mask & ~127) == 0, "unknown term bits");
+    M->terms = mask;
+    if (mask & T_VDEL4) ensure_vdel4(M);
+    if ((mask & T_TDEL4) && M->params.ntracers > 0 && !M->trdel2_c.p)
+      M->trdel2_c.alloc((size_t)M->params.ntracers * M->mesh->dm.host.ncells *
+                        M->mesh->dm.host.nlayers);
+  });
+}
+
+int omega_model_get_terms(omega_model_t M, int *mask) {
+  return guarded([&] {
+    OMEGA_REQUIRE(M && mask, "null argument");
+    *mask = M->terms;
+  });
+}
+
+int omega_model_set_launch(omega_model_t M, int rows, int lpt, int swizzle) {
+  return guarded([&] {
+    OMEGA_REQUIRE(M, "null model");
+    if (rows > 0) {
+      OMEGA_REQUIRE(rows <= 256, "rows_per_block must be <= 256");
+      M->lc.rows = rows;
+    }
+    if (lpt > 0) {
+      OMEGA_REQUIRE(lpt == 1 || lpt == 2, "levels_per_thread must be 1 or 2");
+      M->lc.lpt = lpt;
+    }
+    if (swizzle >= 0) M->lc.swizzle = swizzle ? 1 : 0;
+  });
+}
+
+int omega_model_get_launch(omega_model_t M, int *rows, int *lpt, int *swizzle) {
+  return guarded([&] {
+    OMEGA_REQUIRE(M, "null model");
+    if (rows) *rows = M->lc.rows;
+    if (lpt) *lpt = M->lc.lpt;
+    if (swizzle) *swizzle = M->lc.swizzle;
+  });
+}
+
+// --------------------------------------------------------------------------
+int omega_stepper_create(omega_model_t M, int kind, int nstages, omega_stepper_t *out) {
+  return guarded([&] {
+    OMEGA_REQUIRE(M && out, "null argument");
+    OMEGA_REQUIRE(kind == OMEGA_STEPPER_RK4 || kind == OMEGA_STEPPER_LSRK, "bad stepper kind");
+    auto S = std::make_unique<omega_stepper_s>();
+    S->model = M;
+    S->kind = kind;
+    const HostMesh &h = M->mesh->dm.host;
+    const size_t L = h.nlayers, ntr = M->params.ntracers;
+    if (kind == OMEGA_STEPPER_RK4) {
+      // rk4.cpp:14-25
+      S->nstages = 4;
+      S->rka = {1. / 2, 1. / 2, 1};
+      S->rkb = {1. / 6, 1. / 3, 1. / 3, 1. / 6};
+      S->rkc = {1. / 2, 1. / 2, 1};
+    } else {
+      OMEGA_REQUIRE(nstages >= 1, "nstages must be >= 1");
+      S->nstages = nstages;
+      S->rka.assign(nstages, 0.0);
+      S->rkb.assign(nstages, 0.0);
+      S->rkc.assign(nstages, 0.0);
+      if (nstages == 5) {
+        // Carpenter-Kennedy (5,4) 2N-storage coefficients (lsrk.cpp:16-28)
+        S->rka = {0., -567301805773. / 1357537059087., -2404267990393. / 2016746695238.,
+                  -3550918686646. / 2091501179385., -1275806237668. / 842570457699.};
+        S->rkb = {1432997174477. / 9575080441755., 5161836677717. / 13612068292357.,
+                  1720146321549. / 2090206949498., 3134564353537. / 4481467310338.,
+                  2277821191437. / 14882151754819.};
+        S->rkc = {0., 1432997174477. / 9575080441755., 2526269341429. / 6820363962896.,
+                  2006345519317. / 3224310063776., 2802321613138. / 2924317926251.};
+      } else if (nstages == 1) { // forward Euler (lsrk.cpp:29-33)
+        S->rka = {0};
+        S->rkb = {1};
+        S->rkc = {0};
+      }
+      // tendency storage starts at zero (lsrk.cpp:10-14)
+      S->tend[0].alloc((size_t)h.ncells * L);
+      S->tend[1].alloc((size_t)h.nedges * L);
+      S->tend[2].alloc(ntr * h.ncells * L);
+    }
+    *out = S.release();
+  });
+}
+
+int omega_stepper_destroy(omega_stepper_t S) {
+  return guarded([&] { delete S; });
+}
+
+int omega_stepper_set_graph(omega_stepper_t S, int enable) {
+  return guarded([&] {
+    OMEGA_REQUIRE(S, "null stepper");
+    S->use_graph = enable != 0;
+  });
+}
+
+int omega_stepper_do_step(omega_stepper_t St, double t, double dt, omega_state_t S,
+                          omega_stream_t stream) {
+  return guarded([&] {
+    OMEGA_REQUIRE(St, "null stepper");
+    check_state(St->model, S, "do_step");
+    do_step_maybe_graph(St, S, t, dt, as_stream(stream));
+  });
+}
+
+int omega_stepper_run(omega_stepper_t St, double t0, double dt, int nsteps, int t_offset,
+                      omega_state_t S, omega_stream_t stream) {
+  return guarded([&] {
+    OMEGA_REQUIRE(St, "null stepper");
+    OMEGA_REQUIRE(nsteps >= 0, "nsteps must be >= 0");
+    check_state(St->model, S, "run");
+    for (int i = 0; i < nsteps; ++i)
+      do_step_maybe_graph(St, S, t0 + (i + t_offset) * dt, dt, as_stream(stream));
+  });
+}
+
+// --------------------------------------------------------------------------
+namespace {
+int diag_integral(omega_model_t M, omega_state_t S, double *out, omega_stream_t stream,
+                  int which) {
+  return guarded([&] {
+    OMEGA_REQUIRE(M && out, "null argument");
+    check_state(M, S, "integral");
+    const DeviceMesh &dm = M->mesh->dm;
+    const HostMesh &h = dm.host;
+    hipStream_t s = as_stream(stream);
+    launch_reduce(which, h.ncells, h.nvertices, h.max_edges, h.nlayers, dm.noc.p, dm.eoc.p,
+                  dm.eov.p, dm.ae_c.p, dm.area_cell.p, dm.dcs_v.p, dm.area_tri.p, M->f_v.p,
+                  M->params.grav, S->ptr(0), S->ptr(1), M->partial.p, M->result.p, s);
+    OMEGA_HIP_CHECK(hipMemcpyAsync(out, M->result.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    OMEGA_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+} // namespace
+
+int omega_energy_integral(omega_model_t M, omega_state_t S, double *out, omega_stream_t s) {
+  return diag_integral(M, S, out, s, 0);
+}
+int omega_mass_integral(omega_model_t M, omega_state_t S, double *out, omega_stream_t s) {
+  return diag_integral(M, S, out, s, 1);
+}
+int omega_circulation_integral(omega_model_t M, omega_state_t S, double *out, omega_stream_t s) {
+  return diag_integral(M, S, out, s, 2);
+}
+
+// --------------------------------------------------------------------------
+int omega_add_host_field(omega_mesh_t mesh, int kind, double *dst, const double *src, int ncols,
+                         double alpha, omega_stream_t stream) {
+  return guarded([&] {
+    OMEGA_REQUIRE(mesh && dst && src, "null argument");
+    DeviceMesh &dm = mesh->dm;
+    dm.upload();
+    const std::vector<int> *p = kind == 0 ? &dm.cell_n2o : kind == 1 ? &dm.edge_n2o
+                                : kind == 2 ? &dm.vert_n2o : nullptr;
+    OMEGA_REQUIRE(p, "kind must be 0, 1 or 2");
+    const int L = dm.host.nlayers;
+    OMEGA_REQUIRE(ncols == 1 || ncols == L, "ncols must be 1 or nlayers");
+    const size_t n = p->size();
+    std::vector<double> tmp(n * ncols);
+    permute_rows(src, tmp.data(), *p, ncols, true);
+    if (mesh->staging.n < tmp.size()) mesh->staging.alloc(tmp.size());
+    hipStream_t s = as_stream(stream);
+    OMEGA_HIP_CHECK(hipMemcpyAsync(mesh->staging.p, tmp.data(), tmp.size() * sizeof(double),
+                                   hipMemcpyHostToDevice, s));
+    launch_add_field(dst, mesh->staging.p, (int)n, L, ncols, alpha, s);
+    OMEGA_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int omega_synchronize(omega_stream_t stream) {
+  return guarded([&] {
+    if (stream) OMEGA_HIP_CHECK(hipStreamSynchronize(as_stream(stream)));
+    else OMEGA_HIP_CHECK(hipDeviceSynchronize());
+  });
+}
+
+int omega_timing_enable(int enable) {
+  return guarded([&] {
+    TimingTable &T = timing();
+    std::lock_guard<std::mutex> g(T.mu);
+    T.enabled = enable != 0;
+  });
+}
+
+int omega_timing_reset(void) {
+  return guarded([&] {
+    TimingTable &T = timing();
+    std::lock_guard<std::mutex> g(T.mu);
+    T.drain();
+    std::fill(T.total_ms.begin(), T.total_ms.end(), 0.0);
+    std::fill(T.count.begin(), T.count.end(), 0);
+  });
+}
+
+int omega_timing_read(const char *name, double *total_ms, int64_t *launches) {
+  return guarded([&] {
+    OMEGA_REQUIRE(name && total_ms && launches, "null argument");
+    TimingTable &T = timing();
+    std::lock_guard<std::mutex> g(T.mu);
+    T.drain();
+    const int slot = T.slot_of(name);
+    *total_ms = T.total_ms[slot];
+    *launches = T.count[slot];
+  });
+}
+
+} // extern "C"

@@ -1,0 +1,698 @@
+// Forward kernels of the shallow-water hot path for gfx950 (MI355X).
+//
+// Layout: every field is [entity][level] with the level fastest, so the
+// lanes of a row of threads cover consecutive levels of one entity and each
+// neighbour access is one contiguous, coalesced row.  A block owns a stretch
+// of cells along the space-filling curve (blockDim.y rows) and the edges and
+// vertices those cells own; blockDim.x lanes times N levels per lane cover a
+// chunk of levels (blockIdx.y).  Blocks are optionally renumbered so that
+// each of the 8 XCDs walks a contiguous part of the curve and its neighbour
+// re-reads stay in that XCD's L2.
+//
+// All operators are gathers: each output element is written by exactly one
+// thread, no atomics are used, and every sum runs over the same terms in the
+// same order as the reference, so results are bit-reproducible.  The library
+// is compiled with -ffp-contract=off; together with host-precomputed leading
+// factors (see mesh_host.cpp) the arithmetic is bit-identical to the
+// reference's expressions.
+#include "internal.hpp"
+
+namespace omega_hip {
+
+namespace {
+
+constexpr int kTrGroup = 4; // tracers handled per pass in the cell kernels
+
+template <int N> struct Vd {
+  double v[N];
+  __device__ double &operator[](int i) { return v[i]; }
+  __device__ double operator[](int i) const { return v[i]; }
+};
+
+template <int N> __device__ __forceinline__ Vd<N> ld(const double *__restrict__ p) {
+  Vd<N> r;
+  if constexpr (N == 2) {
+    const double2 t = *reinterpret_cast<const double2 *>(p);
+    r.v[0] = t.x;
+    r.v[1] = t.y;
+  } else {
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.v[i] = p[i];
+  }
+  return r;
+}
+
+template <int N> __device__ __forceinline__ void st(double *__restrict__ p, const Vd<N> &x) {
+  if constexpr (N == 2) {
+    *reinterpret_cast<double2 *>(p) = make_double2(x.v[0], x.v[1]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < N; ++i) p[i] = x.v[i];
+  }
+}
+
+template <int N> __device__ __forceinline__ Vd<N> zero() {
+  Vd<N> r;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.v[i] = 0.0;
+  return r;
+}
+
+// Block b of G -> position on the curve such that XCD (b mod 8) processes a
+// contiguous range (workgroups are dealt to the 8 XCDs round-robin).
+__device__ __forceinline__ int logical_block(int b, int G, int swz) {
+  if (!swz) return b;
+  const int x = b & 7, i = b >> 3;
+  const int q = G >> 3, r = G & 7;
+  return x * q + min(x, r) + i;
+}
+
+// Stage epilogue: consumes the tendency T of one (entity, level-vector).
+template <int N>
+__device__ __forceinline__ void epilogue(const StageArgs &a, const FieldIO &f, size_t idx,
+                                         const Vd<N> &T) {
+  switch (a.epi) {
+  case EPI_STORE:
+    st<N>(f.tend + idx, T);
+    break;
+  case EPI_RK4_FIRST: {
+    // reference rk4.cpp:145-162 (state += dt*b*k) and :168-189 (provis)
+    const Vd<N> y = ld<N>(f.in + idx);
+    Vd<N> acc, out;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      acc[i] = y[i] + a.db * T[i];
+      out[i] = y[i] + a.da * T[i];
+    }
+    st<N>(f.acc + idx, acc);
+    st<N>(f.out + idx, out);
+    break;
+  }
+  case EPI_RK4_MID: {
+    Vd<N> acc = ld<N>(f.acc + idx);
+    const Vd<N> yo = ld<N>(f.yold + idx);
+    Vd<N> out;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      acc[i] = acc[i] + a.db * T[i];
+      out[i] = yo[i] + a.da * T[i];
+    }
+    st<N>(f.acc + idx, acc);
+    st<N>(f.out + idx, out);
+    break;
+  }
+  case EPI_RK4_LAST: {
+    Vd<N> acc = ld<N>(f.acc + idx);
+#pragma unroll
+    for (int i = 0; i < N; ++i) acc[i] = acc[i] + a.db * T[i];
+    st<N>(f.out + idx, acc);
+    break;
+  }
+  case EPI_LSRK: {
+    // reference lsrk.cpp:70-87 (state += dt*b*tend); tend keeps T
+    st<N>(f.tend + idx, T);
+    const Vd<N> y = ld<N>(f.in + idx);
+    Vd<N> out;
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] = y[i] + a.db * T[i];
+    st<N>(f.out + idx, out);
+    break;
+  }
+  }
+}
+
+// Starting value of a tendency: 0 (replace) or a*tend (increment mode after
+// the low-storage scaling tend *= a, lsrk.cpp:47-63).
+template <int N>
+__device__ __forceinline__ Vd<N> tend_base(const StageArgs &a, const FieldIO &f, size_t idx) {
+  if (!a.base_from_tend) return zero<N>();
+  Vd<N> t = ld<N>(f.tend + idx);
+#pragma unroll
+  for (int i = 0; i < N; ++i) t[i] = t[i] * a.a;
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics: vertex vorticity / thickness, cell KE / divergence, tracer
+// Laplacian for tracer del4.
+// ---------------------------------------------------------------------------
+
+// VorticityAuxVars::compute_vort_vertex (vorticity_aux.hpp:85-116); stores
+// rvort and 1/thickness, from which nrv = inv*rvort and npv = inv*f are
+// formed where needed.
+template <int N>
+__device__ __forceinline__ void vertex_diag(const MeshView &m, int v, int k,
+                                            const double *__restrict__ h,
+                                            const double *__restrict__ vn, const DiagOut &d) {
+  const int L = m.L;
+  Vd<N> rv = zero<N>(), th = zero<N>();
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int e = m.eov[3 * v + j];
+    const int c = m.cov[3 * v + j];
+    const double crv = m.crv[3 * v + j];
+    const double cth = m.cth[3 * v + j];
+    const Vd<N> ve = ld<N>(vn + (size_t)e * L + k);
+    const Vd<N> hc = ld<N>(h + (size_t)c * L + k);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      rv[i] = rv[i] + crv * ve[i];
+      th[i] = th[i] + cth * hc[i];
+    }
+  }
+  Vd<N> inv;
+#pragma unroll
+  for (int i = 0; i < N; ++i) inv[i] = 1.0 / th[i];
+  st<N>(d.rvort_v + (size_t)v * L + k, rv);
+  st<N>(d.inv_thick_v + (size_t)v * L + k, inv);
+}
+
+// KineticAuxVars::compute_kinetic_cell (kinetic_aux.hpp:44-73) and, for
+// tracer del4, TracerHyperDiffusionOnCell::compute_tracer_del2
+// (tracer_hyperdiff_cell.hpp:94-129) with norm_tr = tr*(1/h)
+// (tracer_aux.hpp:31-51) formed on the fly.
+template <int N>
+__device__ __forceinline__ void cell_diag(const MeshView &m, int terms, int ntr, int c, int k,
+                                          const double *__restrict__ h,
+                                          const double *__restrict__ vn,
+                                          const double *__restrict__ tr, const DiagOut &d) {
+  const int L = m.L, me = m.me;
+  const int n = m.noc[c];
+  if (terms & (T_KE | T_VDEL2 | T_VDEL4)) {
+    Vd<N> ke = zero<N>(), div = zero<N>();
+    for (int j = 0; j < n; ++j) {
+      const int e = m.eoc[c * me + j];
+      const double cke = m.cke[c * me + j];
+      const double cd = m.cdiv[c * me + j];
+      const Vd<N> ve = ld<N>(vn + (size_t)e * L + k);
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        ke[i] = ke[i] + cke * ve[i] * ve[i] * 0.25;
+        div[i] = div[i] + cd * ve[i];
+      }
+    }
+    st<N>(d.ke_c + (size_t)c * L + k, ke);
+    st<N>(d.div_c + (size_t)c * L + k, div);
+  }
+  if ((terms & T_TDEL4) && ntr > 0) {
+    const Vd<N> hs = ld<N>(h + (size_t)c * L + k);
+    Vd<N> inv_s;
+#pragma unroll
+    for (int i = 0; i < N; ++i) inv_s[i] = 1.0 / hs[i];
+    for (int l0 = 0; l0 < ntr; l0 += kTrGroup) {
+      Vd<N> d2[kTrGroup];
+#pragma unroll
+      for (int g = 0; g < kTrGroup; ++g) d2[g] = zero<N>();
+      for (int j = 0; j < n; ++j) {
+        const int e = m.eoc[c * me + j];
+        const int a0 = m.coe[2 * e], a1 = m.coe[2 * e + 1];
+        const double cd = m.cdiv[c * me + j];
+        const double idc = m.inv_dc[e];
+        const Vd<N> h0 = ld<N>(h + (size_t)a0 * L + k);
+        const Vd<N> h1 = ld<N>(h + (size_t)a1 * L + k);
+        Vd<N> inv0, inv1, cm;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          inv0[i] = (a0 == c) ? inv_s[i] : 1.0 / h0[i];
+          inv1[i] = (a1 == c) ? inv_s[i] : 1.0 / h1[i];
+          cm[i] = cd * (0.5 * (h0[i] + h1[i]));
+        }
+#pragma unroll
+        for (int g = 0; g < kTrGroup; ++g) {
+          const int l = l0 + g;
+          if (l < ntr) {
+            const double *trl = tr + (size_t)l * m.nc * L;
+            const Vd<N> t0 = ld<N>(trl + (size_t)a0 * L + k);
+            const Vd<N> t1 = ld<N>(trl + (size_t)a1 * L + k);
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+              const double grad = (t1[i] * inv1[i] - t0[i] * inv0[i]) * idc;
+              d2[g][i] = d2[g][i] + cm[i] * grad;
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int g = 0; g < kTrGroup; ++g) {
+        const int l = l0 + g;
+        if (l < ntr) st<N>(d.trdel2_c + ((size_t)l * m.nc + c) * L + k, d2[g]);
+      }
+    }
+  }
+}
+
+template <int N>
+__global__ void __launch_bounds__(256)
+    k_diag(MeshView m, int terms, int ntr, const double *__restrict__ h,
+           const double *__restrict__ vn, const double *__restrict__ tr, DiagOut d, int swz) {
+  const int s = logical_block(blockIdx.x, gridDim.x, swz);
+  const int R = blockDim.y;
+  const int c0 = s * R;
+  if (c0 >= m.nc) return;
+  const int c1 = min(c0 + R, m.nc);
+  const int k = (blockIdx.y * blockDim.x + threadIdx.x) * N;
+  if (k >= m.L) return;
+  if (terms & (T_PV | T_VDEL2 | T_VDEL4)) {
+    const int vb = m.vofs[c0], ve = m.vofs[c1];
+    for (int v = vb + (int)threadIdx.y; v < ve; v += R) vertex_diag<N>(m, v, k, h, vn, d);
+  }
+  for (int c = c0 + (int)threadIdx.y; c < c1; c += R) cell_diag<N>(m, terms, ntr, c, k, h, vn, tr, d);
+}
+
+// ---------------------------------------------------------------------------
+// Momentum del4 pre-passes (vel_hyperdiff_edge.hpp:140-210)
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(256) k_vdel4_edge(MeshView m, DiagOut d, int swz) {
+  const int s = logical_block(blockIdx.x, gridDim.x, swz);
+  const int R = blockDim.y;
+  const int c0 = s * R;
+  if (c0 >= m.nc) return;
+  const int c1 = min(c0 + R, m.nc);
+  const int k = (blockIdx.y * blockDim.x + threadIdx.x) * N;
+  if (k >= m.L) return;
+  const int L = m.L;
+  const int eb = m.eofs[c0], ee = m.eofs[c1];
+  for (int e = eb + (int)threadIdx.y; e < ee; e += R) {
+    const int a0 = m.coe[2 * e], a1 = m.coe[2 * e + 1];
+    const int b0 = m.voe[2 * e], b1 = m.voe[2 * e + 1];
+    const double idc = m.inv_dc[e], idv = m.inv_dv4[e];
+    const Vd<N> d0 = ld<N>(d.div_c + (size_t)a0 * L + k), d1 = ld<N>(d.div_c + (size_t)a1 * L + k);
+    const Vd<N> r0 = ld<N>(d.rvort_v + (size_t)b0 * L + k), r1 = ld<N>(d.rvort_v + (size_t)b1 * L + k);
+    Vd<N> out;
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] = (d1[i] - d0[i]) * idc - (r1[i] - r0[i]) * idv;
+    st<N>(d.vdel2_e + (size_t)e * L + k, out);
+  }
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) k_vdel4_vc(MeshView m, DiagOut d, int swz) {
+  const int s = logical_block(blockIdx.x, gridDim.x, swz);
+  const int R = blockDim.y;
+  const int c0 = s * R;
+  if (c0 >= m.nc) return;
+  const int c1 = min(c0 + R, m.nc);
+  const int k = (blockIdx.y * blockDim.x + threadIdx.x) * N;
+  if (k >= m.L) return;
+  const int L = m.L, me = m.me;
+  const int vb = m.vofs[c0], ve = m.vofs[c1];
+  for (int v = vb + (int)threadIdx.y; v < ve; v += R) {
+    Vd<N> rv = zero<N>();
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int e = m.eov[3 * v + j];
+      const double crv = m.crv[3 * v + j];
+      const Vd<N> x = ld<N>(d.vdel2_e + (size_t)e * L + k);
+#pragma unroll
+      for (int i = 0; i < N; ++i) rv[i] = rv[i] + crv * x[i];
+    }
+    st<N>(d.vdel2_rv_v + (size_t)v * L + k, rv);
+  }
+  for (int c = c0 + (int)threadIdx.y; c < c1; c += R) {
+    Vd<N> acc = zero<N>();
+    const int n = m.noc[c];
+    for (int j = 0; j < n; ++j) {
+      const int e = m.eoc[c * me + j];
+      const double cd = m.cdiv[c * me + j];
+      const Vd<N> x = ld<N>(d.vdel2_e + (size_t)e * L + k);
+#pragma unroll
+      for (int i = 0; i < N; ++i) acc[i] = acc[i] + cd * x[i];
+    }
+    st<N>(d.vdel2_div_c + (size_t)c * L + k, acc);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tendencies + stage update
+// ---------------------------------------------------------------------------
+
+// Edge-averaged normalised relative and planetary vorticity of edge e
+// (vorticity_aux.hpp:108-131): nrv_v = inv_thick*rvort, npv_v = inv_thick*f.
+template <int N>
+__device__ __forceinline__ void edge_vort(const MeshView &m, const ModelCoef &mc, const DiagOut &d,
+                                          int e, int k, Vd<N> &nrv, Vd<N> &npv) {
+  const int L = m.L;
+  const int b0 = m.voe[2 * e], b1 = m.voe[2 * e + 1];
+  const double f0 = mc.f_v[b0], f1 = mc.f_v[b1];
+  const Vd<N> r0 = ld<N>(d.rvort_v + (size_t)b0 * L + k), r1 = ld<N>(d.rvort_v + (size_t)b1 * L + k);
+  const Vd<N> i0 = ld<N>(d.inv_thick_v + (size_t)b0 * L + k);
+  const Vd<N> i1 = ld<N>(d.inv_thick_v + (size_t)b1 * L + k);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    nrv[i] = 0.5 * (i0[i] * r0[i] + i1[i] * r1[i]);
+    npv[i] = 0.5 * (i0[i] * f0 + i1[i] * f1);
+  }
+}
+
+// ShallowWaterModel::compute_vn_tendency (shallow_water_model.cpp:186-217):
+// PV flux (pv_flux_edge.hpp:56-84), KE gradient (ke_grad_edge.hpp:37-51),
+// SSH gradient (ssh_grad_edge.hpp:37-50), del2 (vel_diff_edge.hpp:61-86),
+// del4 (vel_hyperdiff_edge.hpp:212-237), in that order.
+template <int N>
+__device__ __forceinline__ void edge_tend(const MeshView &m, const ModelCoef &mc,
+                                          const StageArgs &a, const DiagOut &d, int e, int k) {
+  const int L = m.L;
+  const size_t idx = (size_t)e * L + k;
+  if (!a.do_vn) {
+    if (a.epi == EPI_STORE) return; // tendency left untouched
+    epilogue<N>(a, a.vn, idx, tend_base<N>(a, a.vn, idx));
+    return;
+  }
+  const double *__restrict__ h = a.h.in;
+  const double *__restrict__ vn = a.vn.in;
+  Vd<N> T = tend_base<N>(a, a.vn, idx);
+  const int c0 = m.coe[2 * e], c1 = m.coe[2 * e + 1];
+  const double idc = m.inv_dc[e];
+  if (a.terms & T_PV) {
+    Vd<N> nrv_e, npv_e;
+    edge_vort<N>(m, mc, d, e, k, nrv_e, npv_e);
+    Vd<N> qs;
+#pragma unroll
+    for (int i = 0; i < N; ++i) qs[i] = nrv_e[i] + npv_e[i];
+    Vd<N> qt = zero<N>();
+    const int n = m.noe[e];
+    for (int j = 0; j < n; ++j) {
+      const int ej = m.eoe[e * m.me2 + j];
+      const double wj = m.w[e * m.me2 + j];
+      const int a0 = m.coe[2 * ej], a1 = m.coe[2 * ej + 1];
+      const Vd<N> h0 = ld<N>(h + (size_t)a0 * L + k), h1 = ld<N>(h + (size_t)a1 * L + k);
+      const Vd<N> vj = ld<N>(vn + (size_t)ej * L + k);
+      Vd<N> nrv_j, npv_j;
+      edge_vort<N>(m, mc, d, ej, k, nrv_j, npv_j);
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const double fh = 0.5 * (h0[i] + h1[i]);
+        const double nv = (qs[i] + nrv_j[i] + npv_j[i]) * 0.5;
+        qt[i] = qt[i] + wj * fh * vj[i] * nv;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) T[i] = T[i] + qt[i];
+  }
+  if (a.terms & T_KE) {
+    const Vd<N> k0 = ld<N>(d.ke_c + (size_t)c0 * L + k), k1 = ld<N>(d.ke_c + (size_t)c1 * L + k);
+#pragma unroll
+    for (int i = 0; i < N; ++i) T[i] = T[i] - (k1[i] - k0[i]) * idc;
+  }
+  if (a.terms & T_SSH) {
+    const Vd<N> h0 = ld<N>(h + (size_t)c0 * L + k), h1 = ld<N>(h + (size_t)c1 * L + k);
+#pragma unroll
+    for (int i = 0; i < N; ++i) T[i] = T[i] - mc.grav * (h1[i] - h0[i]) * idc;
+  }
+  if (a.terms & (T_VDEL2 | T_VDEL4)) {
+    const int b0 = m.voe[2 * e], b1 = m.voe[2 * e + 1];
+    const double idv = m.inv_dv[e];
+    const int ml = m.maxlev_e[e];
+    if (a.terms & T_VDEL2) {
+      const Vd<N> d0 = ld<N>(d.div_c + (size_t)c0 * L + k), d1 = ld<N>(d.div_c + (size_t)c1 * L + k);
+      const Vd<N> r0 = ld<N>(d.rvort_v + (size_t)b0 * L + k), r1 = ld<N>(d.rvort_v + (size_t)b1 * L + k);
+      const double cv = mc.cv2[e];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const double del2u = (d1[i] - d0[i]) * idc - (r1[i] - r0[i]) * idv;
+        const double coef = (k + i < ml) ? cv : 0.0;
+        T[i] = T[i] + coef * del2u;
+      }
+    }
+    if (a.terms & T_VDEL4) {
+      const Vd<N> d0 = ld<N>(d.vdel2_div_c + (size_t)c0 * L + k);
+      const Vd<N> d1 = ld<N>(d.vdel2_div_c + (size_t)c1 * L + k);
+      const Vd<N> r0 = ld<N>(d.vdel2_rv_v + (size_t)b0 * L + k);
+      const Vd<N> r1 = ld<N>(d.vdel2_rv_v + (size_t)b1 * L + k);
+      const double cv = mc.cv4[e];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const double del2u = (d1[i] - d0[i]) * idc - (r1[i] - r0[i]) * idv;
+        const double coef = (k + i < ml) ? cv : 0.0;
+        T[i] = T[i] - coef * del2u;
+      }
+    }
+  }
+  epilogue<N>(a, a.vn, idx, T);
+}
+
+// compute_h_tendency (shallow_water_model.cpp:71-84, thickness_hadv_cell.hpp:48-72)
+// and compute_tr_tendency (shallow_water_model.cpp:285-309): tracer advection
+// (tracer_hadv_cell.hpp:60-90), del2 (tracer_diff_cell.hpp:61-92) and del4
+// (tracer_hyperdiff_cell.hpp:131-161), each summed separately then added.
+template <int N>
+__device__ __forceinline__ void cell_tend(const MeshView &m, const ModelCoef &mc,
+                                          const StageArgs &a, const DiagOut &d, int c, int k) {
+  const int L = m.L, me = m.me, NC = m.nc;
+  const double *__restrict__ h = a.h.in;
+  const double *__restrict__ vn = a.vn.in;
+  const int n = m.noc[c];
+  const size_t hidx = (size_t)c * L + k;
+  if (a.do_h) {
+    Vd<N> T = tend_base<N>(a, a.h, hidx);
+    Vd<N> acc = zero<N>();
+    for (int j = 0; j < n; ++j) {
+      const int e = m.eoc[c * me + j];
+      const int a0 = m.coe[2 * e], a1 = m.coe[2 * e + 1];
+      const double cd = m.cdiv[c * me + j];
+      const Vd<N> h0 = ld<N>(h + (size_t)a0 * L + k), h1 = ld<N>(h + (size_t)a1 * L + k);
+      const Vd<N> ve = ld<N>(vn + (size_t)e * L + k);
+#pragma unroll
+      for (int i = 0; i < N; ++i) acc[i] = acc[i] - cd * (0.5 * (h0[i] + h1[i])) * ve[i];
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) T[i] = T[i] + acc[i];
+    epilogue<N>(a, a.h, hidx, T);
+  } else if (a.epi != EPI_STORE) {
+    epilogue<N>(a, a.h, hidx, tend_base<N>(a, a.h, hidx));
+  }
+
+  const int ntr = a.ntr;
+  if (ntr <= 0) return;
+  const bool td2 = a.terms & T_TDEL2, td4 = a.terms & T_TDEL4;
+  const double *__restrict__ tr = a.tr.in;
+  const Vd<N> hs = ld<N>(h + hidx);
+  Vd<N> inv_s;
+#pragma unroll
+  for (int i = 0; i < N; ++i) inv_s[i] = 1.0 / hs[i];
+  for (int l0 = 0; l0 < ntr; l0 += kTrGroup) {
+    Vd<N> adv[kTrGroup], dif[kTrGroup], hyp[kTrGroup];
+#pragma unroll
+    for (int g = 0; g < kTrGroup; ++g) {
+      adv[g] = zero<N>();
+      dif[g] = zero<N>();
+      hyp[g] = zero<N>();
+    }
+    for (int j = 0; j < n; ++j) {
+      const int e = m.eoc[c * me + j];
+      const int a0 = m.coe[2 * e], a1 = m.coe[2 * e + 1];
+      const double cd = m.cdiv[c * me + j];
+      const double idc = m.inv_dc[e];
+      const Vd<N> h0 = ld<N>(h + (size_t)a0 * L + k), h1 = ld<N>(h + (size_t)a1 * L + k);
+      const Vd<N> ve = ld<N>(vn + (size_t)e * L + k);
+      Vd<N> inv0, inv1, fh;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        inv0[i] = (a0 == c) ? inv_s[i] : 1.0 / h0[i];
+        inv1[i] = (a1 == c) ? inv_s[i] : 1.0 / h1[i];
+        fh[i] = 0.5 * (h0[i] + h1[i]);
+      }
+      const double c2 = td2 ? mc.ct2[c * me + j] : 0.0;
+      const double sc2 = td2 ? m.s2[e] : 0.0;
+      const double c4 = td4 ? mc.ct4[c * me + j] : 0.0;
+#pragma unroll
+      for (int g = 0; g < kTrGroup; ++g) {
+        const int l = l0 + g;
+        if (l < ntr) {
+          const double *trl = tr + (size_t)l * NC * L;
+          const Vd<N> t0 = ld<N>(trl + (size_t)a0 * L + k), t1 = ld<N>(trl + (size_t)a1 * L + k);
+          Vd<N> n0, n1;
+#pragma unroll
+          for (int i = 0; i < N; ++i) {
+            n0[i] = t0[i] * inv0[i];
+            n1[i] = t1[i] * inv1[i];
+            const double ntr_e = (n0[i] + n1[i]) * 0.5;
+            adv[g][i] = adv[g][i] - cd * fh[i] * ntr_e * ve[i];
+          }
+          if (td2) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+              const double grad = (n1[i] - n0[i]) * idc;
+              dif[g][i] = dif[g][i] + c2 * fh[i] * sc2 * grad;
+            }
+          }
+          if (td4) {
+            const double *dl = d.trdel2_c + (size_t)l * NC * L;
+            const Vd<N> q0 = ld<N>(dl + (size_t)a0 * L + k), q1 = ld<N>(dl + (size_t)a1 * L + k);
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+              const double grad = (q1[i] - q0[i]) * idc;
+              hyp[g][i] = hyp[g][i] - c4 * grad;
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < kTrGroup; ++g) {
+      const int l = l0 + g;
+      if (l < ntr) {
+        const size_t tidx = ((size_t)l * NC + c) * L + k;
+        Vd<N> T = tend_base<N>(a, a.tr, tidx);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          T[i] = T[i] + adv[g][i];
+          if (td2) T[i] = T[i] + dif[g][i];
+          if (td4) T[i] = T[i] + hyp[g][i];
+        }
+        epilogue<N>(a, a.tr, tidx, T);
+      }
+    }
+  }
+}
+
+template <int N>
+__global__ void __launch_bounds__(256)
+    k_tend(MeshView m, ModelCoef mc, StageArgs a, DiagOut d, int swz) {
+  const int s = logical_block(blockIdx.x, gridDim.x, swz);
+  const int R = blockDim.y;
+  const int c0 = s * R;
+  if (c0 >= m.nc) return;
+  const int c1 = min(c0 + R, m.nc);
+  const int k = (blockIdx.y * blockDim.x + threadIdx.x) * N;
+  if (k >= m.L) return;
+  const int eb = m.eofs[c0], ee = m.eofs[c1];
+  for (int e = eb + (int)threadIdx.y; e < ee; e += R) edge_tend<N>(m, mc, a, d, e, k);
+  for (int c = c0 + (int)threadIdx.y; c < c1; c += R) cell_tend<N>(m, mc, a, d, c, k);
+}
+
+// ---------------------------------------------------------------------------
+// Element-wise helpers
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_axpy(double *__restrict__ y, double alpha,
+                                              const double *__restrict__ x, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    y[i] = y[i] + alpha * x[i];
+}
+
+__global__ void __launch_bounds__(256) k_xpay(double *__restrict__ y, const double *__restrict__ x,
+                                              double alpha, const double *__restrict__ z,
+                                              size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    y[i] = x[i] + alpha * z[i];
+}
+
+__global__ void __launch_bounds__(256) k_scale(double *__restrict__ y, double alpha, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    y[i] = y[i] * alpha;
+}
+
+__global__ void __launch_bounds__(256) k_add_field(double *__restrict__ dst,
+                                                   const double *__restrict__ src, int n, int L,
+                                                   int ncols, double alpha) {
+  const size_t total = (size_t)n * L;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t e = i / L, k = i % L;
+    const double v = ncols == 1 ? src[e] : src[e * L + k];
+    dst[i] = dst[i] + alpha * v;
+  }
+}
+
+int grid_for(size_t n) {
+  size_t b = (n + 255) / 256;
+  if (b > 65536) b = 65536;
+  return (int)(b ? b : 1);
+}
+
+struct Geo {
+  dim3 grid, block;
+};
+
+Geo geometry(const MeshView &m, const Launch &lc, int N) {
+  // lanes per row: enough to cover L levels with N levels per lane, capped
+  // at 64 (one wavefront); further levels go to blockIdx.y
+  int lanes = (m.L + N - 1) / N;
+  if (lanes > 64) lanes = 64;
+  int lx = 1;
+  while (lx < lanes) lx <<= 1;
+  // rows = 0: fill a 256-thread block
+  int rows = lc.rows > 0 ? lc.rows : 256 / lx;
+  if (lx * rows > 256) rows = 256 / lx;
+  if (rows < 1) rows = 1;
+  const int chunks = (m.L + lx * N - 1) / (lx * N);
+  const int blocks = (m.nc + rows - 1) / rows;
+  return Geo{dim3(blocks, chunks), dim3(lx, rows)};
+}
+
+} // namespace
+
+void launch_diag(const MeshView &m, const Launch &lc, const ModelCoef &mc, int terms, int ntr,
+                 const Real *h, const Real *vn, const Real *tr, const DiagOut &d, hipStream_t s) {
+  (void)mc;
+  TimerScope ts("diag", s);
+  const int N = (lc.lpt == 2 && m.L % 2 == 0) ? 2 : 1;
+  const Geo g = geometry(m, lc, N);
+  if (N == 2)
+    hipLaunchKernelGGL(k_diag<2>, g.grid, g.block, 0, s, m, terms, ntr, h, vn, tr, d, lc.swizzle);
+  else
+    hipLaunchKernelGGL(k_diag<1>, g.grid, g.block, 0, s, m, terms, ntr, h, vn, tr, d, lc.swizzle);
+  OMEGA_HIP_CHECK(hipGetLastError());
+}
+
+void launch_vdel4(const MeshView &m, const Launch &lc, const DiagOut &d, hipStream_t s) {
+  TimerScope ts("vdel4", s);
+  const int N = (lc.lpt == 2 && m.L % 2 == 0) ? 2 : 1;
+  const Geo g = geometry(m, lc, N);
+  if (N == 2) {
+    hipLaunchKernelGGL(k_vdel4_edge<2>, g.grid, g.block, 0, s, m, d, lc.swizzle);
+    hipLaunchKernelGGL(k_vdel4_vc<2>, g.grid, g.block, 0, s, m, d, lc.swizzle);
+  } else {
+    hipLaunchKernelGGL(k_vdel4_edge<1>, g.grid, g.block, 0, s, m, d, lc.swizzle);
+    hipLaunchKernelGGL(k_vdel4_vc<1>, g.grid, g.block, 0, s, m, d, lc.swizzle);
+  }
+  OMEGA_HIP_CHECK(hipGetLastError());
+}
+
+void launch_tend(const MeshView &m, const Launch &lc, const ModelCoef &mc, const StageArgs &a,
+                 const DiagOut &d, hipStream_t s) {
+  TimerScope ts("tend", s);
+  const int N = (lc.lpt == 2 && m.L % 2 == 0) ? 2 : 1;
+  const Geo g = geometry(m, lc, N);
+  if (N == 2)
+    hipLaunchKernelGGL(k_tend<2>, g.grid, g.block, 0, s, m, mc, a, d, lc.swizzle);
+  else
+    hipLaunchKernelGGL(k_tend<1>, g.grid, g.block, 0, s, m, mc, a, d, lc.swizzle);
+  OMEGA_HIP_CHECK(hipGetLastError());
+}
+
+void launch_axpy(Real *y, Real alpha, const Real *x, size_t n, hipStream_t s) {
+  if (!n) return;
+  TimerScope ts("update", s);
+  hipLaunchKernelGGL(k_axpy, dim3(grid_for(n)), dim3(256), 0, s, y, alpha, x, n);
+  OMEGA_HIP_CHECK(hipGetLastError());
+}
+
+void launch_xpay(Real *y, const Real *x, Real alpha, const Real *z, size_t n, hipStream_t s) {
+  if (!n) return;
+  TimerScope ts("update", s);
+  hipLaunchKernelGGL(k_xpay, dim3(grid_for(n)), dim3(256), 0, s, y, x, alpha, z, n);
+  OMEGA_HIP_CHECK(hipGetLastError());
+}
+
+void launch_scale(Real *y, Real alpha, size_t n, hipStream_t s) {
+  if (!n) return;
+  TimerScope ts("update", s);
+  hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(256), 0, s, y, alpha, n);
+  OMEGA_HIP_CHECK(hipGetLastError());
+}
+
+void launch_add_field(Real *dst, const Real *src, int n, int L, int ncols, Real alpha,
+                      hipStream_t s) {
+  const size_t total = (size_t)n * L;
+  if (!total) return;
+  hipLaunchKernelGGL(k_add_field, dim3(grid_for(total)), dim3(256), 0, s, dst, src, n, L, ncols,
+                     alpha);
+  OMEGA_HIP_CHECK(hipGetLastError());
+}
+
+} // namespace omega_hip

@@ -1,0 +1,261 @@
+"""ctypes binding of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Loaded only by tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+bench.py, always as the checker.  The product path (omega_playground_amd)
+never imports this module.
+
+The oracle restates the reference's arithmetic (see swe_oracle.cpp for the
+file:line citations).  State arrays are caller-owned numpy float64 arrays in
+the reference's original entity numbering and LayoutRight shape:
+h[ncells, L], vn[nedges, L], tr[ntracers, ncells, L].
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle_swe.so")
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (g++)."""
+    if force and os.path.exists(_LIB_PATH):
+        os.remove(_LIB_PATH)
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+class Params(C.Structure):
+    # src/model/shallow_water_params.hpp:7-18 (same field order)
+    _fields_ = [
+        ("f0", C.c_double),
+        ("grav", C.c_double),
+        ("drag_coeff", C.c_double),
+        ("visc_del2", C.c_double),
+        ("visc_del4", C.c_double),
+        ("eddy_diff2", C.c_double),
+        ("eddy_diff4", C.c_double),
+        ("ntracers", C.c_int),
+        ("disable_h_tendency", C.c_int),
+        ("disable_vn_tendency", C.c_int),
+    ]
+
+
+ADDFN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                    C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+        lp = C.POINTER(C.c_long)
+        L.orc_mesh_planar_hex.restype = vp
+        L.orc_mesh_planar_hex.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int]
+        L.orc_mesh_free.argtypes = [vp]
+        L.orc_mesh_info.argtypes = [vp, ip, dp]
+        L.orc_mesh_array.restype = vp
+        L.orc_mesh_array.argtypes = [vp, C.c_char_p, ip, lp, lp]
+        L.orc_mesh_finalize.argtypes = [vp]
+        L.orc_model_create.restype = vp
+        L.orc_model_create.argtypes = [vp, C.POINTER(Params)]
+        L.orc_model_free.argtypes = [vp]
+        L.orc_model_array.restype = vp
+        L.orc_model_array.argtypes = [vp, C.c_char_p, ip, lp, lp]
+        L.orc_model_set_additional.argtypes = [vp, ADDFN, vp]
+        L.orc_model_set_terms.argtypes = [vp, C.c_int]
+        L.orc_model_get_terms.argtypes = [vp]
+        L.orc_model_get_terms.restype = C.c_int
+        L.orc_aux_compute.argtypes = [vp, dp, dp, dp]
+        L.orc_compute_tendency.argtypes = [vp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_int]
+        L.orc_stepper_create.restype = vp
+        L.orc_stepper_create.argtypes = [vp, C.c_int, C.c_int]
+        L.orc_stepper_free.argtypes = [vp]
+        L.orc_stepper_coeffs.argtypes = [vp, dp, dp, dp]
+        L.orc_stepper_nstages.argtypes = [vp]
+        L.orc_stepper_nstages.restype = C.c_int
+        L.orc_do_steps.argtypes = [vp, C.c_double, C.c_double, C.c_int, C.c_int, dp, dp, dp]
+        for fn in ("orc_energy",):
+            getattr(L, fn).restype = C.c_double
+            getattr(L, fn).argtypes = [vp, dp, dp]
+        L.orc_mass.restype = C.c_double
+        L.orc_mass.argtypes = [vp, dp]
+        L.orc_circulation.restype = C.c_double
+        L.orc_circulation.argtypes = [vp, dp]
+        L.orc_set_num_threads.argtypes = [C.c_int]
+        L.orc_get_max_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def set_num_threads(n: int) -> None:
+    lib().orc_set_num_threads(int(n))
+
+
+def max_threads() -> int:
+    return lib().orc_get_max_threads()
+
+
+def _view(ptr, dtype, n0, n1):
+    if not ptr:
+        raise KeyError("no such array")
+    ct = C.c_int if dtype == 0 else C.c_double
+    npd = np.int32 if dtype == 0 else np.float64
+    count = n0 * n1
+    buf = (ct * count).from_address(ptr)
+    a = np.frombuffer(buf, dtype=npd, count=count)
+    return a.reshape(n0, n1) if n1 != 1 else a
+
+
+class Mesh:
+    """Planar hexagonal mesh, reference numbering
+    (src/mesh/planar_hexagonal_mesh.cpp:8-398)."""
+
+    def __init__(self, nx: int, ny: int, dc: float | None = None, nlayers: int = 1):
+        if dc is None:
+            dc = 1.0 / nx  # planar_hexagonal_mesh.cpp:5-6
+        self._h = lib().orc_mesh_planar_hex(nx, ny, dc, nlayers)
+        ints = (C.c_int * 6)()
+        reals = (C.c_double * 3)()
+        lib().orc_mesh_info(self._h, ints, reals)
+        (self.nx, self.ny, self.nlayers, self.ncells, self.nedges, self.nvertices) = list(ints)
+        self.dc, self.period_x, self.period_y = list(reals)
+
+    def __getattr__(self, name):
+        if name.startswith("_"):
+            raise AttributeError(name)
+        dt, n0, n1 = C.c_int(), C.c_long(), C.c_long()
+        p = lib().orc_mesh_array(self._h, name.encode(), C.byref(dt), C.byref(n0), C.byref(n1))
+        if not p:
+            raise AttributeError(name)
+        return _view(p, dt.value, n0.value, n1.value)
+
+    def finalize(self):
+        lib().orc_mesh_finalize(self._h)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.orc_mesh_free(self._h)
+            self._h = None
+
+
+PV, KE, SSH, VDEL2, VDEL4, TDEL2, TDEL4 = 1, 2, 4, 8, 16, 32, 64
+
+
+class Model:
+    """ShallowWaterModel restatement (src/model/shallow_water_model.cpp)."""
+
+    def __init__(self, mesh: Mesh, f0=0.0, grav=9.81, drag_coeff=0.0, visc_del2=0.0,
+                 visc_del4=0.0, eddy_diff2=0.0, eddy_diff4=0.0, ntracers=0,
+                 disable_h_tendency=False, disable_vn_tendency=False):
+        self.mesh = mesh
+        self.params = Params(f0, grav, drag_coeff, visc_del2, visc_del4, eddy_diff2, eddy_diff4,
+                             int(ntracers), int(disable_h_tendency), int(disable_vn_tendency))
+        self._h = lib().orc_model_create(mesh._h, C.byref(self.params))
+        self._cb = None
+        self.ntracers = int(ntracers)
+
+    def __getattr__(self, name):
+        if name.startswith("_"):
+            raise AttributeError(name)
+        dt, n0, n1 = C.c_int(), C.c_long(), C.c_long()
+        p = lib().orc_model_array(self._h, name.encode(), C.byref(dt), C.byref(n0), C.byref(n1))
+        if not p:
+            raise AttributeError(name)
+        return _view(p, dt.value, n0.value, n1.value)
+
+    @property
+    def terms(self) -> int:
+        return lib().orc_model_get_terms(self._h)
+
+    @terms.setter
+    def terms(self, mask: int) -> None:
+        lib().orc_model_set_terms(self._h, int(mask))
+
+    def set_additional_tendency(self, fn):
+        """fn(h_tend, vn_tend, h, vn, t) with numpy views, modifies tendencies in place."""
+        m = self.mesh
+        L = m.nlayers
+
+        def tramp(user, ht, vt, h, vn, t):
+            a = np.ctypeslib.as_array(ht, shape=(m.ncells, L))
+            b = np.ctypeslib.as_array(vt, shape=(m.nedges, L))
+            c = np.ctypeslib.as_array(h, shape=(m.ncells, L))
+            d = np.ctypeslib.as_array(vn, shape=(m.nedges, L))
+            fn(a, b, c, d, t)
+
+        self._cb = ADDFN(tramp) if fn is not None else None
+        lib().orc_model_set_additional(self._h, self._cb if fn is not None else ADDFN(), None)
+
+    def new_state(self):
+        m = self.mesh
+        L = m.nlayers
+        return (np.zeros((m.ncells, L)), np.zeros((m.nedges, L)),
+                np.zeros((self.ntracers, m.ncells, L)))
+
+    def aux_compute(self, h, vn, tr):
+        lib().orc_aux_compute(self._h, _dptr(h), _dptr(vn), _dptr(tr))
+
+    def compute_tendency(self, tend, state, t=0.0, increment=False):
+        ht, vt, tt = tend
+        h, vn, tr = state
+        lib().orc_compute_tendency(self._h, _dptr(ht), _dptr(vt), _dptr(tt), _dptr(h), _dptr(vn),
+                                   _dptr(tr), float(t), int(bool(increment)))
+
+    def energy(self, h, vn):
+        return lib().orc_energy(self._h, _dptr(h), _dptr(vn))
+
+    def mass(self, h):
+        return lib().orc_mass(self._h, _dptr(h))
+
+    def circulation(self, vn):
+        return lib().orc_circulation(self._h, _dptr(vn))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.orc_model_free(self._h)
+            self._h = None
+
+
+class Stepper:
+    """RK4Stepper (kind='rk4', src/time_integration/rk4.cpp) or
+    LSRKStepper (kind='lsrk', src/time_integration/lsrk.cpp)."""
+
+    def __init__(self, model: Model, kind: str = "rk4", nstages: int = 5):
+        self.model = model
+        self._h = lib().orc_stepper_create(model._h, 0 if kind == "rk4" else 1, int(nstages))
+        self.nstages = lib().orc_stepper_nstages(self._h)
+
+    def coeffs(self):
+        a = np.zeros(self.nstages)
+        b = np.zeros(self.nstages)
+        c = np.zeros(self.nstages)
+        lib().orc_stepper_coeffs(self._h, _dptr(a), _dptr(b), _dptr(c))
+        return a, b, c
+
+    def do_step(self, t, dt, state):
+        self.run(t, dt, 1, state)
+
+    def run(self, t0, dt, nsteps, state, t_from_step_plus_one=False):
+        """nsteps steps with t = t0 + step*dt (or (step+1)*dt)."""
+        h, vn, tr = state
+        lib().orc_do_steps(self._h, float(t0), float(dt), int(nsteps),
+                           int(bool(t_from_step_plus_one)), _dptr(h), _dptr(vn), _dptr(tr))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.orc_stepper_free(self._h)
+            self._h = None
