@@ -89,7 +89,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--launch", default="", help="rows,lpt,swizzle")
+    ap.add_argument("--launch", default="", help="rows,lpt,swizzle,levels_per_block")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -100,8 +100,7 @@ def main():
 
     cfg, mesh, params, model, state0 = build_case(args.config, args.ordering)
     if args.launch:
-        r, l, s = (int(x) for x in args.launch.split(","))
-        model.set_launch(r, l, s)
+        model.set_launch(*(int(x) for x in args.launch.split(",")))
     ntr, L, nc = cfg["ntracers"], cfg["nlayers"], mesh.ncells
     dt = wl.dt_rule(cfg["dc"], cfg["grav"])
     state = swe.ShallowWaterState(mesh, ntr)

@@ -125,6 +125,11 @@ int omega_mesh_device_array(omega_mesh_t mesh, const char *name, const double **
  * 1 edges, 2 vertices. */
 int omega_mesh_permutation(omega_mesh_t mesh, int kind, int *new2old);
 
+/* 1 if every edge of the mesh follows the regular-hexagon stencil pattern
+ * the specialised kernels are compiled for, else 0 (host-side check; no
+ * reference counterpart). */
+int omega_mesh_regular_hex(omega_mesh_t mesh, int *is_regular);
+
 /* ----------------------------------------------------------------------
  * State (ShallowWaterState, src/model/shallow_water_state.hpp:9-16)
  * -------------------------------------------------------------------- */
@@ -205,13 +210,25 @@ int omega_model_compute_tendency(omega_model_t model, omega_state_t tend, omega_
 int omega_model_set_terms(omega_model_t model, int mask);
 int omega_model_get_terms(omega_model_t model, int *mask);
 
-/* Launch shape: rows (entities) per block, levels per thread (1 or 2) and
- * whether blocks are swizzled so each XCD walks a contiguous stretch of the
- * cell curve.  0 keeps the current value. */
+/* Launch shape: cells per block (0: fill 256 threads), levels per thread
+ * (1 or 2), whether blocks are swizzled so each XCD walks a contiguous
+ * stretch of the cell curve (-1 keeps it), and levels per block (0: all).
+ * No reference counterpart (Kokkos chose its tiles, src/common.hpp:78-104);
+ * these are the autotuner's knobs.  Non-positive values keep the current
+ * setting unless stated. */
 int omega_model_set_launch(omega_model_t model, int rows_per_block, int levels_per_thread,
-                           int xcd_swizzle);
+                           int xcd_swizzle, int levels_per_block);
 int omega_model_get_launch(omega_model_t model, int *rows_per_block, int *levels_per_thread,
-                           int *xcd_swizzle);
+                           int *xcd_swizzle, int *levels_per_block);
+
+/* Kernel family: 0 (default) uses the pattern-specialised kernels when the
+ * mesh is a regular hexagonal mesh verified against the built-in stencil
+ * pattern and nlayers is a multiple of 64, 1 forces the general kernels.
+ * omega_model_kernel_variant reports what runs: 2 specialised, 1 general.
+ * No reference counterpart (performance control only; both give the same
+ * bits). */
+int omega_model_set_variant(omega_model_t model, int variant);
+int omega_model_kernel_variant(omega_model_t model, int *in_use);
 
 /* ----------------------------------------------------------------------
  * Time steppers (src/time_integration/time_stepper.hpp:8-38)

@@ -259,12 +259,15 @@ void run_stage(omega_model_s *M, StageArgs a, hipStream_t s) {
   a.ntr = M->params.ntracers;
   // diagnostics are needed by the vn tendency and by tracer del4 only
   const bool need_diag = a.do_vn || ((M->terms & T_TDEL4) && a.ntr > 0);
+  const bool hex = hex_usable(mv, M->lc);
   if (need_diag) {
     int dterms = a.do_vn ? M->terms : (M->terms & T_TDEL4);
-    launch_diag(mv, M->lc, mc, dterms, a.ntr, a.h.in, a.vn.in, a.tr.in, d, s);
+    if (hex) launch_diag_hex(mv, M->lc, dterms, a.ntr, a.h.in, a.vn.in, a.tr.in, d, s);
+    else launch_diag(mv, M->lc, mc, dterms, a.ntr, a.h.in, a.vn.in, a.tr.in, d, s);
     if (a.do_vn && (M->terms & T_VDEL4)) launch_vdel4(mv, M->lc, d, s);
   }
-  launch_tend(mv, M->lc, mc, a, d, s);
+  if (hex) launch_tend_hex(mv, M->lc, mc, a, d, s);
+  else launch_tend(mv, M->lc, mc, a, d, s);
 }
 
 FieldIO fio(const Real *in, Real *tend, Real *acc, const Real *yold, Real *out) {
@@ -671,6 +674,13 @@ int omega_mesh_permutation(omega_mesh_t mesh, int kind, int *new2old) {
   });
 }
 
+int omega_mesh_regular_hex(omega_mesh_t mesh, int *is_regular) {
+  return guarded([&] {
+    OMEGA_REQUIRE(mesh && is_regular, "null argument");
+    *is_regular = mesh->dm.hex_ok ? 1 : 0;
+  });
+}
+
 // --------------------------------------------------------------------------
 int omega_state_create(omega_mesh_t mesh, int ntracers, omega_state_t *out) {
   return guarded([&] {
@@ -852,7 +862,7 @@ int omega_model_get_terms(omega_model_t M, int *mask) {
   });
 }
 
-int omega_model_set_launch(omega_model_t M, int rows, int lpt, int swizzle) {
+int omega_model_set_launch(omega_model_t M, int rows, int lpt, int swizzle, int chunk) {
   return guarded([&] {
     OMEGA_REQUIRE(M, "null model");
     if (rows > 0) {
@@ -864,15 +874,32 @@ int omega_model_set_launch(omega_model_t M, int rows, int lpt, int swizzle) {
       M->lc.lpt = lpt;
     }
     if (swizzle >= 0) M->lc.swizzle = swizzle ? 1 : 0;
+    if (chunk >= 0) M->lc.chunk = chunk;
   });
 }
 
-int omega_model_get_launch(omega_model_t M, int *rows, int *lpt, int *swizzle) {
+int omega_model_set_variant(omega_model_t M, int variant) {
+  return guarded([&] {
+    OMEGA_REQUIRE(M, "null model");
+    OMEGA_REQUIRE(variant >= 0 && variant <= 1, "variant must be 0 or 1");
+    M->lc.variant = variant;
+  });
+}
+
+int omega_model_kernel_variant(omega_model_t M, int *in_use) {
+  return guarded([&] {
+    OMEGA_REQUIRE(M && in_use, "null argument");
+    *in_use = hex_usable(M->mesh->dm.view(), M->lc) ? 2 : 1;
+  });
+}
+
+int omega_model_get_launch(omega_model_t M, int *rows, int *lpt, int *swizzle, int *chunk) {
   return guarded([&] {
     OMEGA_REQUIRE(M, "null model");
     if (rows) *rows = M->lc.rows;
     if (lpt) *lpt = M->lc.lpt;
     if (swizzle) *swizzle = M->lc.swizzle;
+    if (chunk) *chunk = M->lc.chunk;
   });
 }
 

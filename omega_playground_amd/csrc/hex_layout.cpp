@@ -1,0 +1,80 @@
+// Records for the pattern-specialised kernels on the regular hexagonal mesh.
+//
+// For every edge the record holds its ten edges-on-edge and the ten distinct
+// vertices and cells they touch, in first-appearance order; every edge must
+// reproduce the tables of hex_pattern.hpp exactly, every cell must own three
+// edges (its local edges 0..2, in class order) and two vertices, and every
+// cell edge must list the cell second for j < 3 and first for j >= 3.  Any
+// mismatch leaves hex_ok false and the general kernels are used.
+#include <algorithm>
+
+#include "hex_pattern.hpp"
+#include "internal.hpp"
+
+namespace omega_hip {
+
+namespace {
+
+int local_index(std::vector<int> &list, int id) {
+  for (size_t i = 0; i < list.size(); ++i)
+    if (list[i] == id) return (int)i;
+  list.push_back(id);
+  return (int)list.size() - 1;
+}
+
+} // namespace
+
+bool build_hex_records(DeviceMesh &dm) {
+  const HostMesh &m = dm.host;
+  const int NC = m.ncells, NE = m.nedges, ME = m.max_edges;
+  dm.hex_ok = false;
+  if (ME < 6 || dm.me2 != hex::kSlots) return false;
+  // cells_on_cell in device numbering, derived from the cell's edges
+  dm.coc_h.assign((size_t)NC * 6, 0);
+  for (int c = 0; c < NC; ++c) {
+    if (dm.noc_h[c] != 6) return false;
+    if (dm.eofs_h[c + 1] - dm.eofs_h[c] != 3 || dm.vofs_h[c + 1] - dm.vofs_h[c] != 2) return false;
+    for (int j = 0; j < 6; ++j) {
+      const int e = dm.eoc_h[(size_t)c * ME + j];
+      if (j < 3 && e != dm.eofs_h[c] + j) return false;
+      const int a0 = dm.coe_h[2 * (size_t)e], a1 = dm.coe_h[2 * (size_t)e + 1];
+      const int other = hex::cell_is_second(j) ? a0 : a1;
+      const int self = hex::cell_is_second(j) ? a1 : a0;
+      if (self != c || other == c) return false;
+      dm.coc_h[(size_t)c * 6 + j] = other;
+    }
+  }
+  dm.hexrec_h.assign((size_t)NE * hex::kRecInts, 0);
+  std::vector<int> uv, uc;
+  for (int c = 0; c < NC; ++c) {
+    for (int k = 0; k < 3; ++k) {
+      const int e = dm.eofs_h[c] + k;
+      if (dm.noe_h[e] != hex::kSlots) return false;
+      uv.clear();
+      uc.clear();
+      int *rec = &dm.hexrec_h[(size_t)e * hex::kRecInts];
+      for (int j = 0; j < hex::kSlots; ++j) {
+        const int ej = dm.eoe_h[(size_t)e * dm.me2 + j];
+        rec[j] = ej;
+        const int v0 = local_index(uv, dm.voe_h[2 * (size_t)ej]);
+        const int v1 = local_index(uv, dm.voe_h[2 * (size_t)ej + 1]);
+        const int q0 = local_index(uc, dm.coe_h[2 * (size_t)ej]);
+        const int q1 = local_index(uc, dm.coe_h[2 * (size_t)ej + 1]);
+        if (v0 != hex::LV0[k][j] || v1 != hex::LV1[k][j] || q0 != hex::LC0[k][j] ||
+            q1 != hex::LC1[k][j])
+          return false;
+      }
+      if ((int)uv.size() != hex::kUniq || (int)uc.size() != hex::kUniq) return false;
+      for (int s = 0; s < 2; ++s) {
+        if (uv[hex::OWNV[k][s]] != dm.voe_h[2 * (size_t)e + s]) return false;
+        if (uc[hex::OWNC[k][s]] != dm.coe_h[2 * (size_t)e + s]) return false;
+      }
+      std::copy(uv.begin(), uv.end(), rec + 10);
+      std::copy(uc.begin(), uc.end(), rec + 20);
+    }
+  }
+  dm.hex_ok = true;
+  return true;
+}
+
+} // namespace omega_hip

@@ -118,6 +118,10 @@ struct MeshView {
   const Real *s2, *s4;          // mesh_scaling_del2/4
   const Real *crv;              // (dc*inv_area_triangle)*sign [nv][3]
   const Real *cth;              // kite*inv_area_triangle     [nv][3]
+  // regular-hexagon records (valid when hex_ok, see hex_pattern.hpp)
+  int hex_ok;
+  const int *hexrec;            // [ne][30]
+  const int *coc;               // cells_on_cell [nc][6]
 };
 
 struct DeviceMesh {
@@ -135,6 +139,10 @@ struct DeviceMesh {
   // and batch builders)
   std::vector<int> noc_h, eoc_h, coe_h, voe_h, noe_h, eoe_h, eov_h, cov_h, maxlev_h;
   std::vector<Real> cdiv_h, cke_h, w_h, inv_dc_h, inv_dv_h, inv_dv4_h, s2_h, s4_h, crv_h, cth_h;
+  // regular-hexagon records for the pattern-specialised kernels
+  bool hex_ok = false;
+  std::vector<int> hexrec_h, coc_h;
+  DBuf<int> hexrec, coc;
 
   bool uploaded = false;
 
@@ -145,6 +153,8 @@ struct DeviceMesh {
 
 // Cell ordering along a space-filling curve (host).
 std::vector<int> order_cells(const HostMesh &m, int ordering);
+// Builds and verifies the regular-hexagon records (hex_layout.cpp).
+bool build_hex_records(DeviceMesh &dm);
 
 // ---------------------------------------------------------------------------
 // Launch configuration and kernel parameter blocks
@@ -153,6 +163,8 @@ struct Launch {
   int rows = 0;      // entities per block (blockDim.y); 0 fills 256 threads
   int lpt = 1;       // levels per thread (1 or 2)
   int swizzle = 1;   // XCD-contiguous block order
+  int chunk = 0;     // levels per block (0: all levels, up to 64 lanes)
+  int variant = 0;   // 0: pattern-specialised kernels when the mesh allows, 1: general
 };
 
 // Enabled-term bits (same meaning as omega_model_set_terms).
@@ -213,6 +225,12 @@ void launch_diag(const MeshView &m, const Launch &lc, const ModelCoef &mc, int t
 void launch_vdel4(const MeshView &m, const Launch &lc, const DiagOut &d, hipStream_t s);
 void launch_tend(const MeshView &m, const Launch &lc, const ModelCoef &mc, const StageArgs &a,
                  const DiagOut &d, hipStream_t s);
+// Pattern-specialised kernels for the regular hexagonal mesh (kernels_hex.hip)
+bool hex_usable(const MeshView &m, const Launch &lc);
+void launch_diag_hex(const MeshView &m, const Launch &lc, int terms, int ntr, const Real *h,
+                     const Real *vn, const Real *tr, const DiagOut &d, hipStream_t s);
+void launch_tend_hex(const MeshView &m, const Launch &lc, const ModelCoef &mc, const StageArgs &a,
+                     const DiagOut &d, hipStream_t s);
 // y += alpha*x elementwise
 void launch_axpy(Real *y, Real alpha, const Real *x, size_t n, hipStream_t s);
 // y = x + alpha*z elementwise

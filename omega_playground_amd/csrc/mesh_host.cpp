@@ -409,6 +409,7 @@ void DeviceMesh::build_device(int ord) {
     }
   }
 
+  build_hex_records(*this);
   uploaded = false;
 }
 
@@ -424,6 +425,10 @@ void DeviceMesh::upload() {
   s2.upload(s2_h); s4.upload(s4_h); maxlev_e.upload(maxlev_h);
   eov.upload(eov_h); cov.upload(cov_h); crv.upload(crv_h); cth.upload(cth_h);
   eofs.upload(eofs_h); vofs.upload(vofs_h);
+  if (hex_ok) {
+    hexrec.upload(hexrec_h);
+    coc.upload(coc_h);
+  }
 
   auto perm_upload = [](DBuf<Real> &dst, const std::vector<double> &src,
                         const std::vector<int> &n2o) {
@@ -473,6 +478,9 @@ MeshView DeviceMesh::view() const {
   v.eov = eov.p; v.cov = cov.p; v.eofs = eofs.p; v.vofs = vofs.p; v.maxlev_e = maxlev_e.p;
   v.cdiv = cdiv.p; v.cke = cke.p; v.w = w.p; v.inv_dc = inv_dc.p; v.inv_dv = inv_dv.p;
   v.inv_dv4 = inv_dv4.p; v.s2 = s2.p; v.s4 = s4.p; v.crv = crv.p; v.cth = cth.p;
+  v.hex_ok = hex_ok ? 1 : 0;
+  v.hexrec = hexrec.p;
+  v.coc = coc.p;
   return v;
 }
 

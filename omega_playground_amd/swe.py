@@ -83,6 +83,7 @@ def lib():
         "omega_mesh_host_array": ([vp, C.c_char_p, pp, ip, i64p, i64p], C.c_int),
         "omega_mesh_device_array": ([vp, C.c_char_p, pp, i64p], C.c_int),
         "omega_mesh_permutation": ([vp, C.c_int, ip], C.c_int),
+        "omega_mesh_regular_hex": ([vp, ip], C.c_int),
         "omega_state_create": ([vp, C.c_int, pp], C.c_int),
         "omega_state_destroy": ([vp], C.c_int),
         "omega_state_upload": ([vp, dp, dp, dp, vp], C.c_int),
@@ -96,8 +97,10 @@ def lib():
         "omega_model_compute_tendency": ([vp, vp, vp, C.c_double, C.c_int, vp], C.c_int),
         "omega_model_set_terms": ([vp, C.c_int], C.c_int),
         "omega_model_get_terms": ([vp, ip], C.c_int),
-        "omega_model_set_launch": ([vp, C.c_int, C.c_int, C.c_int], C.c_int),
-        "omega_model_get_launch": ([vp, ip, ip, ip], C.c_int),
+        "omega_model_set_launch": ([vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+        "omega_model_get_launch": ([vp, ip, ip, ip, ip], C.c_int),
+        "omega_model_set_variant": ([vp, C.c_int], C.c_int),
+        "omega_model_kernel_variant": ([vp, ip], C.c_int),
         "omega_stepper_create": ([vp, C.c_int, C.c_int, pp], C.c_int),
         "omega_stepper_destroy": ([vp], C.c_int),
         "omega_stepper_do_step": ([vp, C.c_double, C.c_double, vp, vp], C.c_int),
@@ -176,6 +179,13 @@ class MPASMesh:
             check(lib().omega_mesh_permutation(self._h, kind, out.ctypes.data_as(C.POINTER(C.c_int))))
             self._perm[kind] = out
         return self._perm[kind]
+
+    @property
+    def regular_hex(self) -> bool:
+        """True when the pattern-specialised kernels can run on this mesh."""
+        v = C.c_int()
+        check(lib().omega_mesh_regular_hex(self._h, C.byref(v)))
+        return bool(v.value)
 
     def device_array(self, name: str):
         p, n = C.c_void_p(), C.c_int64()
@@ -366,13 +376,25 @@ class ShallowWaterModel:
     def terms(self, mask: int):
         check(lib().omega_model_set_terms(self._h, int(mask)))
 
-    def set_launch(self, rows_per_block=0, levels_per_thread=0, xcd_swizzle=-1):
-        check(lib().omega_model_set_launch(self._h, rows_per_block, levels_per_thread, xcd_swizzle))
+    def set_launch(self, rows_per_block=0, levels_per_thread=0, xcd_swizzle=-1,
+                   levels_per_block=-1):
+        check(lib().omega_model_set_launch(self._h, rows_per_block, levels_per_thread, xcd_swizzle,
+                                           levels_per_block))
 
     def get_launch(self):
-        a, b, c = C.c_int(), C.c_int(), C.c_int()
-        check(lib().omega_model_get_launch(self._h, C.byref(a), C.byref(b), C.byref(c)))
-        return a.value, b.value, c.value
+        a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        check(lib().omega_model_get_launch(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        return a.value, b.value, c.value, d.value
+
+    def set_variant(self, variant: int):
+        """0: pattern-specialised kernels when possible, 1: general kernels."""
+        check(lib().omega_model_set_variant(self._h, int(variant)))
+
+    def kernel_variant(self) -> int:
+        """2 if the pattern-specialised kernels run, 1 for the general ones."""
+        v = C.c_int()
+        check(lib().omega_model_kernel_variant(self._h, C.byref(v)))
+        return v.value
 
     def set_f_vertex(self, f_vertex: np.ndarray):
         f = np.ascontiguousarray(f_vertex, dtype=np.float64)

@@ -98,3 +98,27 @@ def test_mpas_mesh_roundtrip_layout():
     for f in ("edge_sign_on_cell", "kite_index_on_cell", "edge_sign_on_vertex"):
         assert np.array_equal(getattr(g, f), getattr(o, f)), f
     assert np.array_equal(g.permutation(1), np.arange(o.nedges))
+
+
+@pytest.mark.parametrize("ordering", ["original", "hilbert", "morton"])
+@pytest.mark.parametrize("nx,ny", [(8, 8), (16, 10), (33, 20), (64, 64)])
+def test_regular_hex_pattern_detected(ordering, nx, ny):
+    """Every planar mesh large enough to have ten distinct stencil vertices
+    and cells per edge follows the specialised-kernel pattern."""
+    g = swe.PlanarHexagonalMesh(nx, ny, 1.0, 64, ordering=ordering)
+    assert g.regular_hex
+
+
+def test_regular_hex_pattern_rejects_degenerate_and_general_meshes():
+    # 3x3 periodic mesh: stencils wrap onto themselves
+    assert not swe.PlanarHexagonalMesh(3, 3, 1.0, 1).regular_hex
+    # a general mesh whose edges-on-edge lists are reversed breaks the pattern
+    o = po.Mesh(10, 8, 2.0, 3)
+    arrays = {f: np.array(getattr(o, f)) for f in (
+        "nedges_on_cell", "edges_on_cell", "vertices_on_cell", "nedges_on_edge", "edges_on_edge",
+        "cells_on_edge", "vertices_on_edge", "edges_on_vertex", "cells_on_vertex", "area_cell",
+        "dc_edge", "dv_edge", "weights_on_edge", "area_triangle", "kiteareas_on_vertex")}
+    assert swe.MPASMesh.from_arrays(arrays, nlayers=3).regular_hex
+    arrays["edges_on_edge"][:, :10] = arrays["edges_on_edge"][:, 9::-1]
+    arrays["weights_on_edge"][:, :10] = arrays["weights_on_edge"][:, 9::-1]
+    assert not swe.MPASMesh.from_arrays(arrays, nlayers=3).regular_hex

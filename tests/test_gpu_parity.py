@@ -37,7 +37,8 @@ def upload(gm, ntr, arrays):
 
 
 @pytest.mark.parametrize("ordering", ["original", "hilbert", "morton"])
-@pytest.mark.parametrize("launch", [(4, 1, 0), (4, 2, 1), (2, 2, 1), (8, 1, 1)])
+@pytest.mark.parametrize("launch", [(4, 1, 0, 0), (4, 2, 1, 0), (2, 2, 1, 4), (8, 1, 1, 2),
+                                    (0, 2, 1, 6)])
 def test_tendency_bitwise(ordering, launch):
     om, omodel, gm, gmodel, state = make_pair(14, 10, 8, 3, ordering=ordering)
     gmodel.set_launch(*launch)
@@ -177,3 +178,47 @@ def test_diagnostics_match_oracle():
         v2 = fn(gs, gmodel)
         assert v1 == v2  # deterministic
         assert abs(v1 - ref) <= 1e-13 * max(abs(ref), 1.0)
+
+
+@pytest.mark.parametrize("variant,lpt", [(0, 2), (0, 1), (1, 2)])
+@pytest.mark.parametrize("ordering", ["hilbert", "original"])
+def test_specialised_kernels_bitwise_L64(variant, lpt, ordering):
+    """nlayers = 64 on a planar mesh runs the pattern-specialised kernels
+    (variant 0, one or two levels per lane); variant 1 forces the general
+    ones.  All must reproduce the oracle bit for bit, for a tendency and for
+    RK4 / LSRK steps."""
+    om, omodel, gm, gmodel, state = make_pair(12, 10, 64, 3, ordering=ordering)
+    gmodel.set_variant(variant)
+    gmodel.set_launch(0, lpt)
+    assert gmodel.kernel_variant() == (2 if variant == 0 else 1)
+    tend_o = omodel.new_state()
+    omodel.compute_tendency(tend_o, state, 0.0)
+    gs = upload(gm, 3, state)
+    gt = swe.ShallowWaterState(gm, 3)
+    gmodel.compute_tendency(gt, gs, 0.0)
+    for name, a, b in zip(("h", "vn", "tr"), gt.download(), tend_o):
+        assert same_bits(a, b), name
+    for kind in ("rk4", "lsrk"):
+        ost = tuple(a.copy() for a in state)
+        po.Stepper(omodel, kind, 5).run(0.0, 45.0, 3, ost)
+        gs = upload(gm, 3, state)
+        gst = swe.RK4Stepper(gmodel) if kind == "rk4" else swe.LSRKStepper(gmodel, 5)
+        gst.set_graph(True)
+        gst.run(0.0, 45.0, 3, gs)
+        for name, a, b in zip(("h", "vn", "tr"), gs.download(), ost):
+            assert same_bits(a, b), (kind, name)
+
+
+@pytest.mark.parametrize("terms", [1, 2, 4, 8, 16, 32, 64])
+def test_specialised_each_term_bitwise(terms):
+    om, omodel, gm, gmodel, state = make_pair(10, 10, 64, 2)
+    assert gmodel.kernel_variant() == 2
+    omodel.terms = terms
+    gmodel.terms = terms
+    tend_o = omodel.new_state()
+    omodel.compute_tendency(tend_o, state, 0.0)
+    gs = upload(gm, 2, state)
+    gt = swe.ShallowWaterState(gm, 2)
+    gmodel.compute_tendency(gt, gs, 0.0)
+    for a, b in zip(gt.download(), tend_o):
+        assert same_bits(a, b)
