@@ -18,7 +18,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle_swe.so")
+# ORACLE_LIB selects another build (the mutation check's planted-mistake builds)
+_LIB_PATH = os.environ.get("ORACLE_LIB") or os.path.join(_HERE, "liboracle_swe.so")
 _lib = None
 
 
@@ -175,7 +176,9 @@ class Model:
         p = lib().orc_model_array(self._h, name.encode(), C.byref(dt), C.byref(n0), C.byref(n1))
         if not p:
             raise AttributeError(name)
-        return _view(p, dt.value, n0.value, n1.value)
+        a = _view(p, dt.value, n0.value, n1.value)
+        # level-resolved arrays stay [entity, level] even with one level
+        return a if name in ("f_vertex", "f_edge") else a.reshape(n0.value, n1.value)
 
     @property
     def terms(self) -> int:

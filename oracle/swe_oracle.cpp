@@ -23,9 +23,21 @@
 #define ORC_PARALLEL_FOR
 #endif
 
+// Planted mistakes for the mutation check of the pinning tests
+// (tests/test_oracle_mutants.py builds the oracle with -DORC_MUTANT=n):
+//  1 PV flux drops the neighbour's planetary term    2 SSH gradient sign
+//  3 del2 curl half divided by dc instead of dv       4 tracer edge value without 1/2
+//  5 RK4 weights b0 and b1 swapped                    6 LSRK5 a2 sign flipped
+//  7 kinetic energy without 1/4                       8 vertex circulation sign flipped
+// 0 (the default) is the restatement itself.
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 namespace orc {
 
 static constexpr Real pi = M_PI; // src/common.hpp:27
+static constexpr int mutant = ORC_MUTANT;
 
 // ---------------------------------------------------------------------------
 // Mesh
@@ -361,7 +373,8 @@ void Model::aux_compute(const Real *h, const Real *vn, const Real *tr) {
         const Int jedge = M.edges_on_vertex[3 * iv + j];
         const Int jcell = M.cells_on_vertex[3 * iv + j];
         rvort += M.dc_edge[jedge] * inv_area_triangle *
-                 M.edge_sign_on_vertex[3 * iv + j] * vn[(size_t)jedge * L + k];
+                 (mutant == 8 ? -1.0 : 1.0) * M.edge_sign_on_vertex[3 * iv + j] *
+                 vn[(size_t)jedge * L + k];
         thick += M.kiteareas_on_vertex[3 * iv + j] * inv_area_triangle *
                  h[(size_t)jcell * L + k];
       }
@@ -385,7 +398,7 @@ void Model::aux_compute(const Real *h, const Real *vn, const Real *tr) {
         const Int jedge = M.edges_on_cell[ic * ME + j];
         const Real area_edge = M.dv_edge[jedge] * M.dc_edge[jedge];
         const Real v = vn[(size_t)jedge * L + k];
-        ke += inv_area_cell * area_edge * v * v * 0.25;
+        ke += inv_area_cell * area_edge * v * v * (mutant == 7 ? 1.0 : 0.25);
         div += inv_area_cell * M.dv_edge[jedge] * M.edge_sign_on_cell[ic * ME + j] * v;
       }
       vel_div_cell[(size_t)ic * L + k] = div;
@@ -514,7 +527,8 @@ void Model::compute_vn_tendency(Real *vn_tend, const Real *h, const Real *vn,
           const Int jedge = M.edges_on_edge[ie * 2 * ME + j];
           const Real norm_vort =
               (norm_rvort_edge[(size_t)ie * L + k] + norm_pvort_edge[(size_t)ie * L + k] +
-               norm_rvort_edge[(size_t)jedge * L + k] + norm_pvort_edge[(size_t)jedge * L + k]) *
+               norm_rvort_edge[(size_t)jedge * L + k] +
+               (mutant == 1 ? 0.0 : norm_pvort_edge[(size_t)jedge * L + k])) *
               0.5;
           qt += M.weights_on_edge[ie * 2 * ME + j] * flux_h_edge[(size_t)jedge * L + k] *
                 vn[(size_t)jedge * L + k] * norm_vort;
@@ -525,12 +539,14 @@ void Model::compute_vn_tendency(Real *vn_tend, const Real *h, const Real *vn,
         vt -= (ke_cell[(size_t)c1 * L + k] - ke_cell[(size_t)c0 * L + k]) * inv_dc_edge;
       }
       if (en_ssh) { // ssh_grad_edge.hpp:37-50
-        vt -= grav * (h[(size_t)c1 * L + k] - h[(size_t)c0 * L + k]) * inv_dc_edge;
+        vt -= (mutant == 2 ? -grav : grav) * (h[(size_t)c1 * L + k] - h[(size_t)c0 * L + k]) *
+              inv_dc_edge;
       }
       if (en_vdel2) { // vel_diff_edge.hpp:61-86
         const Real del2u =
             ((vel_div_cell[(size_t)c1 * L + k] - vel_div_cell[(size_t)c0 * L + k]) * inv_dc_edge -
-             (rvort_vertex[(size_t)v1 * L + k] - rvort_vertex[(size_t)v0 * L + k]) * inv_dv_edge);
+             (rvort_vertex[(size_t)v1 * L + k] - rvort_vertex[(size_t)v0 * L + k]) *
+                 (mutant == 3 ? inv_dc_edge : inv_dv_edge));
         vt += M.edge_mask[(size_t)ie * L + k] * visc_del2 * M.mesh_scaling_del2[ie] * del2u;
       }
       if (en_vdel4) { // vel_hyperdiff_edge.hpp:212-237
@@ -596,7 +612,7 @@ void Model::compute_tr_tendency(Real *tr_tend, const Real *tr, const Real *vn,
           const Int jc0 = M.cells_on_edge[2 * jedge], jc1 = M.cells_on_edge[2 * jedge + 1];
           const Real norm_tr_edge = (norm_tr_cell[((size_t)l * NC + jc0) * L + k] +
                                      norm_tr_cell[((size_t)l * NC + jc1) * L + k]) *
-                                    0.5;
+                                    (mutant == 4 ? 1.0 : 0.5);
           accum -= M.dv_edge[jedge] * inv_area_cell * M.edge_sign_on_cell[ic * ME + j] *
                    flux_h[(size_t)jedge * L + k] * norm_tr_edge * vn[(size_t)jedge * L + k];
         }
@@ -661,6 +677,7 @@ Stepper::Stepper(Model *m, Int kind_, Int nstages_) : model(m), kind(kind_) {
     nstages = 4;
     rka = {1. / 2, 1. / 2, 1};
     rkb = {1. / 6, 1. / 3, 1. / 3, 1. / 6};
+    if (mutant == 5) std::swap(rkb[0], rkb[1]);
     rkc = {1. / 2, 1. / 2, 1};
     h_provis.assign(M.ncells * L, 0);
     vn_provis.assign(M.nedges * L, 0);
@@ -681,6 +698,7 @@ Stepper::Stepper(Model *m, Int kind_, Int nstages_) : model(m), kind(kind_) {
              2277821191437. / 14882151754819.};
       rkc = {0., 1432997174477. / 9575080441755., 2526269341429. / 6820363962896.,
              2006345519317. / 3224310063776., 2802321613138. / 2924317926251.};
+      if (mutant == 6) rka[2] = -rka[2];
     }
     if (nstages == 1) {
       rka = {0};
