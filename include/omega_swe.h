@@ -269,6 +269,65 @@ int omega_circulation_integral(omega_model_t model, omega_state_t state, double 
                                omega_stream_t stream);
 
 /* ----------------------------------------------------------------------
+ * Discrete adjoint.  No reference counterpart: the reference has no
+ * adjoint; these differentiate its forward operators
+ * (src/model/shallow_water_model.cpp:43-347) and steppers
+ * (src/time_integration/rk4.cpp:28-104, lsrk.cpp:36-89) exactly, in the
+ * same gather-only, atomic-free style as the forward kernels.  Cotangents
+ * are omega_state_t objects of the model's mesh.  param_grad, when not NULL,
+ * points to 4 host doubles to which the gradient with respect to
+ * (visc_del2, visc_del4, eddy_diff2, eddy_diff4) is ADDED; such calls
+ * synchronise the stream.  A term contributes only while it is enabled
+ * (omega_model_set_terms can enable a term whose coefficient is zero to
+ * differentiate with respect to that coefficient).  Models with an
+ * additional-tendency hook are rejected (OMEGA_ERR_ARG).
+ * -------------------------------------------------------------------- */
+typedef struct omega_tape_s *omega_tape_t;
+
+/* state_cot += J(state)^T tend_cot, J the Jacobian of compute_tendency. */
+int omega_model_tendency_vjp(omega_model_t model, omega_state_t state, omega_state_t tend_cot,
+                             omega_state_t state_cot, double *param_grad,
+                             omega_stream_t stream);
+
+/* cot <- (d y_{n+1} / d y_n)^T cot for one do_step(t, dt) from y_n = state.
+ * The stage states are recomputed from state with the forward kernels. */
+int omega_stepper_step_vjp(omega_stepper_t stepper, double t, double dt, omega_state_t state,
+                           omega_state_t cot, double *param_grad, omega_stream_t stream);
+
+/* Binomial checkpointing over nsteps steps with at most n_snapshots stored
+ * states (the initial state included) and, when byte_cap > 0, at most
+ * byte_cap bytes of snapshots.  The forward sweep advances state from y_0 to
+ * y_nsteps; the reverse sweep maps dJ/dy_nsteps (cot on entry) to dJ/dy_0
+ * (cot on exit), recomputing intermediate states from the snapshots with the
+ * minimal number of forward steps (Griewank's binomial schedule). */
+int omega_tape_create(omega_stepper_t stepper, int nsteps, int n_snapshots, int64_t byte_cap,
+                      omega_tape_t *out);
+int omega_tape_destroy(omega_tape_t tape);
+int omega_tape_forward(omega_tape_t tape, double t0, double dt, omega_state_t state,
+                       omega_stream_t stream);
+int omega_tape_reverse(omega_tape_t tape, omega_state_t cot, double *param_grad,
+                       omega_stream_t stream);
+/* Snapshots in use, forward steps run by the last forward + reverse sweeps
+ * (the step adjoints' own stage recomputation not counted), snapshot bytes. */
+int omega_tape_info(omega_tape_t tape, int *n_snapshots, int64_t *forward_steps,
+                    int64_t *snapshot_bytes);
+/* Minimal forward steps to reverse nsteps steps with n_snapshots snapshots. */
+int64_t omega_revolve_cost(int64_t nsteps, int n_snapshots);
+
+/* Least-squares misfit J = sum_f w_f/2 sum_i a_i sum_k (y - obs)^2 over
+ * f = h (cell areas), vn (edge areas dc*dv/2), tracers (cell areas);
+ * cot = dJ/dy.  weights[3] = (w_h, w_vn, w_tr).  Synchronises. */
+int omega_misfit(omega_model_t model, omega_state_t state, omega_state_t obs,
+                 const double *weights, omega_state_t cot, double *J, omega_stream_t stream);
+
+/* J(y_nsteps) of omega_misfit after nsteps steps from state (left unchanged),
+ * grad = dJ/dy_0 (overwritten), param_grad += dJ/d(coefficients). */
+int omega_adjoint_run(omega_stepper_t stepper, double t0, double dt, int nsteps,
+                      int n_snapshots, int64_t byte_cap, omega_state_t state, omega_state_t obs,
+                      const double *weights, double *J, omega_state_t grad, double *param_grad,
+                      omega_stream_t stream);
+
+/* ----------------------------------------------------------------------
  * Utilities
  * -------------------------------------------------------------------- */
 

@@ -1,0 +1,89 @@
+// Discrete adjoint of the tendency and of the time steppers.
+#pragma once
+
+#include "internal.hpp"
+
+namespace omega_hip {
+
+// Transposed connectivity and coefficients in device numbering, built on
+// the host the first time an adjoint is requested.
+struct AdjMeshView {
+  const Real *ecd;    // [ne][2] cdiv of the edge's slot in cells_on_edge[0..1]
+  const Real *ecke;   // [ne][2] cke of the same slots
+  const Real *ecrv;   // [ne][2] crv of the edge's slot in vertices_on_edge[0..1]
+  const int *voc;     // [nc][me] vertices_on_cell
+  const Real *ccth;   // [nc][me] cth of the cell's slot in each of its vertices
+  const int *tofs;    // [ne+1] edges that list e among their edges_on_edge
+  const int *tedge;   // [nnz] source edge
+  const Real *tw;     // [nnz] its weight for e
+};
+
+struct AdjLayout {
+  DBuf<Real> ecd, ecke, ecrv, ccth, tw;
+  DBuf<int> voc, tofs, tedge;
+  DBuf<Real> area_e;          // [ne] dc*dv/2, the area an edge accounts for
+  std::vector<int> cslot_h;   // [ne][2] slot of the edge in cells_on_edge[0..1]
+  bool built = false;
+  AdjMeshView view() const {
+    return AdjMeshView{ecd.p, ecke.p, ecrv.p, voc.p, ccth.p, tofs.p, tedge.p, tw.p};
+  }
+};
+
+void build_adj_layout(const DeviceMesh &dm, AdjLayout &a);
+
+// Model-dependent transposed tracer coefficients.
+struct AdjCoef {
+  const Real *ect2;   // [ne][2] ct2 of the edge's slot in cells_on_edge[0..1]
+  const Real *ect4;   // [ne][2] ct4 likewise
+};
+
+// Cotangent buffers of one prognostic state (device numbering).
+struct CotIO {
+  const Real *h, *vn, *tr; // cotangent of the tendency (input)
+  Real *oh, *ovn, *otr;    // cotangent of the state (accumulated output)
+};
+
+// Adjoint intermediates (cotangents of diagnostics).
+struct AdjScratch {
+  Real *sbar_e;   // [ne][L] cotangent of the edge PV sum Q_e + P_e
+  Real *zbar_v;   // [nv][L] rvort
+  Real *tbar_v;   // [nv][L] thickness at vertices
+  Real *kbar_c;   // [nc][L] kinetic energy
+  Real *dbar_c;   // [nc][L] divergence
+  Real *gbar_c;   // [ntr][nc][L] tracer Laplacian (tracer del4)
+  Real *fbar_e;   // [ne][L] edge thickness
+  Real *d2bar_c;  // [nc][L] momentum del4: del2 divergence
+  Real *z2bar_v;  // [nv][L] momentum del4: del2 vorticity
+  Real *e2bar_e;  // [ne][L] momentum del4: del2 edge
+  Real *partial;  // [4][nparts] parameter-gradient block partials
+  Real *pgrad;    // [4] accumulated parameter gradient
+};
+
+struct AdjArgs {
+  int terms, do_h, do_vn, ntr;
+  Real grav;
+  const Real *h, *vn, *tr;  // state at which the tendency is linearised
+  CotIO c;
+  int nparts;               // partial slots available per parameter
+};
+
+void launch_tendency_vjp(const MeshView &m, const AdjMeshView &am, const Launch &lc,
+                         const ModelCoef &mc, const AdjCoef &ac, const AdjArgs &a,
+                         const DiagOut &d, const AdjScratch &s, hipStream_t st);
+int adj_partial_slots(const MeshView &m, const Launch &lc);
+
+// y[i] = v
+void launch_fill(Real *y, Real v, size_t n, hipStream_t s);
+// y = alpha*x + beta*z (y may alias x or z)
+void launch_lincomb(Real *y, Real alpha, const Real *x, Real beta, const Real *z, size_t n,
+                    hipStream_t s);
+// Weighted least-squares misfit of one field: for entity i, level k,
+// r = y - obs, cot = w*area[i]*r, partial sums of w*area[i]*r^2/2 into
+// partial[0..misfit_blocks(n*L)); returns nothing (fold with launch_sum).
+int misfit_blocks(size_t n);
+void launch_misfit(const Real *y, const Real *obs, const Real *area, int nent, int L, int nrep,
+                   Real w, Real *cot, Real *partial, hipStream_t s);
+// out[0] (+)= fixed-order sum of partial[0..n)
+void launch_sum(const Real *partial, int n, Real *out, int accumulate, hipStream_t s);
+
+} // namespace omega_hip

@@ -1,107 +1,21 @@
-// C ABI of the shallow-water library: object lifetimes, stage orchestration,
-// graph capture, diagnostics and timing.
+// C ABI of the shallow-water library: object lifetimes, host copies,
+// diagnostics and timing.  Stage orchestration lives in stepper.cpp.
 #include <algorithm>
 #include <cmath>
-#include <map>
 #include <mutex>
-#include <typeinfo>
 
-#include "internal.hpp"
+#include "objects.hpp"
 
 using namespace omega_hip;
 
 // ---------------------------------------------------------------------------
-// Opaque objects
-// ---------------------------------------------------------------------------
-struct omega_mesh_s {
-  DeviceMesh dm;
-  DBuf<Real> staging; // device scratch for omega_add_host_field
-};
-
-struct omega_state_s {
-  omega_mesh_s *mesh = nullptr;
-  int ntr = 0;
-  size_t n[3] = {0, 0, 0}; // element counts of h, vn, tr
-  DBuf<Real> buf[2][3];    // [buffer][field]
-  int cur = 0;             // buffer holding the state
-  Real *ptr(int f) { return buf[cur][f].p; }
-  Real *alt(int f) {
-    if (buf[1 - cur][f].n != n[f]) buf[1 - cur][f].alloc(n[f]);
-    return buf[1 - cur][f].p;
-  }
-};
-
-struct omega_model_s {
-  omega_mesh_s *mesh = nullptr;
-  omega_params params{};
-  int terms = 0;
-  Launch lc;
-  DBuf<Real> f_v, cv2, cv4, ct2, ct4;
-  DBuf<Real> rvort_v, inv_thick_v, ke_c, div_c, trdel2_c, vdel2_e, vdel2_rv_v, vdel2_div_c;
-  DBuf<Real> partial, result;
-  omega_additional_tendency_fn add_fn = nullptr;
-  void *add_user = nullptr;
-
-  ModelCoef coef() const {
-    return ModelCoef{params.grav, f_v.p, cv2.p, cv4.p, ct2.p, ct4.p};
-  }
-  DiagOut diag() {
-    return DiagOut{rvort_v.p, inv_thick_v.p, ke_c.p, div_c.p, trdel2_c.p,
-                   vdel2_e.p, vdel2_rv_v.p, vdel2_div_c.p};
-  }
-};
-
-struct GraphKey {
-  double dt;
-  const void *state;
-  int cur;
-  bool operator<(const GraphKey &o) const {
-    if (dt != o.dt) return dt < o.dt;
-    if (state != o.state) return state < o.state;
-    return cur < o.cur;
-  }
-};
-
-struct omega_stepper_s {
-  omega_model_s *model = nullptr;
-  int kind = OMEGA_STEPPER_RK4;
-  int nstages = 4;
-  std::vector<Real> rka, rkb, rkc;
-  // fused RK4: accumulator and two provisional states; unfused RK4: tend,
-  // provisional and old states; LSRK: tend
-  DBuf<Real> acc[3], p1[3], p2[3], tend[3], provis[3], old[3];
-  bool use_graph = false;
-  std::map<GraphKey, hipGraphExec_t> graphs;
-  hipStream_t cap_stream = nullptr; // private stream the step is captured on
-  ~omega_stepper_s() {
-    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
-    if (cap_stream) (void)hipStreamDestroy(cap_stream);
-  }
-};
-
-// ---------------------------------------------------------------------------
 // Error plumbing
 // ---------------------------------------------------------------------------
-namespace {
+namespace omega_hip {
 thread_local std::string g_last_error;
+} // namespace omega_hip
 
-template <class F> int guarded(F &&f) {
-  try {
-    f();
-    return OMEGA_OK;
-  } catch (const Error &e) {
-    g_last_error = e.what();
-    return e.code;
-  } catch (const std::exception &e) {
-    g_last_error = e.what();
-    return OMEGA_ERR_STATE;
-  } catch (...) {
-    g_last_error = "unknown error";
-    return OMEGA_ERR_STATE;
-  }
-}
-
-hipStream_t as_stream(omega_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+namespace {
 
 // ---------------------------------------------------------------------------
 // Timing
@@ -154,13 +68,15 @@ TimingTable &timing() {
   return t;
 }
 
+} // namespace
+
+namespace omega_hip {
 bool stream_capturing(hipStream_t s) {
   hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
   return st == hipStreamCaptureStatusActive;
 }
-
-} // namespace
+} // namespace omega_hip
 
 namespace omega_hip {
 TimerScope::TimerScope(const char *name, hipStream_t st) : slot(-1), s(st) {
@@ -193,270 +109,6 @@ TimerScope::~TimerScope() {
 // ---------------------------------------------------------------------------
 // Model helpers
 // ---------------------------------------------------------------------------
-namespace {
-
-void model_build_coefficients(omega_model_s *M) {
-  const DeviceMesh &dm = M->mesh->dm;
-  const HostMesh &hm = dm.host;
-  const int NC = hm.ncells, NE = hm.nedges, NV = hm.nvertices, ME = hm.max_edges;
-  const size_t L = hm.nlayers, ntr = M->params.ntracers;
-  std::vector<Real> f(NV, M->params.f0);
-  M->f_v.upload(f);
-  std::vector<Real> cv2(NE), cv4(NE);
-  for (int e = 0; e < NE; ++e) {
-    // mask*visc*scaling with mask = 1 (vel_diff_edge.hpp:83-84,
-    // vel_hyperdiff_edge.hpp:234-235)
-    cv2[e] = M->params.visc_del2 * dm.s2_h[e];
-    cv4[e] = M->params.visc_del4 * dm.s4_h[e];
-  }
-  M->cv2.upload(cv2);
-  M->cv4.upload(cv4);
-  std::vector<Real> ct2((size_t)NC * ME, 0.0), ct4((size_t)NC * ME, 0.0);
-  for (int i = 0; i < NC; ++i) {
-    const int c = dm.cell_n2o[i];
-    const double inv_area = 1. / hm.area_cell[c];
-    for (int j = 0; j < dm.noc_h[i]; ++j) {
-      const int eo = hm.edges_on_cell[(size_t)c * ME + j];
-      const int en = dm.edge_o2n[eo];
-      const double sign = hm.edge_sign_on_cell[(size_t)c * ME + j];
-      // leading factors of tracer_diff_cell.hpp:82-84 and
-      // tracer_hyperdiff_cell.hpp:152-154
-      ct2[(size_t)i * ME + j] = M->params.eddy_diff2 * inv_area * hm.dv_edge[eo] * sign;
-      ct4[(size_t)i * ME + j] =
-          M->params.eddy_diff4 * inv_area * hm.dv_edge[eo] * sign * dm.s4_h[en];
-    }
-  }
-  M->ct2.upload(ct2);
-  M->ct4.upload(ct4);
-  M->rvort_v.alloc((size_t)NV * L);
-  M->inv_thick_v.alloc((size_t)NV * L);
-  M->ke_c.alloc((size_t)NC * L);
-  M->div_c.alloc((size_t)NC * L);
-  if (ntr > 0) M->trdel2_c.alloc(ntr * NC * L);
-  M->partial.alloc((size_t)reduce_blocks(std::max(NC, NV)) + 1);
-  M->result.alloc(4);
-}
-
-void ensure_vdel4(omega_model_s *M) {
-  const HostMesh &hm = M->mesh->dm.host;
-  const size_t L = hm.nlayers;
-  if (!M->vdel2_e.p) {
-    M->vdel2_e.alloc((size_t)hm.nedges * L);
-    M->vdel2_rv_v.alloc((size_t)hm.nvertices * L);
-    M->vdel2_div_c.alloc((size_t)hm.ncells * L);
-  }
-}
-
-// One tendency evaluation (+ optional fused stage update).
-void run_stage(omega_model_s *M, StageArgs a, hipStream_t s) {
-  const MeshView mv = M->mesh->dm.view();
-  const ModelCoef mc = M->coef();
-  if (M->terms & T_VDEL4) ensure_vdel4(M);
-  DiagOut d = M->diag();
-  a.terms = M->terms;
-  a.do_h = !M->params.disable_h_tendency;
-  a.do_vn = !M->params.disable_vn_tendency;
-  a.ntr = M->params.ntracers;
-  // diagnostics are needed by the vn tendency and by tracer del4 only
-  const bool need_diag = a.do_vn || ((M->terms & T_TDEL4) && a.ntr > 0);
-  const bool hex = hex_usable(mv, M->lc);
-  if (need_diag) {
-    int dterms = a.do_vn ? M->terms : (M->terms & T_TDEL4);
-    if (hex) launch_diag_hex(mv, M->lc, dterms, a.ntr, a.h.in, a.vn.in, a.tr.in, d, s);
-    else launch_diag(mv, M->lc, mc, dterms, a.ntr, a.h.in, a.vn.in, a.tr.in, d, s);
-    if (a.do_vn && (M->terms & T_VDEL4)) launch_vdel4(mv, M->lc, d, s);
-  }
-  if (hex) launch_tend_hex(mv, M->lc, mc, a, d, s);
-  else launch_tend(mv, M->lc, mc, a, d, s);
-}
-
-FieldIO fio(const Real *in, Real *tend, Real *acc, const Real *yold, Real *out) {
-  return FieldIO{in, tend, acc, yold, out};
-}
-
-void call_hook(omega_model_s *M, Real *ht, Real *vt, const Real *h, const Real *vn, double t,
-               hipStream_t s) {
-  if (!M->add_fn) return;
-  const int rc = M->add_fn(M->add_user, ht, vt, h, vn, t, s);
-  if (rc != 0) throw Error(OMEGA_ERR_CALLBACK, "additional tendency hook returned " + std::to_string(rc));
-}
-
-void check_state(omega_model_s *M, omega_state_s *S, const char *what) {
-  OMEGA_REQUIRE(S != nullptr, std::string(what) + ": null state");
-  OMEGA_REQUIRE(S->mesh == M->mesh, std::string(what) + ": state belongs to another mesh");
-  OMEGA_REQUIRE(S->ntr >= M->params.ntracers,
-                std::string(what) + ": state has fewer tracers than the model");
-}
-
-void ensure3(DBuf<Real> (&b)[3], omega_state_s *S) {
-  for (int f = 0; f < 3; ++f)
-    if (b[f].n != S->n[f]) b[f].alloc(S->n[f]);
-}
-
-// compute_tendency (shallow_water_model.cpp:313-347), writing tend buffers.
-void compute_tendency_ptrs(omega_model_s *M, Real *ht, Real *vt, Real *tt, const Real *h,
-                           const Real *vn, const Real *tr, double t, int add_mode,
-                           hipStream_t s) {
-  StageArgs a{};
-  a.epi = EPI_STORE;
-  a.base_from_tend = add_mode == OMEGA_ADD_INCREMENT;
-  a.a = 1.0;
-  a.h = fio(h, ht, nullptr, nullptr, nullptr);
-  a.vn = fio(vn, vt, nullptr, nullptr, nullptr);
-  a.tr = fio(tr, tt, nullptr, nullptr, nullptr);
-  run_stage(M, a, s);
-  call_hook(M, ht, vt, h, vn, t, s);
-}
-
-// ---------------------------------------------------------------------------
-// Steppers
-// ---------------------------------------------------------------------------
-
-// RK4 with a fused tendency + update per stage (rk4.cpp:28-104 reorganised:
-// the accumulator lives in acc until the last stage writes the state).
-void rk4_fused_step(omega_stepper_s *St, omega_state_s *S, double dt, hipStream_t s) {
-  omega_model_s *M = St->model;
-  ensure3(St->acc, S);
-  ensure3(St->p1, S);
-  ensure3(St->p2, S);
-  Real *Y[3] = {S->ptr(0), S->ptr(1), S->ptr(2)};
-  Real *A[3] = {St->acc[0].p, St->acc[1].p, St->acc[2].p};
-  Real *P1[3] = {St->p1[0].p, St->p1[1].p, St->p1[2].p};
-  Real *P2[3] = {St->p2[0].p, St->p2[1].p, St->p2[2].p};
-  for (int stage = 0; stage < 4; ++stage) {
-    StageArgs a{};
-    a.base_from_tend = 0;
-    a.db = dt * St->rkb[stage];
-    a.da = stage < 3 ? dt * St->rka[stage] : 0.0;
-    Real **in = stage == 0 ? Y : (stage == 2 ? P2 : P1);
-    Real **out = stage == 0 ? P1 : (stage == 1 ? P2 : (stage == 2 ? P1 : Y));
-    a.epi = stage == 0 ? EPI_RK4_FIRST : (stage == 3 ? EPI_RK4_LAST : EPI_RK4_MID);
-    a.h = fio(in[0], nullptr, A[0], Y[0], out[0]);
-    a.vn = fio(in[1], nullptr, A[1], Y[1], out[1]);
-    a.tr = fio(in[2], nullptr, A[2], Y[2], out[2]);
-    run_stage(M, a, s);
-  }
-}
-
-// RK4 exactly as rk4.cpp:28-104 (used when an additional-tendency hook is set).
-void rk4_unfused_step(omega_stepper_s *St, omega_state_s *S, double t, double dt, hipStream_t s) {
-  omega_model_s *M = St->model;
-  ensure3(St->tend, S);
-  ensure3(St->provis, S);
-  ensure3(St->old, S);
-  for (int f = 0; f < 3; ++f)
-    if (S->n[f])
-      OMEGA_HIP_CHECK(hipMemcpyAsync(St->old[f].p, S->ptr(f), S->n[f] * sizeof(Real),
-                                     hipMemcpyDeviceToDevice, s));
-  compute_tendency_ptrs(M, St->tend[0].p, St->tend[1].p, St->tend[2].p, S->ptr(0), S->ptr(1),
-                        S->ptr(2), t, OMEGA_ADD_REPLACE, s);
-  for (int stage = 0; stage < 4; ++stage) {
-    const double db = dt * St->rkb[stage];
-    for (int f = 0; f < 3; ++f) launch_axpy(S->ptr(f), db, St->tend[f].p, S->n[f], s);
-    if (stage < 3) {
-      const double stagetime = t + St->rkc[stage] * dt;
-      const double da = dt * St->rka[stage];
-      for (int f = 0; f < 3; ++f)
-        launch_xpay(St->provis[f].p, St->old[f].p, da, St->tend[f].p, S->n[f], s);
-      compute_tendency_ptrs(M, St->tend[0].p, St->tend[1].p, St->tend[2].p, St->provis[0].p,
-                            St->provis[1].p, St->provis[2].p, stagetime, OMEGA_ADD_REPLACE, s);
-    }
-  }
-}
-
-// Low-storage RK with the state alternating between two buffers per stage
-// (lsrk.cpp:36-89 fused: tend = a*tend + F(y); y' = y + dt*b*tend).
-void lsrk_fused_step(omega_stepper_s *St, omega_state_s *S, double dt, hipStream_t s) {
-  omega_model_s *M = St->model;
-  for (int stage = 0; stage < St->nstages; ++stage) {
-    StageArgs a{};
-    a.epi = EPI_LSRK;
-    a.a = St->rka[stage];
-    a.base_from_tend = St->rka[stage] != 0.0;
-    a.db = dt * St->rkb[stage];
-    Real *in[3] = {S->ptr(0), S->ptr(1), S->ptr(2)};
-    Real *out[3] = {S->alt(0), S->alt(1), S->alt(2)};
-    a.h = fio(in[0], St->tend[0].p, nullptr, nullptr, out[0]);
-    a.vn = fio(in[1], St->tend[1].p, nullptr, nullptr, out[1]);
-    a.tr = fio(in[2], St->tend[2].p, nullptr, nullptr, out[2]);
-    run_stage(M, a, s);
-    S->cur = 1 - S->cur;
-  }
-}
-
-// LSRK exactly as lsrk.cpp:36-89 (used when an additional-tendency hook is set).
-void lsrk_unfused_step(omega_stepper_s *St, omega_state_s *S, double t, double dt, hipStream_t s) {
-  omega_model_s *M = St->model;
-  for (int stage = 0; stage < St->nstages; ++stage) {
-    for (int f = 0; f < 3; ++f) launch_scale(St->tend[f].p, St->rka[stage], S->n[f], s);
-    const double stagetime = t + St->rkc[stage] * dt;
-    compute_tendency_ptrs(M, St->tend[0].p, St->tend[1].p, St->tend[2].p, S->ptr(0), S->ptr(1),
-                          S->ptr(2), stagetime, OMEGA_ADD_INCREMENT, s);
-    const double db = dt * St->rkb[stage];
-    for (int f = 0; f < 3; ++f) launch_axpy(S->ptr(f), db, St->tend[f].p, S->n[f], s);
-  }
-}
-
-void step_once(omega_stepper_s *St, omega_state_s *S, double t, double dt, hipStream_t s) {
-  const bool fused = St->model->add_fn == nullptr;
-  if (St->kind == OMEGA_STEPPER_RK4) {
-    if (fused) rk4_fused_step(St, S, dt, s);
-    else rk4_unfused_step(St, S, t, dt, s);
-  } else {
-    if (fused) lsrk_fused_step(St, S, dt, s);
-    else lsrk_unfused_step(St, S, t, dt, s);
-  }
-}
-
-void do_step_maybe_graph(omega_stepper_s *St, omega_state_s *S, double t, double dt,
-                         hipStream_t s) {
-  const bool graphable = St->use_graph && St->model->add_fn == nullptr && !stream_capturing(s);
-  if (!graphable) {
-    step_once(St, S, t, dt, s);
-    return;
-  }
-  if (St->kind == OMEGA_STEPPER_LSRK) {
-    // make sure both buffers exist before capture
-    for (int f = 0; f < 3; ++f) (void)S->alt(f);
-  }
-  const GraphKey key{dt, S, S->cur};
-  auto it = St->graphs.find(key);
-  if (it == St->graphs.end()) {
-    // every buffer the step touches is allocated before the capture starts
-    if (St->kind == OMEGA_STEPPER_RK4) {
-      ensure3(St->acc, S);
-      ensure3(St->p1, S);
-      ensure3(St->p2, S);
-    }
-    if (St->model->terms & T_VDEL4) ensure_vdel4(St->model);
-    // capture on a private stream (the null stream cannot be captured); the
-    // graph is then launched on the caller's stream
-    if (!St->cap_stream)
-      OMEGA_HIP_CHECK(hipStreamCreateWithFlags(&St->cap_stream, hipStreamNonBlocking));
-    hipStream_t cs = St->cap_stream;
-    const int cur0 = S->cur;
-    hipGraph_t g;
-    OMEGA_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    try {
-      step_once(St, S, t, dt, cs);
-    } catch (...) {
-      hipGraph_t dummy;
-      (void)hipStreamEndCapture(cs, &dummy);
-      S->cur = cur0;
-      throw;
-    }
-    OMEGA_HIP_CHECK(hipStreamEndCapture(cs, &g));
-    hipGraphExec_t ex;
-    OMEGA_HIP_CHECK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
-    OMEGA_HIP_CHECK(hipGraphDestroy(g));
-    S->cur = cur0; // capture recorded the flips; replay applies them below
-    it = St->graphs.emplace(key, ex).first;
-  }
-  OMEGA_HIP_CHECK(hipGraphLaunch(it->second, s));
-  if (St->kind == OMEGA_STEPPER_LSRK && (St->nstages % 2 == 1)) S->cur = 1 - S->cur;
-}
-
-} // namespace
 
 // ---------------------------------------------------------------------------
 // ABI

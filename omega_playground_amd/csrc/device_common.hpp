@@ -103,6 +103,14 @@ __device__ __forceinline__ void epilogue(const StageArgs &a, const FieldIO &f, s
     st<N>(f.out + idx, out);
     break;
   }
+  case EPI_PROVIS: {
+    const Vd<N> yo = ld<N>(f.yold + idx);
+    Vd<N> out;
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] = yo[i] + a.da * T[i];
+    st<N>(f.out + idx, out);
+    break;
+  }
   }
 }
 

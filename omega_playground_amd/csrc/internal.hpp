@@ -179,6 +179,7 @@ enum Epilogue : int {
   EPI_RK4_MID = 2,   // acc += db*T; pout = yold + da*T
   EPI_RK4_LAST = 3,  // y = acc + db*T
   EPI_LSRK = 4,      // tend = base + F; yout = yin + db*T
+  EPI_PROVIS = 5,    // pout = yold + da*T (stage recomputation for the adjoint)
 };
 
 // Buffers of one prognostic field (h, vn, or the tracer block) for a stage.

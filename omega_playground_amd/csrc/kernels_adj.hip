@@ -1,0 +1,662 @@
+// Discrete adjoint (vector-Jacobian product) of the tendency, gather-only.
+//
+// Given a state y and a cotangent k of the tendency F(y), these kernels add
+// J(y)^T k to a state cotangent and accumulate the gradient with respect to
+// the four mixing coefficients (visc_del2, visc_del4, eddy_diff2,
+// eddy_diff4).  Every forward scatter (an edge quantity feeding its two
+// cells, a vertex quantity feeding its three edges, an edge feeding the ten
+// edges that list it in edges_on_edge) becomes a gather over the transposed
+// connectivity, so each output element is written by exactly one thread,
+// no atomics are used and the result is bit-reproducible.  The parameter
+// gradients are reduced through fixed per-block partials.
+//
+// Notation (forward, reference file:line):
+//   zeta_v = sum crv u            (vorticity_aux.hpp:94-106)
+//   theta_v = sum cth h, iota = 1/theta, q = iota zeta, p = iota f
+//   K_c = sum cke u u /4, D_c = sum cdiv u   (kinetic_aux.hpp:53-65)
+//   F_e = (h_c0 + h_c1)/2, Q_e = (q_v0 + q_v1)/2, P_e likewise
+//   nu_lc = tr_lc / h_c                      (tracer_aux.hpp:39-44)
+//   G_lc = sum cdiv F (nu_c1 - nu_c0)/dc     (tracer_hyperdiff_cell.hpp:94-129)
+//   h_t = -sum cdiv F u                      (thickness_hadv_cell.hpp:48-72)
+//   u_t = sum W F_j u_j (Q_e+P_e+Q_j+P_j)/2 - dK/dc - g dh/dc
+//         + cv2 [dD/dc - dzeta/dv] - cv4 [dD2/dc - dZ2/dv]
+//                                  (pv_flux_edge.hpp, ke_grad_edge.hpp,
+//                                   ssh_grad_edge.hpp, vel_diff_edge.hpp,
+//                                   vel_hyperdiff_edge.hpp)
+//   tr_t = -sum cdiv F nu_e u + sum ct2 F s2 dnu/dc - sum ct4 dG/dc
+//                                  (tracer_hadv/diff/hyperdiff_cell.hpp)
+#include <algorithm>
+
+#include "adjoint.hpp"
+#include "device_common.hpp"
+
+namespace omega_hip {
+
+namespace {
+
+using namespace dev;
+
+constexpr int kAB = 256; // threads per block (lanes x rows)
+
+template <int N>
+__device__ __forceinline__ Vd<N> rd(const double *__restrict__ p, int e, int L, int k) {
+  return ld<N>(p + (size_t)e * L + k);
+}
+
+// Block-wide sum of two values into two partial slots (all threads call it).
+__device__ void block_partials2(double a, double b, double *out_a, double *out_b) {
+  __shared__ double sa[kAB], sb[kAB];
+  const int t = threadIdx.y * blockDim.x + threadIdx.x;
+  const int n = blockDim.x * blockDim.y;
+  sa[t] = a;
+  sb[t] = b;
+  __syncthreads();
+  for (int s = kAB / 2; s > 0; s >>= 1) {
+    if (t < s && t + s < n) {
+      sa[t] = sa[t] + sa[t + s];
+      sb[t] = sb[t] + sb[t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    *out_a = sa[0];
+    *out_b = sb[0];
+  }
+}
+
+struct Seg {
+  int c0, c1, k;
+  bool active;
+};
+
+template <int N> __device__ __forceinline__ Seg segment(const MeshView &m, int swz) {
+  Seg g;
+  const int s = logical_block(blockIdx.x, gridDim.x, swz);
+  g.c0 = s * (int)blockDim.y;
+  g.c1 = min(g.c0 + (int)blockDim.y, m.nc);
+  g.k = (blockIdx.y * blockDim.x + threadIdx.x) * N;
+  g.active = g.c0 < m.nc && g.k < m.L;
+  return g;
+}
+
+template <int N>
+__device__ __forceinline__ double mask(int k, int i, int ml, double c) {
+  return (k + i < ml) ? c : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Momentum del4 chain: cotangents of the del2 divergence / vorticity, then
+// of the del2 edge field (vel_hyperdiff_edge.hpp:140-237 transposed)
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(kAB)
+    k_adj_v4a(MeshView m, ModelCoef mc, AdjArgs a, AdjScratch s, int swz) {
+  const Seg g = segment<N>(m, swz);
+  if (!g.active) return;
+  const int L = m.L, k = g.k;
+  for (int v = m.vofs[g.c0] + (int)threadIdx.y; v < m.vofs[g.c1]; v += blockDim.y) {
+    Vd<N> z = zero<N>();
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int e = m.eov[3 * v + j];
+      const double sg = (m.voe[2 * e + 1] == v) ? 1.0 : -1.0;
+      const Vd<N> ut = rd<N>(a.c.vn, e, L, k);
+      const int ml = m.maxlev_e[e];
+#pragma unroll
+      for (int i = 0; i < N; ++i) z[i] += sg * mask<N>(k, i, ml, mc.cv4[e]) * ut[i] * m.inv_dv[e];
+    }
+    st<N>(s.z2bar_v + (size_t)v * L + k, z);
+  }
+  for (int c = g.c0 + (int)threadIdx.y; c < g.c1; c += blockDim.y) {
+    Vd<N> d2 = zero<N>();
+    for (int j = 0; j < m.noc[c]; ++j) {
+      const int e = m.eoc[c * m.me + j];
+      const double sg = (m.coe[2 * e + 1] == c) ? -1.0 : 1.0;
+      const Vd<N> ut = rd<N>(a.c.vn, e, L, k);
+      const int ml = m.maxlev_e[e];
+#pragma unroll
+      for (int i = 0; i < N; ++i) d2[i] += sg * mask<N>(k, i, ml, mc.cv4[e]) * ut[i] * m.inv_dc[e];
+    }
+    st<N>(s.d2bar_c + (size_t)c * L + k, d2);
+  }
+}
+
+template <int N>
+__global__ void __launch_bounds__(kAB)
+    k_adj_v4b(MeshView m, AdjMeshView am, AdjScratch s, int swz) {
+  const Seg g = segment<N>(m, swz);
+  if (!g.active) return;
+  const int L = m.L, k = g.k;
+  for (int e = m.eofs[g.c0] + (int)threadIdx.y; e < m.eofs[g.c1]; e += blockDim.y) {
+    const int c0 = m.coe[2 * e], c1 = m.coe[2 * e + 1], v0 = m.voe[2 * e], v1 = m.voe[2 * e + 1];
+    const Vd<N> z0 = rd<N>(s.z2bar_v, v0, L, k), z1 = rd<N>(s.z2bar_v, v1, L, k);
+    const Vd<N> d0 = rd<N>(s.d2bar_c, c0, L, k), d1 = rd<N>(s.d2bar_c, c1, L, k);
+    Vd<N> out;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+      out[i] = am.ecrv[2 * e] * z0[i] + am.ecrv[2 * e + 1] * z1[i] + am.ecd[2 * e] * d0[i] +
+               am.ecd[2 * e + 1] * d1[i];
+    st<N>(s.e2bar_e + (size_t)e * L + k, out);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// PV flux: cotangent of S_e = Q_e + P_e, from the edge's own sum and from the
+// sums of the edges that list it (transposed edges_on_edge)
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(kAB)
+    k_adj_pv(MeshView m, AdjMeshView am, AdjArgs a, AdjScratch s, int swz) {
+  const Seg g = segment<N>(m, swz);
+  if (!g.active) return;
+  const int L = m.L, k = g.k;
+  for (int e = m.eofs[g.c0] + (int)threadIdx.y; e < m.eofs[g.c1]; e += blockDim.y) {
+    const Vd<N> ue = rd<N>(a.vn, e, L, k);
+    const Vd<N> ute = rd<N>(a.c.vn, e, L, k);
+    const Vd<N> he0 = rd<N>(a.h, m.coe[2 * e], L, k), he1 = rd<N>(a.h, m.coe[2 * e + 1], L, k);
+    Vd<N> own = zero<N>(), oth = zero<N>();
+    for (int j = 0; j < m.noe[e]; ++j) {
+      const int ej = m.eoe[e * m.me2 + j];
+      const double w = m.w[e * m.me2 + j];
+      const Vd<N> h0 = rd<N>(a.h, m.coe[2 * ej], L, k), h1 = rd<N>(a.h, m.coe[2 * ej + 1], L, k);
+      const Vd<N> uj = rd<N>(a.vn, ej, L, k);
+#pragma unroll
+      for (int i = 0; i < N; ++i) own[i] += w * (0.5 * (h0[i] + h1[i])) * uj[i];
+    }
+    for (int q = am.tofs[e]; q < am.tofs[e + 1]; ++q) {
+      const int t = am.tedge[q];
+      const double w = am.tw[q];
+      const Vd<N> ut = rd<N>(a.c.vn, t, L, k);
+#pragma unroll
+      for (int i = 0; i < N; ++i) oth[i] += ut[i] * w;
+    }
+    Vd<N> out;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const double Fe = 0.5 * (he0[i] + he1[i]);
+      out[i] = 0.5 * (ute[i] * own[i] + oth[i] * Fe * ue[i]);
+    }
+    st<N>(s.sbar_e + (size_t)e * L + k, out);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Vertex and cell diagnostics cotangents
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(kAB)
+    k_adj_vc(MeshView m, ModelCoef mc, AdjCoef ac, AdjArgs a, DiagOut d, AdjScratch s, int swz) {
+  const Seg g = segment<N>(m, swz);
+  if (!g.active) return;
+  const int L = m.L, k = g.k, NC = m.nc;
+  const bool pv = a.do_vn && (a.terms & T_PV), vd2 = a.do_vn && (a.terms & T_VDEL2),
+             vd4 = a.do_vn && (a.terms & T_VDEL4), ke = a.do_vn && (a.terms & T_KE);
+  if (pv || vd2 || vd4) {
+    for (int v = m.vofs[g.c0] + (int)threadIdx.y; v < m.vofs[g.c1]; v += blockDim.y) {
+      Vd<N> qb = zero<N>(), zb = zero<N>();
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int e = m.eov[3 * v + j];
+        const double sg = (m.voe[2 * e + 1] == v) ? -1.0 : 1.0;
+        if (pv) {
+          const Vd<N> sb = rd<N>(s.sbar_e, e, L, k);
+#pragma unroll
+          for (int i = 0; i < N; ++i) qb[i] += 0.5 * sb[i];
+        }
+        if (vd2) {
+          const Vd<N> ut = rd<N>(a.c.vn, e, L, k);
+          const int ml = m.maxlev_e[e];
+#pragma unroll
+          for (int i = 0; i < N; ++i) zb[i] += sg * mask<N>(k, i, ml, mc.cv2[e]) * ut[i] * m.inv_dv[e];
+        }
+        if (vd4) {
+          const Vd<N> eb = rd<N>(s.e2bar_e, e, L, k);
+#pragma unroll
+          for (int i = 0; i < N; ++i) zb[i] += sg * eb[i] * m.inv_dv4[e];
+        }
+      }
+      Vd<N> tb = zero<N>();
+      if (pv) {
+        const Vd<N> io = rd<N>(d.inv_thick_v, v, L, k), z = rd<N>(d.rvort_v, v, L, k);
+        const double f = mc.f_v[v];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          zb[i] += io[i] * qb[i];
+          tb[i] = -io[i] * io[i] * (z[i] + f) * qb[i];
+        }
+      }
+      st<N>(s.zbar_v + (size_t)v * L + k, zb);
+      st<N>(s.tbar_v + (size_t)v * L + k, tb);
+    }
+  }
+  const bool td4 = (a.terms & T_TDEL4) && a.ntr > 0;
+  for (int c = g.c0 + (int)threadIdx.y; c < g.c1; c += blockDim.y) {
+    Vd<N> kb = zero<N>(), db = zero<N>();
+    const int n = m.noc[c];
+    for (int j = 0; j < n; ++j) {
+      const int e = m.eoc[c * m.me + j];
+      const bool first = m.coe[2 * e] == c;
+      if (ke || vd2) {
+        const Vd<N> ut = rd<N>(a.c.vn, e, L, k);
+        const int ml = m.maxlev_e[e];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          if (ke) kb[i] += (first ? 1.0 : -1.0) * ut[i] * m.inv_dc[e];
+          if (vd2) db[i] += (first ? -1.0 : 1.0) * mask<N>(k, i, ml, mc.cv2[e]) * ut[i] * m.inv_dc[e];
+        }
+      }
+      if (vd4) {
+        const Vd<N> eb = rd<N>(s.e2bar_e, e, L, k);
+#pragma unroll
+        for (int i = 0; i < N; ++i) db[i] += (first ? -1.0 : 1.0) * eb[i] * m.inv_dc[e];
+      }
+    }
+    if (ke) st<N>(s.kbar_c + (size_t)c * L + k, kb);
+    if (vd2 || vd4) st<N>(s.dbar_c + (size_t)c * L + k, db);
+    if (td4) {
+      for (int l = 0; l < a.ntr; ++l) {
+        const double *tl = a.c.tr + (size_t)l * NC * L;
+        Vd<N> gb = zero<N>();
+        for (int j = 0; j < n; ++j) {
+          const int e = m.eoc[c * m.me + j];
+          const int a0 = m.coe[2 * e], a1 = m.coe[2 * e + 1];
+          const double sg = (a1 == c) ? -1.0 : 1.0;
+          const Vd<N> t0 = rd<N>(tl, a0, L, k), t1 = rd<N>(tl, a1, L, k);
+#pragma unroll
+          for (int i = 0; i < N; ++i)
+            gb[i] += sg * (ac.ect4[2 * e] * t0[i] + ac.ect4[2 * e + 1] * t1[i]) * m.inv_dc[e];
+        }
+        st<N>(s.gbar_c + ((size_t)l * NC + c) * L + k, gb);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Edge pass: velocity cotangent (accumulated) and edge-thickness cotangent
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(kAB)
+    k_adj_edge(MeshView m, AdjMeshView am, ModelCoef mc, AdjCoef ac, AdjArgs a, DiagOut d,
+               AdjScratch s, int swz) {
+  const Seg g = segment<N>(m, swz);
+  const int L = m.L, k = g.k, NC = m.nc;
+  const bool pv = a.do_vn && (a.terms & T_PV), vd2 = a.do_vn && (a.terms & T_VDEL2),
+             vd4 = a.do_vn && (a.terms & T_VDEL4), ke = a.do_vn && (a.terms & T_KE);
+  const bool td2 = a.terms & T_TDEL2, td4 = a.terms & T_TDEL4;
+  double p_v2 = 0.0, p_v4 = 0.0;
+  if (g.active) {
+    for (int e = m.eofs[g.c0] + (int)threadIdx.y; e < m.eofs[g.c1]; e += blockDim.y) {
+      const int c0 = m.coe[2 * e], c1 = m.coe[2 * e + 1], v0 = m.voe[2 * e], v1 = m.voe[2 * e + 1];
+      const double cd0 = am.ecd[2 * e], cd1 = am.ecd[2 * e + 1], idc = m.inv_dc[e];
+      const Vd<N> ue = rd<N>(a.vn, e, L, k);
+      const Vd<N> h0 = rd<N>(a.h, c0, L, k), h1 = rd<N>(a.h, c1, L, k);
+      Vd<N> F, ub = zero<N>(), fb = zero<N>();
+#pragma unroll
+      for (int i = 0; i < N; ++i) F[i] = 0.5 * (h0[i] + h1[i]);
+      if (a.do_h) {
+        const Vd<N> g0 = rd<N>(a.c.h, c0, L, k), g1 = rd<N>(a.c.h, c1, L, k);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          const double hc = cd0 * g0[i] + cd1 * g1[i];
+          ub[i] -= F[i] * hc;
+          fb[i] -= ue[i] * hc;
+        }
+      }
+      for (int l = 0; l < a.ntr; ++l) {
+        const size_t off = (size_t)l * NC * L;
+        const Vd<N> t0 = rd<N>(a.tr + off, c0, L, k), t1 = rd<N>(a.tr + off, c1, L, k);
+        const Vd<N> b0 = rd<N>(a.c.tr + off, c0, L, k), b1 = rd<N>(a.c.tr + off, c1, L, k);
+        Vd<N> g0 = zero<N>(), g1 = zero<N>();
+        if (td4) {
+          g0 = rd<N>(s.gbar_c + off, c0, L, k);
+          g1 = rd<N>(s.gbar_c + off, c1, L, k);
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          const double n0 = t0[i] * (1.0 / h0[i]), n1 = t1[i] * (1.0 / h1[i]);
+          const double ne = (n0 + n1) * 0.5, grad = (n1 - n0) * idc;
+          const double w = cd0 * b0[i] + cd1 * b1[i];
+          ub[i] -= w * F[i] * ne;
+          fb[i] -= w * ue[i] * ne;
+          if (td2) fb[i] += (ac.ect2[2 * e] * b0[i] + ac.ect2[2 * e + 1] * b1[i]) * m.s2[e] * grad;
+          if (td4) fb[i] += (cd0 * g0[i] + cd1 * g1[i]) * grad;
+        }
+      }
+      if (a.do_vn) {
+        if (pv) {
+          // S_e = Q_e + P_e of this edge
+          const double fv0 = mc.f_v[v0], fv1 = mc.f_v[v1];
+          const Vd<N> i0 = rd<N>(d.inv_thick_v, v0, L, k), i1 = rd<N>(d.inv_thick_v, v1, L, k);
+          const Vd<N> z0 = rd<N>(d.rvort_v, v0, L, k), z1 = rd<N>(d.rvort_v, v1, L, k);
+          Vd<N> Se;
+#pragma unroll
+          for (int i = 0; i < N; ++i)
+            Se[i] = 0.5 * (i0[i] * z0[i] + i1[i] * z1[i]) + 0.5 * (i0[i] * fv0 + i1[i] * fv1);
+          for (int q = am.tofs[e]; q < am.tofs[e + 1]; ++q) {
+            const int t = am.tedge[q];
+            const double w = am.tw[q];
+            const int b0 = m.voe[2 * t], b1 = m.voe[2 * t + 1];
+            const double ft0 = mc.f_v[b0], ft1 = mc.f_v[b1];
+            const Vd<N> j0 = rd<N>(d.inv_thick_v, b0, L, k), j1 = rd<N>(d.inv_thick_v, b1, L, k);
+            const Vd<N> y0 = rd<N>(d.rvort_v, b0, L, k), y1 = rd<N>(d.rvort_v, b1, L, k);
+            const Vd<N> ut = rd<N>(a.c.vn, t, L, k);
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+              const double St =
+                  0.5 * (j0[i] * y0[i] + j1[i] * y1[i]) + 0.5 * (j0[i] * ft0 + j1[i] * ft1);
+              const double nv = (St + Se[i]) * 0.5;
+              ub[i] += ut[i] * w * F[i] * nv;
+              fb[i] += ut[i] * w * ue[i] * nv;
+            }
+          }
+        }
+        if (pv || vd2 || vd4) {
+          const Vd<N> zb0 = rd<N>(s.zbar_v, v0, L, k), zb1 = rd<N>(s.zbar_v, v1, L, k);
+#pragma unroll
+          for (int i = 0; i < N; ++i) ub[i] += am.ecrv[2 * e] * zb0[i] + am.ecrv[2 * e + 1] * zb1[i];
+        }
+        if (ke) {
+          const Vd<N> k0 = rd<N>(s.kbar_c, c0, L, k), k1 = rd<N>(s.kbar_c, c1, L, k);
+#pragma unroll
+          for (int i = 0; i < N; ++i)
+            ub[i] += (am.ecke[2 * e] * k0[i] + am.ecke[2 * e + 1] * k1[i]) * ue[i] * 0.5;
+        }
+        if (vd2 || vd4) {
+          const Vd<N> d0 = rd<N>(s.dbar_c, c0, L, k), d1 = rd<N>(s.dbar_c, c1, L, k);
+#pragma unroll
+          for (int i = 0; i < N; ++i) ub[i] += cd0 * d0[i] + cd1 * d1[i];
+        }
+        const Vd<N> ut = rd<N>(a.c.vn, e, L, k);
+        const int ml = m.maxlev_e[e];
+        if (vd2) {
+          const Vd<N> D0 = rd<N>(d.div_c, c0, L, k), D1 = rd<N>(d.div_c, c1, L, k);
+          const Vd<N> Z0 = rd<N>(d.rvort_v, v0, L, k), Z1 = rd<N>(d.rvort_v, v1, L, k);
+#pragma unroll
+          for (int i = 0; i < N; ++i)
+            p_v2 += ut[i] * mask<N>(k, i, ml, m.s2[e]) *
+                    ((D1[i] - D0[i]) * idc - (Z1[i] - Z0[i]) * m.inv_dv[e]);
+        }
+        if (vd4) {
+          const Vd<N> D0 = rd<N>(d.vdel2_div_c, c0, L, k), D1 = rd<N>(d.vdel2_div_c, c1, L, k);
+          const Vd<N> Z0 = rd<N>(d.vdel2_rv_v, v0, L, k), Z1 = rd<N>(d.vdel2_rv_v, v1, L, k);
+#pragma unroll
+          for (int i = 0; i < N; ++i)
+            p_v4 -= ut[i] * mask<N>(k, i, ml, m.s4[e]) *
+                    ((D1[i] - D0[i]) * idc - (Z1[i] - Z0[i]) * m.inv_dv[e]);
+        }
+      }
+      Vd<N> acc = rd<N>(a.c.ovn, e, L, k);
+#pragma unroll
+      for (int i = 0; i < N; ++i) acc[i] += ub[i];
+      st<N>(a.c.ovn + (size_t)e * L + k, acc);
+      st<N>(s.fbar_e + (size_t)e * L + k, fb);
+    }
+  }
+  const int b = blockIdx.y * gridDim.x + blockIdx.x;
+  block_partials2(p_v2, p_v4, s.partial + b, s.partial + a.nparts + b);
+}
+
+// ---------------------------------------------------------------------------
+// Cell pass: thickness and tracer cotangents (accumulated)
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ void __launch_bounds__(kAB)
+    k_adj_cell(MeshView m, AdjMeshView am, ModelCoef mc, AdjCoef ac, AdjArgs a, DiagOut d,
+               AdjScratch s, int swz) {
+  const Seg g = segment<N>(m, swz);
+  const int L = m.L, k = g.k, NC = m.nc, me = m.me;
+  const bool pv = a.do_vn && (a.terms & T_PV), ssh = a.do_vn && (a.terms & T_SSH);
+  const bool td2 = a.terms & T_TDEL2, td4 = a.terms & T_TDEL4;
+  double p_t2 = 0.0, p_t4 = 0.0;
+  if (g.active) {
+    for (int c = g.c0 + (int)threadIdx.y; c < g.c1; c += blockDim.y) {
+      const int n = m.noc[c];
+      const Vd<N> hc = rd<N>(a.h, c, L, k);
+      Vd<N> hb = zero<N>();
+      for (int j = 0; j < n; ++j) {
+        const int e = m.eoc[c * me + j];
+        const Vd<N> fb = rd<N>(s.fbar_e, e, L, k);
+#pragma unroll
+        for (int i = 0; i < N; ++i) hb[i] += 0.5 * fb[i];
+        if (ssh) {
+          const double sg = (m.coe[2 * e] == c) ? 1.0 : -1.0;
+          const Vd<N> ut = rd<N>(a.c.vn, e, L, k);
+#pragma unroll
+          for (int i = 0; i < N; ++i) hb[i] += mc.grav * sg * ut[i] * m.inv_dc[e];
+        }
+        if (pv) {
+          const int v = am.voc[c * me + j];
+          const Vd<N> tb = rd<N>(s.tbar_v, v, L, k);
+#pragma unroll
+          for (int i = 0; i < N; ++i) hb[i] += am.ccth[c * me + j] * tb[i];
+        }
+      }
+      for (int l = 0; l < a.ntr; ++l) {
+        const size_t off = (size_t)l * NC * L;
+        const Vd<N> tcl = rd<N>(a.tr + off, c, L, k);
+        const Vd<N> bcl = rd<N>(a.c.tr + off, c, L, k);
+        Vd<N> nb = zero<N>(), t2 = zero<N>(), t4 = zero<N>();
+        for (int j = 0; j < n; ++j) {
+          const int e = m.eoc[c * me + j];
+          const int a0 = m.coe[2 * e], a1 = m.coe[2 * e + 1];
+          const double sg = (a1 == c) ? 1.0 : -1.0;
+          const double cd0 = am.ecd[2 * e], cd1 = am.ecd[2 * e + 1], idc = m.inv_dc[e];
+          const Vd<N> h0 = rd<N>(a.h, a0, L, k), h1 = rd<N>(a.h, a1, L, k);
+          const Vd<N> ue = rd<N>(a.vn, e, L, k);
+          const Vd<N> b0 = rd<N>(a.c.tr + off, a0, L, k), b1 = rd<N>(a.c.tr + off, a1, L, k);
+          const Vd<N> tt0 = rd<N>(a.tr + off, a0, L, k), tt1 = rd<N>(a.tr + off, a1, L, k);
+          Vd<N> g0 = zero<N>(), g1 = zero<N>(), G0 = zero<N>(), G1 = zero<N>();
+          if (td4) {
+            g0 = rd<N>(s.gbar_c + off, a0, L, k);
+            g1 = rd<N>(s.gbar_c + off, a1, L, k);
+            G0 = rd<N>(d.trdel2_c + off, a0, L, k);
+            G1 = rd<N>(d.trdel2_c + off, a1, L, k);
+          }
+          const double cdc = m.cdiv[c * me + j];
+#pragma unroll
+          for (int i = 0; i < N; ++i) {
+            const double F = 0.5 * (h0[i] + h1[i]);
+            const double w = cd0 * b0[i] + cd1 * b1[i];
+            nb[i] += 0.5 * (-w * F * ue[i]);
+            const double grad = (tt1[i] * (1.0 / h1[i]) - tt0[i] * (1.0 / h0[i])) * idc;
+            if (td2) {
+              const double w2 = ac.ect2[2 * e] * b0[i] + ac.ect2[2 * e + 1] * b1[i];
+              nb[i] += sg * w2 * F * m.s2[e] * idc;
+              t2[i] += cdc * F * m.s2[e] * grad;
+            }
+            if (td4) {
+              const double wg = cd0 * g0[i] + cd1 * g1[i];
+              nb[i] += sg * wg * F * idc;
+              t4[i] -= cdc * m.s4[e] * (G1[i] - G0[i]) * idc;
+            }
+          }
+        }
+        Vd<N> acc = rd<N>(a.c.otr + off, c, L, k);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          const double ih = 1.0 / hc[i];
+          acc[i] += nb[i] * ih;
+          hb[i] -= nb[i] * tcl[i] * ih * ih;
+          p_t2 += bcl[i] * t2[i];
+          p_t4 += bcl[i] * t4[i];
+        }
+        st<N>(a.c.otr + off + (size_t)c * L + k, acc);
+      }
+      Vd<N> acc = rd<N>(a.c.oh, c, L, k);
+#pragma unroll
+      for (int i = 0; i < N; ++i) acc[i] += hb[i];
+      st<N>(a.c.oh + (size_t)c * L + k, acc);
+    }
+  }
+  const int b = blockIdx.y * gridDim.x + blockIdx.x;
+  block_partials2(p_t2, p_t4, s.partial + 2 * a.nparts + b, s.partial + 3 * a.nparts + b);
+}
+
+// pgrad[p] += sum of the block partials of parameter p, in a fixed order
+__global__ void __launch_bounds__(kAB) k_adj_param_reduce(double *partial, int nparts,
+                                                         int nused, double *pgrad) {
+  __shared__ double sh[kAB];
+  const int p = blockIdx.x;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nused; i += kAB) acc += partial[p * nparts + i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kAB / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) pgrad[p] += sh[0];
+}
+
+struct AGeo {
+  dim3 grid, block;
+  int N;
+};
+
+AGeo adj_geometry(const MeshView &m, const Launch &lc) {
+  const int N = (m.L % 2 == 0) ? 2 : 1;
+  int lanes = (m.L + N - 1) / N;
+  if (lanes > 64) lanes = 64;
+  int lx = 1;
+  while (lx < lanes) lx <<= 1;
+  const int rows = kAB / lx;
+  const int chunks = (m.L + lx * N - 1) / (lx * N);
+  return AGeo{dim3((m.nc + rows - 1) / rows, chunks), dim3(lx, rows), N};
+}
+
+} // namespace
+
+int adj_partial_slots(const MeshView &m, const Launch &lc) {
+  const AGeo g = adj_geometry(m, lc);
+  return (int)(g.grid.x * g.grid.y);
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise helpers and the least-squares misfit
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kEB = 256;
+constexpr int kMisfitMaxBlocks = 1024;
+
+__global__ void __launch_bounds__(kEB) k_fill(double *y, double v, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)kEB + threadIdx.x; i < n; i += (size_t)gridDim.x * kEB)
+    y[i] = v;
+}
+
+__global__ void __launch_bounds__(kEB)
+    k_lincomb(double *y, double alpha, const double *x, double beta, const double *z, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)kEB + threadIdx.x; i < n; i += (size_t)gridDim.x * kEB)
+    y[i] = alpha * x[i] + beta * z[i];
+}
+
+__device__ void block_sum_store(double v, double *out) {
+  __shared__ double sh[kEB];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kEB / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
+}
+
+__global__ void __launch_bounds__(kEB)
+    k_misfit(const double *y, const double *obs, const double *area, int nent, int L, size_t n,
+             double w, double *cot, double *partial) {
+  double acc = 0.0;
+  for (size_t i = blockIdx.x * (size_t)kEB + threadIdx.x; i < n; i += (size_t)gridDim.x * kEB) {
+    const int ent = (int)((i / L) % nent);
+    const double wa = w * area[ent];
+    const double r = y[i] - obs[i];
+    cot[i] = wa * r;
+    acc += 0.5 * wa * r * r;
+  }
+  block_sum_store(acc, partial + blockIdx.x);
+}
+
+__global__ void __launch_bounds__(kEB)
+    k_sum(const double *partial, int n, double *out, int accumulate) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += kEB) acc += partial[i];
+  __shared__ double sh[kEB];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kEB / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = accumulate ? *out + sh[0] : sh[0];
+}
+
+unsigned ew_blocks(size_t n) {
+  const size_t b = (n + kEB - 1) / kEB;
+  return (unsigned)std::max<size_t>(1, std::min<size_t>(b, 8192));
+}
+
+} // namespace
+
+void launch_fill(Real *y, Real v, size_t n, hipStream_t s) {
+  if (!n) return;
+  TimerScope ts("update", s);
+  hipLaunchKernelGGL(k_fill, dim3(ew_blocks(n)), dim3(kEB), 0, s, y, v, n);
+  OMEGA_HIP_CHECK(hipGetLastError());
+}
+
+void launch_lincomb(Real *y, Real alpha, const Real *x, Real beta, const Real *z, size_t n,
+                    hipStream_t s) {
+  if (!n) return;
+  TimerScope ts("update", s);
+  hipLaunchKernelGGL(k_lincomb, dim3(ew_blocks(n)), dim3(kEB), 0, s, y, alpha, x, beta, z, n);
+  OMEGA_HIP_CHECK(hipGetLastError());
+}
+
+int misfit_blocks(size_t n) {
+  const size_t b = (n + kEB - 1) / kEB;
+  return (int)std::max<size_t>(1, std::min<size_t>(b, kMisfitMaxBlocks));
+}
+
+void launch_misfit(const Real *y, const Real *obs, const Real *area, int nent, int L, int nrep,
+                   Real w, Real *cot, Real *partial, hipStream_t s) {
+  const size_t n = (size_t)nrep * nent * L;
+  TimerScope ts("reduce", s);
+  hipLaunchKernelGGL(k_misfit, dim3(misfit_blocks(n)), dim3(kEB), 0, s, y, obs, area, nent, L,
+                     n, w, cot, partial);
+  OMEGA_HIP_CHECK(hipGetLastError());
+}
+
+void launch_sum(const Real *partial, int n, Real *out, int accumulate, hipStream_t s) {
+  TimerScope ts("reduce", s);
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(kEB), 0, s, partial, n, out, accumulate);
+  OMEGA_HIP_CHECK(hipGetLastError());
+}
+
+void launch_tendency_vjp(const MeshView &m, const AdjMeshView &am, const Launch &lc,
+                         const ModelCoef &mc, const AdjCoef &ac, const AdjArgs &a,
+                         const DiagOut &d, const AdjScratch &s, hipStream_t st) {
+  TimerScope ts("adjoint", st);
+  const AGeo g = adj_geometry(m, lc);
+  const int swz = lc.swizzle;
+  const bool vd4 = a.do_vn && (a.terms & T_VDEL4);
+  const bool pv = a.do_vn && (a.terms & T_PV);
+#define OMEGA_ADJ_LAUNCH(K, ...)                                                                \
+  do {                                                                                         \
+    if (g.N == 2) hipLaunchKernelGGL(K<2>, g.grid, g.block, 0, st, __VA_ARGS__);               \
+    else hipLaunchKernelGGL(K<1>, g.grid, g.block, 0, st, __VA_ARGS__);                        \
+  } while (0)
+  if (vd4) {
+    OMEGA_ADJ_LAUNCH(k_adj_v4a, m, mc, a, s, swz);
+    OMEGA_ADJ_LAUNCH(k_adj_v4b, m, am, s, swz);
+  }
+  if (pv) OMEGA_ADJ_LAUNCH(k_adj_pv, m, am, a, s, swz);
+  OMEGA_ADJ_LAUNCH(k_adj_vc, m, mc, ac, a, d, s, swz);
+  OMEGA_ADJ_LAUNCH(k_adj_edge, m, am, mc, ac, a, d, s, swz);
+  OMEGA_ADJ_LAUNCH(k_adj_cell, m, am, mc, ac, a, d, s, swz);
+#undef OMEGA_ADJ_LAUNCH
+  hipLaunchKernelGGL(k_adj_param_reduce, dim3(4), dim3(kAB), 0, st, s.partial, a.nparts,
+                     (int)(g.grid.x * g.grid.y), s.pgrad);
+  OMEGA_HIP_CHECK(hipGetLastError());
+}
+
+} // namespace omega_hip

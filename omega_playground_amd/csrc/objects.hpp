@@ -1,0 +1,145 @@
+// Library objects behind the opaque C ABI handles, and the host-side
+// orchestration shared by the ABI, the steppers and the adjoint.
+#pragma once
+
+#include <array>
+#include <map>
+#include <string>
+
+#include "internal.hpp"
+
+using omega_hip::DBuf;
+using omega_hip::DeviceMesh;
+using omega_hip::Launch;
+using omega_hip::ModelCoef;
+using omega_hip::DiagOut;
+using omega_hip::Real;
+
+// ---------------------------------------------------------------------------
+// Opaque objects
+// ---------------------------------------------------------------------------
+#include "adjoint.hpp"
+
+struct omega_mesh_s {
+  DeviceMesh dm;
+  DBuf<Real> staging;             // device scratch for omega_add_host_field
+  omega_hip::AdjLayout adj;       // transposed tables, built on first adjoint use
+};
+
+struct omega_state_s {
+  omega_mesh_s *mesh = nullptr;
+  int ntr = 0;
+  size_t n[3] = {0, 0, 0}; // element counts of h, vn, tr
+  DBuf<Real> buf[2][3];    // [buffer][field]
+  int cur = 0;             // buffer holding the state
+  Real *ptr(int f) { return buf[cur][f].p; }
+  Real *alt(int f) {
+    if (buf[1 - cur][f].n != n[f]) buf[1 - cur][f].alloc(n[f]);
+    return buf[1 - cur][f].p;
+  }
+};
+
+struct omega_model_s {
+  omega_mesh_s *mesh = nullptr;
+  omega_params params{};
+  int terms = 0;
+  Launch lc;
+  DBuf<Real> f_v, cv2, cv4, ct2, ct4;
+  DBuf<Real> rvort_v, inv_thick_v, ke_c, div_c, trdel2_c, vdel2_e, vdel2_rv_v, vdel2_div_c;
+  DBuf<Real> partial, result;
+  omega_additional_tendency_fn add_fn = nullptr;
+  void *add_user = nullptr;
+  // host copies of the tracer mixing coefficients (for the adjoint tables)
+  std::vector<Real> ct2_h, ct4_h;
+  // adjoint: transposed tracer coefficients and cotangent scratch, built on
+  // first use
+  DBuf<Real> ect2, ect4;
+  DBuf<Real> a_sbar, a_zbar, a_tbar, a_kbar, a_dbar, a_gbar, a_fbar, a_d2bar, a_z2bar, a_e2bar,
+      a_partial, a_pgrad, a_mpart;
+  int a_nparts = 0;
+
+  ModelCoef coef() const {
+    return ModelCoef{params.grav, f_v.p, cv2.p, cv4.p, ct2.p, ct4.p};
+  }
+  DiagOut diag() {
+    return DiagOut{rvort_v.p, inv_thick_v.p, ke_c.p, div_c.p, trdel2_c.p,
+                   vdel2_e.p, vdel2_rv_v.p, vdel2_div_c.p};
+  }
+};
+
+// A captured step is valid only for the exact device buffers it recorded:
+// the key holds dt, the buffer parity and every buffer address the step
+// touches, so a state or scratch buffer reallocated at another address
+// never replays a stale graph.
+struct GraphKey {
+  double dt;
+  int cur;
+  std::vector<const void *> bufs;
+  bool operator<(const GraphKey &o) const {
+    if (dt != o.dt) return dt < o.dt;
+    if (cur != o.cur) return cur < o.cur;
+    return bufs < o.bufs;
+  }
+};
+
+struct omega_stepper_s {
+  omega_model_s *model = nullptr;
+  int kind = OMEGA_STEPPER_RK4;
+  int nstages = 4;
+  std::vector<Real> rka, rkb, rkc;
+  // fused RK4: accumulator and two provisional states; unfused RK4: tend,
+  // provisional and old states; LSRK: tend
+  DBuf<Real> acc[3], p1[3], p2[3], tend[3], provis[3], old[3];
+  bool use_graph = false;
+  std::map<GraphKey, hipGraphExec_t> graphs;
+  // step adjoint: recomputed stage states (RK4 stage inputs 2..4, LSRK stage
+  // inputs 2..S), their tendency, the incoming cotangent and stage cotangents
+  DBuf<Real> adj_y[4][3], adj_t[3], adj_gy[3], adj_pb[3], adj_kb[3];
+  hipStream_t cap_stream = nullptr; // private stream the step is captured on
+  ~omega_stepper_s() {
+    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
+  }
+};
+
+// ---------------------------------------------------------------------------
+
+namespace omega_hip {
+
+extern thread_local std::string g_last_error;
+
+// Runs f, converting exceptions to status codes and the thread's last error.
+template <class F> int guarded(F &&f) {
+  try {
+    f();
+    return OMEGA_OK;
+  } catch (const Error &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return OMEGA_ERR_STATE;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return OMEGA_ERR_STATE;
+  }
+}
+
+inline hipStream_t as_stream(omega_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+bool stream_capturing(hipStream_t s);
+
+void model_build_coefficients(omega_model_s *M);
+void ensure_vdel4(omega_model_s *M);
+void run_diag(omega_model_s *M, const Real *h, const Real *vn, const Real *tr, hipStream_t s);
+void run_stage(omega_model_s *M, StageArgs a, hipStream_t s);
+FieldIO fio(const Real *in, Real *tend, Real *acc, const Real *yold, Real *out);
+void check_state(omega_model_s *M, omega_state_s *S, const char *what);
+void ensure3(DBuf<Real> (&b)[3], omega_state_s *S);
+void compute_tendency_ptrs(omega_model_s *M, Real *ht, Real *vt, Real *tt, const Real *h,
+                           const Real *vn, const Real *tr, double t, int add_mode,
+                           hipStream_t s);
+void step_once(omega_stepper_s *St, omega_state_s *S, double t, double dt, hipStream_t s);
+void do_step_maybe_graph(omega_stepper_s *St, omega_state_s *S, double t, double dt,
+                         hipStream_t s);
+
+} // namespace omega_hip

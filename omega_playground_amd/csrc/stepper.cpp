@@ -1,0 +1,299 @@
+// Host-side orchestration: model coefficients, one tendency evaluation
+// (diagnostics + fused tendency/update launch), the RK4 and low-storage RK
+// steppers and their HIP-graph capture.
+#include <algorithm>
+#include <cmath>
+
+#include "objects.hpp"
+
+namespace omega_hip {
+
+void model_build_coefficients(omega_model_s *M) {
+  const DeviceMesh &dm = M->mesh->dm;
+  const HostMesh &hm = dm.host;
+  const int NC = hm.ncells, NE = hm.nedges, NV = hm.nvertices, ME = hm.max_edges;
+  const size_t L = hm.nlayers, ntr = M->params.ntracers;
+  std::vector<Real> f(NV, M->params.f0);
+  M->f_v.upload(f);
+  std::vector<Real> cv2(NE), cv4(NE);
+  for (int e = 0; e < NE; ++e) {
+    // mask*visc*scaling with mask = 1 (vel_diff_edge.hpp:83-84,
+    // vel_hyperdiff_edge.hpp:234-235)
+    cv2[e] = M->params.visc_del2 * dm.s2_h[e];
+    cv4[e] = M->params.visc_del4 * dm.s4_h[e];
+  }
+  M->cv2.upload(cv2);
+  M->cv4.upload(cv4);
+  std::vector<Real> ct2((size_t)NC * ME, 0.0), ct4((size_t)NC * ME, 0.0);
+  for (int i = 0; i < NC; ++i) {
+    const int c = dm.cell_n2o[i];
+    const double inv_area = 1. / hm.area_cell[c];
+    for (int j = 0; j < dm.noc_h[i]; ++j) {
+      const int eo = hm.edges_on_cell[(size_t)c * ME + j];
+      const int en = dm.edge_o2n[eo];
+      const double sign = hm.edge_sign_on_cell[(size_t)c * ME + j];
+      // leading factors of tracer_diff_cell.hpp:82-84 and
+      // tracer_hyperdiff_cell.hpp:152-154
+      ct2[(size_t)i * ME + j] = M->params.eddy_diff2 * inv_area * hm.dv_edge[eo] * sign;
+      ct4[(size_t)i * ME + j] =
+          M->params.eddy_diff4 * inv_area * hm.dv_edge[eo] * sign * dm.s4_h[en];
+    }
+  }
+  M->ct2.upload(ct2);
+  M->ct2_h = ct2;
+  M->ct4_h = ct4;
+  M->ct4.upload(ct4);
+  M->rvort_v.alloc((size_t)NV * L);
+  M->inv_thick_v.alloc((size_t)NV * L);
+  M->ke_c.alloc((size_t)NC * L);
+  M->div_c.alloc((size_t)NC * L);
+  if (ntr > 0) M->trdel2_c.alloc(ntr * NC * L);
+  M->partial.alloc((size_t)reduce_blocks(std::max(NC, NV)) + 1);
+  M->result.alloc(4);
+}
+
+void ensure_vdel4(omega_model_s *M) {
+  const HostMesh &hm = M->mesh->dm.host;
+  const size_t L = hm.nlayers;
+  if (!M->vdel2_e.p) {
+    M->vdel2_e.alloc((size_t)hm.nedges * L);
+    M->vdel2_rv_v.alloc((size_t)hm.nvertices * L);
+    M->vdel2_div_c.alloc((size_t)hm.ncells * L);
+  }
+}
+
+// Diagnostics of a state (vertex vorticity and thickness, cell kinetic
+// energy and divergence, tracer Laplacian, momentum del4 pre-pass) into the
+// model's buffers; only what the enabled terms read.
+void run_diag(omega_model_s *M, const Real *h, const Real *vn, const Real *tr, hipStream_t s) {
+  const MeshView mv = M->mesh->dm.view();
+  const int ntr = M->params.ntracers;
+  const bool do_vn = !M->params.disable_vn_tendency;
+  // diagnostics are needed by the vn tendency and by tracer del4 only
+  if (!(do_vn || ((M->terms & T_TDEL4) && ntr > 0))) return;
+  if (M->terms & T_VDEL4) ensure_vdel4(M);
+  const DiagOut d = M->diag();
+  const int dterms = do_vn ? M->terms : (M->terms & T_TDEL4);
+  if (hex_usable(mv, M->lc)) launch_diag_hex(mv, M->lc, dterms, ntr, h, vn, tr, d, s);
+  else launch_diag(mv, M->lc, M->coef(), dterms, ntr, h, vn, tr, d, s);
+  if (do_vn && (M->terms & T_VDEL4)) launch_vdel4(mv, M->lc, d, s);
+}
+
+// One tendency evaluation (+ optional fused stage update).
+void run_stage(omega_model_s *M, StageArgs a, hipStream_t s) {
+  const MeshView mv = M->mesh->dm.view();
+  const ModelCoef mc = M->coef();
+  a.terms = M->terms;
+  a.do_h = !M->params.disable_h_tendency;
+  a.do_vn = !M->params.disable_vn_tendency;
+  a.ntr = M->params.ntracers;
+  run_diag(M, a.h.in, a.vn.in, a.tr.in, s);
+  const DiagOut d = M->diag();
+  if (hex_usable(mv, M->lc)) launch_tend_hex(mv, M->lc, mc, a, d, s);
+  else launch_tend(mv, M->lc, mc, a, d, s);
+}
+
+FieldIO fio(const Real *in, Real *tend, Real *acc, const Real *yold, Real *out) {
+  return FieldIO{in, tend, acc, yold, out};
+}
+
+void call_hook(omega_model_s *M, Real *ht, Real *vt, const Real *h, const Real *vn, double t,
+               hipStream_t s) {
+  if (!M->add_fn) return;
+  const int rc = M->add_fn(M->add_user, ht, vt, h, vn, t, s);
+  if (rc != 0) throw Error(OMEGA_ERR_CALLBACK, "additional tendency hook returned " + std::to_string(rc));
+}
+
+void check_state(omega_model_s *M, omega_state_s *S, const char *what) {
+  OMEGA_REQUIRE(S != nullptr, std::string(what) + ": null state");
+  OMEGA_REQUIRE(S->mesh == M->mesh, std::string(what) + ": state belongs to another mesh");
+  OMEGA_REQUIRE(S->ntr >= M->params.ntracers,
+                std::string(what) + ": state has fewer tracers than the model");
+}
+
+void ensure3(DBuf<Real> (&b)[3], omega_state_s *S) {
+  for (int f = 0; f < 3; ++f)
+    if (b[f].n != S->n[f]) b[f].alloc(S->n[f]);
+}
+
+// compute_tendency (shallow_water_model.cpp:313-347), writing tend buffers.
+void compute_tendency_ptrs(omega_model_s *M, Real *ht, Real *vt, Real *tt, const Real *h,
+                           const Real *vn, const Real *tr, double t, int add_mode,
+                           hipStream_t s) {
+  StageArgs a{};
+  a.epi = EPI_STORE;
+  a.base_from_tend = add_mode == OMEGA_ADD_INCREMENT;
+  a.a = 1.0;
+  a.h = fio(h, ht, nullptr, nullptr, nullptr);
+  a.vn = fio(vn, vt, nullptr, nullptr, nullptr);
+  a.tr = fio(tr, tt, nullptr, nullptr, nullptr);
+  run_stage(M, a, s);
+  call_hook(M, ht, vt, h, vn, t, s);
+}
+
+// ---------------------------------------------------------------------------
+// Steppers
+// ---------------------------------------------------------------------------
+
+// RK4 with a fused tendency + update per stage (rk4.cpp:28-104 reorganised:
+// the accumulator lives in acc until the last stage writes the state).
+void rk4_fused_step(omega_stepper_s *St, omega_state_s *S, double dt, hipStream_t s) {
+  omega_model_s *M = St->model;
+  ensure3(St->acc, S);
+  ensure3(St->p1, S);
+  ensure3(St->p2, S);
+  Real *Y[3] = {S->ptr(0), S->ptr(1), S->ptr(2)};
+  Real *A[3] = {St->acc[0].p, St->acc[1].p, St->acc[2].p};
+  Real *P1[3] = {St->p1[0].p, St->p1[1].p, St->p1[2].p};
+  Real *P2[3] = {St->p2[0].p, St->p2[1].p, St->p2[2].p};
+  for (int stage = 0; stage < 4; ++stage) {
+    StageArgs a{};
+    a.base_from_tend = 0;
+    a.db = dt * St->rkb[stage];
+    a.da = stage < 3 ? dt * St->rka[stage] : 0.0;
+    Real **in = stage == 0 ? Y : (stage == 2 ? P2 : P1);
+    Real **out = stage == 0 ? P1 : (stage == 1 ? P2 : (stage == 2 ? P1 : Y));
+    a.epi = stage == 0 ? EPI_RK4_FIRST : (stage == 3 ? EPI_RK4_LAST : EPI_RK4_MID);
+    a.h = fio(in[0], nullptr, A[0], Y[0], out[0]);
+    a.vn = fio(in[1], nullptr, A[1], Y[1], out[1]);
+    a.tr = fio(in[2], nullptr, A[2], Y[2], out[2]);
+    run_stage(M, a, s);
+  }
+}
+
+// RK4 exactly as rk4.cpp:28-104 (used when an additional-tendency hook is set).
+void rk4_unfused_step(omega_stepper_s *St, omega_state_s *S, double t, double dt, hipStream_t s) {
+  omega_model_s *M = St->model;
+  ensure3(St->tend, S);
+  ensure3(St->provis, S);
+  ensure3(St->old, S);
+  for (int f = 0; f < 3; ++f)
+    if (S->n[f])
+      OMEGA_HIP_CHECK(hipMemcpyAsync(St->old[f].p, S->ptr(f), S->n[f] * sizeof(Real),
+                                     hipMemcpyDeviceToDevice, s));
+  compute_tendency_ptrs(M, St->tend[0].p, St->tend[1].p, St->tend[2].p, S->ptr(0), S->ptr(1),
+                        S->ptr(2), t, OMEGA_ADD_REPLACE, s);
+  for (int stage = 0; stage < 4; ++stage) {
+    const double db = dt * St->rkb[stage];
+    for (int f = 0; f < 3; ++f) launch_axpy(S->ptr(f), db, St->tend[f].p, S->n[f], s);
+    if (stage < 3) {
+      const double stagetime = t + St->rkc[stage] * dt;
+      const double da = dt * St->rka[stage];
+      for (int f = 0; f < 3; ++f)
+        launch_xpay(St->provis[f].p, St->old[f].p, da, St->tend[f].p, S->n[f], s);
+      compute_tendency_ptrs(M, St->tend[0].p, St->tend[1].p, St->tend[2].p, St->provis[0].p,
+                            St->provis[1].p, St->provis[2].p, stagetime, OMEGA_ADD_REPLACE, s);
+    }
+  }
+}
+
+// Low-storage RK with the state alternating between two buffers per stage
+// (lsrk.cpp:36-89 fused: tend = a*tend + F(y); y' = y + dt*b*tend).
+void lsrk_fused_step(omega_stepper_s *St, omega_state_s *S, double dt, hipStream_t s) {
+  omega_model_s *M = St->model;
+  ensure3(St->tend, S);
+  for (int stage = 0; stage < St->nstages; ++stage) {
+    StageArgs a{};
+    a.epi = EPI_LSRK;
+    a.a = St->rka[stage];
+    a.base_from_tend = St->rka[stage] != 0.0;
+    a.db = dt * St->rkb[stage];
+    Real *in[3] = {S->ptr(0), S->ptr(1), S->ptr(2)};
+    Real *out[3] = {S->alt(0), S->alt(1), S->alt(2)};
+    a.h = fio(in[0], St->tend[0].p, nullptr, nullptr, out[0]);
+    a.vn = fio(in[1], St->tend[1].p, nullptr, nullptr, out[1]);
+    a.tr = fio(in[2], St->tend[2].p, nullptr, nullptr, out[2]);
+    run_stage(M, a, s);
+    S->cur = 1 - S->cur;
+  }
+}
+
+// LSRK exactly as lsrk.cpp:36-89 (used when an additional-tendency hook is set).
+void lsrk_unfused_step(omega_stepper_s *St, omega_state_s *S, double t, double dt, hipStream_t s) {
+  omega_model_s *M = St->model;
+  ensure3(St->tend, S);
+  for (int stage = 0; stage < St->nstages; ++stage) {
+    for (int f = 0; f < 3; ++f) launch_scale(St->tend[f].p, St->rka[stage], S->n[f], s);
+    const double stagetime = t + St->rkc[stage] * dt;
+    compute_tendency_ptrs(M, St->tend[0].p, St->tend[1].p, St->tend[2].p, S->ptr(0), S->ptr(1),
+                          S->ptr(2), stagetime, OMEGA_ADD_INCREMENT, s);
+    const double db = dt * St->rkb[stage];
+    for (int f = 0; f < 3; ++f) launch_axpy(S->ptr(f), db, St->tend[f].p, S->n[f], s);
+  }
+}
+
+void step_once(omega_stepper_s *St, omega_state_s *S, double t, double dt, hipStream_t s) {
+  const bool fused = St->model->add_fn == nullptr;
+  if (St->kind == OMEGA_STEPPER_RK4) {
+    if (fused) rk4_fused_step(St, S, dt, s);
+    else rk4_unfused_step(St, S, t, dt, s);
+  } else {
+    if (fused) lsrk_fused_step(St, S, dt, s);
+    else lsrk_unfused_step(St, S, t, dt, s);
+  }
+}
+
+void do_step_maybe_graph(omega_stepper_s *St, omega_state_s *S, double t, double dt,
+                         hipStream_t s) {
+  const bool graphable = St->use_graph && St->model->add_fn == nullptr && !stream_capturing(s);
+  if (!graphable) {
+    step_once(St, S, t, dt, s);
+    return;
+  }
+  if (St->kind == OMEGA_STEPPER_LSRK) {
+    // make sure both buffers exist before capture
+    for (int f = 0; f < 3; ++f) (void)S->alt(f);
+  }
+  // every buffer the step touches is allocated before the key is formed
+  if (St->kind == OMEGA_STEPPER_RK4) {
+    ensure3(St->acc, S);
+    ensure3(St->p1, S);
+    ensure3(St->p2, S);
+  } else {
+    ensure3(St->tend, S);
+  }
+  omega_model_s *M = St->model;
+  if (M->terms & T_VDEL4) ensure_vdel4(M);
+  GraphKey key{dt, S->cur, {}};
+  for (int f = 0; f < 3; ++f)
+    key.bufs.insert(key.bufs.end(), {S->buf[0][f].p, S->buf[1][f].p, St->acc[f].p,
+                                     St->p1[f].p, St->p2[f].p, St->tend[f].p});
+  key.bufs.insert(key.bufs.end(), {M->vdel2_e.p, M->trdel2_c.p});
+  auto it = St->graphs.find(key);
+  if (it == St->graphs.end()) {
+    // bound the cache: states created and destroyed in a loop (the adjoint's
+    // work states) would otherwise leave a graph each behind
+    if (St->graphs.size() >= 16) {
+      OMEGA_HIP_CHECK(hipStreamSynchronize(s));
+      for (auto &kv : St->graphs) (void)hipGraphExecDestroy(kv.second);
+      St->graphs.clear();
+    }
+    // capture on a private stream (the null stream cannot be captured); the
+    // graph is then launched on the caller's stream
+    if (!St->cap_stream)
+      OMEGA_HIP_CHECK(hipStreamCreateWithFlags(&St->cap_stream, hipStreamNonBlocking));
+    hipStream_t cs = St->cap_stream;
+    const int cur0 = S->cur;
+    hipGraph_t g;
+    OMEGA_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    try {
+      step_once(St, S, t, dt, cs);
+    } catch (...) {
+      hipGraph_t dummy;
+      (void)hipStreamEndCapture(cs, &dummy);
+      S->cur = cur0;
+      throw;
+    }
+    OMEGA_HIP_CHECK(hipStreamEndCapture(cs, &g));
+    hipGraphExec_t ex;
+    OMEGA_HIP_CHECK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+    OMEGA_HIP_CHECK(hipGraphDestroy(g));
+    S->cur = cur0; // capture recorded the flips; replay applies them below
+    it = St->graphs.emplace(key, ex).first;
+  }
+  OMEGA_HIP_CHECK(hipGraphLaunch(it->second, s));
+  if (St->kind == OMEGA_STEPPER_LSRK && (St->nstages % 2 == 1)) S->cur = 1 - S->cur;
+}
+
+
+} // namespace omega_hip

@@ -114,6 +114,17 @@ def lib():
         "omega_timing_enable": ([C.c_int], C.c_int),
         "omega_timing_reset": ([], C.c_int),
         "omega_timing_read": ([C.c_char_p, dp, i64p], C.c_int),
+        "omega_model_tendency_vjp": ([vp, vp, vp, vp, dp, vp], C.c_int),
+        "omega_stepper_step_vjp": ([vp, C.c_double, C.c_double, vp, vp, dp, vp], C.c_int),
+        "omega_tape_create": ([vp, C.c_int, C.c_int, C.c_int64, pp], C.c_int),
+        "omega_tape_destroy": ([vp], C.c_int),
+        "omega_tape_forward": ([vp, C.c_double, C.c_double, vp, vp], C.c_int),
+        "omega_tape_reverse": ([vp, vp, dp, vp], C.c_int),
+        "omega_tape_info": ([vp, ip, i64p, i64p], C.c_int),
+        "omega_revolve_cost": ([C.c_int64, C.c_int], C.c_int64),
+        "omega_misfit": ([vp, vp, vp, dp, vp, dp, vp], C.c_int),
+        "omega_adjoint_run": ([vp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int64, vp, vp,
+                               dp, dp, vp, dp, vp], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -445,6 +456,97 @@ class LSRKStepper(_Stepper):
     """LSRKStepper(model, nstages=5) (src/time_integration/lsrk.cpp)."""
 
     _kind = 1
+
+
+# ---------------------------------------------------------------------------
+# Discrete adjoint (no reference counterpart; differentiates the operators of
+# src/model/shallow_water_model.cpp and src/time_integration/{rk4,lsrk}.cpp)
+# ---------------------------------------------------------------------------
+PARAM_NAMES = ("visc_del2", "visc_del4", "eddy_diff2", "eddy_diff4")
+
+
+def _pgrad_out(param_grad):
+    if param_grad is None:
+        return None, None
+    buf = (C.c_double * 4)(*[float(x) for x in param_grad])
+    return buf, C.cast(buf, C.POINTER(C.c_double))
+
+
+def tendency_vjp(model: ShallowWaterModel, state: ShallowWaterState, tend_cot: ShallowWaterState,
+                 state_cot: ShallowWaterState, stream=None) -> np.ndarray:
+    """state_cot += J(state)^T tend_cot; returns d<tend_cot, F>/d(coefficients)
+    in the order of PARAM_NAMES."""
+    buf, ptr = _pgrad_out([0.0] * 4)
+    check(lib().omega_model_tendency_vjp(model._h, state._h, tend_cot._h, state_cot._h, ptr,
+                                         _stream(stream)))
+    return np.array(buf[:])
+
+
+def step_vjp(stepper: "_Stepper", t: float, dt: float, state: ShallowWaterState,
+             cot: ShallowWaterState, stream=None) -> np.ndarray:
+    """cot <- (dy_{n+1}/dy_n)^T cot for one step from state; returns the
+    coefficient gradient contribution."""
+    buf, ptr = _pgrad_out([0.0] * 4)
+    check(lib().omega_stepper_step_vjp(stepper._h, float(t), float(dt), state._h, cot._h, ptr,
+                                       _stream(stream)))
+    return np.array(buf[:])
+
+
+def revolve_cost(nsteps: int, n_snapshots: int) -> int:
+    """Minimal forward steps of a binomial reversal schedule."""
+    return int(lib().omega_revolve_cost(int(nsteps), int(n_snapshots)))
+
+
+class Tape:
+    """Binomial checkpointing over nsteps steps of a stepper."""
+
+    def __init__(self, stepper: "_Stepper", nsteps: int, n_snapshots: int, byte_cap: int = 0):
+        self.stepper = stepper
+        h = C.c_void_p()
+        check(lib().omega_tape_create(stepper._h, int(nsteps), int(n_snapshots), int(byte_cap),
+                                      C.byref(h)))
+        self._h = h
+
+    def forward(self, t0: float, dt: float, state: ShallowWaterState, stream=None):
+        check(lib().omega_tape_forward(self._h, float(t0), float(dt), state._h, _stream(stream)))
+
+    def reverse(self, cot: ShallowWaterState, stream=None) -> np.ndarray:
+        buf, ptr = _pgrad_out([0.0] * 4)
+        check(lib().omega_tape_reverse(self._h, cot._h, ptr, _stream(stream)))
+        return np.array(buf[:])
+
+    def info(self):
+        n, fs, nb = C.c_int(), C.c_int64(), C.c_int64()
+        check(lib().omega_tape_info(self._h, C.byref(n), C.byref(fs), C.byref(nb)))
+        return {"n_snapshots": n.value, "forward_steps": fs.value, "snapshot_bytes": nb.value}
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _lib is not None:
+            _lib.omega_tape_destroy(self._h)
+            self._h = None
+
+
+def misfit(model: ShallowWaterModel, state: ShallowWaterState, obs: ShallowWaterState,
+           cot: ShallowWaterState, weights=(1.0, 0.0, 1.0), stream=None) -> float:
+    """J = sum_f w_f/2 sum_i area_i sum_k (y - obs)^2; cot = dJ/dy."""
+    w = (C.c_double * 3)(*[float(x) for x in weights])
+    J = C.c_double()
+    check(lib().omega_misfit(model._h, state._h, obs._h, w, cot._h, C.byref(J), _stream(stream)))
+    return J.value
+
+
+def adjoint_run(stepper: "_Stepper", t0: float, dt: float, nsteps: int, state: ShallowWaterState,
+                obs: ShallowWaterState, grad: ShallowWaterState, weights=(1.0, 0.0, 1.0),
+                n_snapshots: int = 10, byte_cap: int = 0, stream=None):
+    """Misfit after nsteps steps from state and its gradient: grad = dJ/dy_0;
+    returns (J, coefficient gradient)."""
+    w = (C.c_double * 3)(*[float(x) for x in weights])
+    J = C.c_double()
+    buf, ptr = _pgrad_out([0.0] * 4)
+    check(lib().omega_adjoint_run(stepper._h, float(t0), float(dt), int(nsteps), int(n_snapshots),
+                                  int(byte_cap), state._h, obs._h, w, C.byref(J), grad._h, ptr,
+                                  _stream(stream)))
+    return J.value, np.array(buf[:])
 
 
 def _integral(fn, state, model, stream=None):
