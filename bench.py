@@ -79,6 +79,105 @@ def cpu_baseline(cfg, sample_steps: int, threads: int):
     }
 
 
+def adjoint_bench(n_snapshots: int = 10, nsteps: int | None = None):
+    """The adjoint workload: RK4 on nx=256, 32 layers, 2 tracers, del2
+    viscosity on; misfit of h and tracers after the trajectory against
+    observations from a run whose initial h was perturbed by a correlated
+    Gaussian random field; gradient with respect to the initial state and
+    the mixing coefficients with binomial checkpointing."""
+    cfg = dict(wl.CONFIGS["c3"])
+    nsteps = nsteps or cfg["nsteps"]
+    mesh = swe.PlanarHexagonalMesh(cfg["nx"], cfg["nx"], cfg["dc"], cfg["nlayers"])
+    params = swe.ShallowWaterParams(f0=cfg["f0"], grav=cfg["grav"], visc_del2=cfg["visc_del2"],
+                                    eddy_diff4=cfg["eddy_diff4"], ntracers=cfg["ntracers"])
+    model = swe.ShallowWaterModel(mesh, params)
+    ntr, L, nc = cfg["ntracers"], cfg["nlayers"], mesh.ncells
+    dt = wl.dt_rule(cfg["dc"], cfg["grav"])
+    st = swe.RK4Stepper(model)
+    st.set_graph(True)
+    y0 = wl.make_state(mesh, cfg, seed=0)
+    obs = swe.ShallowWaterState(mesh, ntr)
+    obs.upload(*wl.adjoint_observations(mesh, cfg, seed=0))
+    st.run(0.0, dt, nsteps, obs)
+    state = swe.ShallowWaterState(mesh, ntr)
+    state.upload(*y0)
+    cot = swe.ShallowWaterState(mesh, ntr)
+    cot.upload(*[np.ones_like(a) for a in y0])
+    # per step: forward step vs step adjoint (stage recomputation included)
+    st.run(0.0, dt, 2, state)
+    swe.step_vjp(st, 0.0, dt, state, cot)
+    swe.synchronize()
+    reps = 20
+    t0 = time.perf_counter()
+    st.run(0.0, dt, reps, state)
+    swe.synchronize()
+    t_fwd = (time.perf_counter() - t0) / reps
+    st.set_graph(False)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        swe.step_vjp(st, 0.0, dt, state, cot)
+    swe.synchronize()
+    t_adj = (time.perf_counter() - t0) / reps
+    st.set_graph(True)
+    # the whole trajectory: forward sweep + checkpointed reverse sweep
+    state.upload(*y0)
+    grad = swe.ShallowWaterState(mesh, ntr)
+    swe.synchronize()
+    t0 = time.perf_counter()
+    J, pgrad = swe.adjoint_run(st, 0.0, dt, nsteps, state, obs, grad, (1.0, 0.0, 1.0),
+                               n_snapshots=n_snapshots)
+    swe.synchronize()
+    t_run = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    st.run(0.0, dt, nsteps, state)
+    swe.synchronize()
+    t_fwd_run = time.perf_counter() - t0
+    work = nc * L * nsteps
+    recompute = swe.revolve_cost(nsteps, n_snapshots)
+    return {
+        "workload": f"RK4 adjoint, planar hex nx={cfg['nx']} ({nc} cells), {L} layers, {ntr} "
+                    f"tracers, del2 visc {cfg['visc_del2']:g}, {nsteps} steps, {n_snapshots} "
+                    f"snapshots, misfit of h and tracers vs perturbed-h0 observations",
+        "fwd_ms_per_step": 1e3 * t_fwd,
+        "step_adjoint_ms": 1e3 * t_adj,
+        "ratio_per_step": t_adj / t_fwd,
+        "forward_run_s": t_fwd_run,
+        "adjoint_run_s": t_run,
+        "ratio_total": t_run / t_fwd_run,
+        "binomial_recompute_steps": recompute,
+        "fwd_adj_cell_layer_steps_per_s": work / t_run,
+        "J": J,
+        "dJ_dvisc_del2": float(pgrad[0]),
+    }
+
+
+def copy_calibration():
+    """A plain device copy of known size, for scaling the HBM byte counters:
+    1 GiB read + 1 GiB written per copy kernel."""
+    import torch
+
+    n = (1 << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "copies": 3}))
+
+
+def load_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 counter summary, when it was measured on this workload."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    ent = rec.get(workload, {}).get(kernel)
+    return ent
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,7 +187,9 @@ def main():
     ap.add_argument("--ordering", default="hilbert")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--skip-adjoint", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--launch", default="", help="rows,lpt,swizzle,levels_per_block")
     args = ap.parse_args()
 
@@ -97,6 +198,10 @@ def main():
               "(the sharding design is described in DESIGN.md); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     cfg, mesh, params, model, state0 = build_case(args.config, args.ordering)
     if args.launch:
@@ -148,6 +253,13 @@ def main():
     log(f"[bench] tend kernel avg {avg_tend_ms:.4f} ms x{tend_n}, diag avg {avg_diag_ms:.4f} ms "
         f"x{diag_n}; B_alg {B:.0f} B/cell-layer-step")
 
+    adj = None
+    if not args.skip_adjoint:
+        log("[bench] adjoint workload")
+        adj = adjoint_bench()
+        log(f"[bench] adjoint: step ratio {adj['ratio_per_step']:.2f}, trajectory ratio "
+            f"{adj['ratio_total']:.2f} ({adj['adjoint_run_s']:.3f} s)")
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
@@ -155,6 +267,9 @@ def main():
         log(f"[bench] CPU baseline: {args.cpu_steps} steps, {threads} threads")
         cpu = cpu_baseline(cfg, args.cpu_steps, threads)
         log(f"[bench] CPU {cpu['value']:.4e} cell-layer-steps/s")
+
+    workload_key = f"{args.config}:{args.ordering}:" + ",".join(str(x) for x in model.get_launch())
+    traffic = load_traffic("tend", workload_key)
 
     out = {
         "metric": "cell-layer-steps/s FP64 fwd+adjoint at 1 GPU; adjoint/fwd ratio; batched speedup",
@@ -176,13 +291,16 @@ def main():
                    "ncells": nc, "nlayers": L, "ntracers": ntr, "stepper": cfg["stepper"],
                    "graph": not args.no_graph, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "kernel": "k_tend (fused tendency + RK4 stage update)",
                      "kernel_avg_ms": avg_tend_ms,
                      "alg_bytes_per_launch": alg_bytes_launch},
         "step_roofline": {"B_alg_per_cell_layer_step": B, "achieved_GBs": step_achieved,
                           "frac": step_achieved / HBM_PEAK_GBS, "diag_kernel_avg_ms": avg_diag_ms},
         "finite": finite,
+        "adjoint": adj,
+        "batched": None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

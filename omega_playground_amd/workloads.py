@@ -234,6 +234,40 @@ def fourier_tracers(mesh, nlayers, ntracers, seed=0, nmodes=16, kmax=8):
     return np.ascontiguousarray(tr)
 
 
+def gaussian_random_field(mesh, amplitude, corr_length, seed=0, cutoff=3.0):
+    """Seeded stationary Gaussian random field at the cell centres of a doubly
+    periodic mesh, with Gaussian covariance exp(-r^2/(2 l^2)) and standard
+    deviation `amplitude` (spectral synthesis over the periodic Fourier
+    modes with |k| l <= cutoff, rescaled to the requested deviation)."""
+    lx, ly = mesh.period_x, mesh.period_y
+    xc, yc = np.asarray(mesh.x_cell), np.asarray(mesh.y_cell)
+    mmax = max(1, int(cutoff * lx / (2 * math.pi * corr_length)))
+    nmax = max(1, int(cutoff * ly / (2 * math.pi * corr_length)))
+    m = np.arange(-mmax, mmax + 1)
+    n = np.arange(-nmax, nmax + 1)
+    kx = 2 * math.pi * m / lx
+    ky = 2 * math.pi * n / ly
+    # spectral density of the Gaussian covariance ~ exp(-|k|^2 l^2 / 2);
+    # amplitudes are its square root
+    spec = np.exp(-0.25 * (kx[:, None] ** 2 + ky[None, :] ** 2) * corr_length ** 2)
+    rng = np.random.default_rng(seed)
+    coef = spec * (rng.standard_normal(spec.shape) + 1j * rng.standard_normal(spec.shape))
+    ex = np.exp(1j * np.outer(xc, kx))            # [nc][M]
+    ey = np.exp(1j * np.outer(yc, ky))            # [nc][N]
+    field = np.real(np.einsum("cm,mn,cn->c", ex, coef, ey, optimize=True))
+    field -= field.mean()
+    return amplitude * field / field.std()
+
+
+def adjoint_observations(mesh, cfg, seed=0):
+    """Initial state of the observation run of the adjoint workload: the
+    initial state with h perturbed by a Gaussian random field of 1e-2 m
+    standard deviation and correlation length 5*dc (same on every layer)."""
+    h, vn, tr = make_state(mesh, cfg, seed=seed)
+    pert = gaussian_random_field(mesh, 1e-2, 5 * cfg["dc"], seed=seed + 17)
+    return np.ascontiguousarray(h + pert[:, None]), vn, tr
+
+
 # Workload table of the throughput metric -----------------------------------
 CONFIGS = {
     "c1": dict(nx=64, dc=10e3, nlayers=64, ntracers=1, stepper="rk4", nsteps=10,
