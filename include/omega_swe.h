@@ -3,9 +3,10 @@
  *
  * Drop-in boundary for mwarusz/omega_playground's mesh / state / tendency /
  * time-stepper API.  Every entry point names the reference interface it
- * replaces (file:line in omega_playground).  The C++ classes in
- * omega_playground_amd/cpp/omega (namespace omega) call these functions;
- * Python reaches them through ctypes.
+ * replaces (file:line in omega_playground).  The header-only C++ classes
+ * under omega_playground_amd/cpp (namespace omega, headers laid out as the
+ * reference's src/) call these functions; Python reaches them through
+ * ctypes.
  *
  * Conventions
  *  - Every function returns OMEGA_OK (0) or an error code; the message of the
@@ -203,6 +204,16 @@ int omega_model_set_additional_tendency(omega_model_t model, omega_additional_te
 int omega_model_compute_tendency(omega_model_t model, omega_state_t tend, omega_state_t state,
                                  double t, int add_mode, omega_stream_t stream);
 
+/* Replaces ShallowWaterModel::compute_h_tendency, compute_vn_tendency and
+ * compute_tr_tendency (src/model/shallow_water_model.hpp:35-40,
+ * shallow_water_model.cpp:43-311): field 0 h, 1 vn, 2 tracers.  Only that
+ * field of tend is written.  The diagnostics are computed from state first
+ * (the reference expects compute_tendency's aux pass to have run); the
+ * additional-tendency hook is not called, as there. */
+int omega_model_compute_field_tendency(omega_model_t model, int field, omega_state_t tend,
+                                       omega_state_t state, int add_mode,
+                                       omega_stream_t stream);
+
 /* Term switches for tests (bit 0 PV flux, 1 KE gradient, 2 SSH gradient,
  * 3 momentum del2, 4 momentum del4, 5 tracer del2, 6 tracer del4); the
  * default is what the constructor enabled.  No reference counterpart: the
@@ -241,6 +252,11 @@ int omega_model_kernel_variant(omega_model_t model, int *in_use);
  * LSRKStepper(model, nstages) (src/time_integration/lsrk.cpp:5-34). */
 int omega_stepper_create(omega_model_t model, int kind, int nstages, omega_stepper_t *out);
 int omega_stepper_destroy(omega_stepper_t stepper);
+
+/* m_rka, m_rkb, m_rkc of RK4Stepper / LSRKStepper (rk4.cpp:14-25,
+ * lsrk.cpp:16-33); each non-NULL array receives nstages values (RK4's a and
+ * c have 3 entries; the 4th is 0). */
+int omega_stepper_coeffs(omega_stepper_t stepper, int *nstages, double *a, double *b, double *c);
 
 /* Replaces TimeStepper::do_step(t, dt, state)
  * (src/time_integration/rk4.cpp:28-104, src/time_integration/lsrk.cpp:36-89).

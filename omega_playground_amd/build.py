@@ -58,5 +58,33 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+REPO = os.path.dirname(HERE)
+CPP_TESTS = os.path.join(REPO, "tests", "cpp")
+
+
+def build_cpp_programs(verbose: bool = False) -> list:
+    """Compiles the C++ programs written against the drop-in C++ API
+    (tests/cpp/*.cpp) with g++ into tests/cpp/bin/, linked to
+    libomega_swe.so through a relative rpath."""
+    out_dir = os.path.join(CPP_TESTS, "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    built = []
+    for src in sorted(glob.glob(os.path.join(CPP_TESTS, "*.cpp"))):
+        exe = os.path.join(out_dir, os.path.splitext(os.path.basename(src))[0])
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Wno-unused-parameter",
+               "-I", os.path.join(REPO, "include"), "-I", os.path.join(HERE, "cpp"),
+               src, "-o", exe, "-L", HERE, "-lomega_swe",
+               "-Wl,-rpath,$ORIGIN/../../../omega_playground_amd"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {src}:\n{r.stdout}{r.stderr}")
+        if r.stderr.strip():
+            sys.stderr.write(r.stderr)
+        built.append(exe)
+        if verbose:
+            print(f"built {exe}")
+    return built
+
+
 if __name__ == "__main__":
     build(verbose=True)

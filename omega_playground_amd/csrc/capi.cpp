@@ -495,6 +495,22 @@ int omega_model_compute_tendency(omega_model_t M, omega_state_t tend, omega_stat
   });
 }
 
+int omega_model_compute_field_tendency(omega_model_t M, int field, omega_state_t tend,
+                                       omega_state_t state, int add_mode,
+                                       omega_stream_t stream) {
+  return guarded([&] {
+    OMEGA_REQUIRE(M, "null model");
+    OMEGA_REQUIRE(field >= 0 && field <= 2, "field must be 0 (h), 1 (vn) or 2 (tracers)");
+    check_state(M, state, "compute_field_tendency(state)");
+    check_state(M, tend, "compute_field_tendency(tend)");
+    OMEGA_REQUIRE(add_mode == OMEGA_ADD_REPLACE || add_mode == OMEGA_ADD_INCREMENT,
+                  "bad add_mode");
+    compute_tendency_ptrs(M, tend->ptr(0), tend->ptr(1), tend->ptr(2), state->ptr(0),
+                          state->ptr(1), state->ptr(2), 0.0, add_mode, as_stream(stream),
+                          1 << field);
+  });
+}
+
 int omega_model_set_terms(omega_model_t M, int mask) {
   return guarded([&] {
     OMEGA_REQUIRE(M, "null model");
@@ -608,6 +624,20 @@ int omega_stepper_set_graph(omega_stepper_t S, int enable) {
   return guarded([&] {
     OMEGA_REQUIRE(S, "null stepper");
     S->use_graph = enable != 0;
+  });
+}
+
+int omega_stepper_coeffs(omega_stepper_t St, int *nstages, double *a, double *b, double *c) {
+  return guarded([&] {
+    OMEGA_REQUIRE(St, "null stepper");
+    if (nstages) *nstages = St->nstages;
+    auto put = [&](double *dst, const std::vector<Real> &src) {
+      if (!dst) return;
+      for (int i = 0; i < St->nstages; ++i) dst[i] = i < (int)src.size() ? src[i] : 0.0;
+    };
+    put(a, St->rka);
+    put(b, St->rkb);
+    put(c, St->rkc);
   });
 }
 

@@ -131,13 +131,14 @@ bool stream_capturing(hipStream_t s);
 void model_build_coefficients(omega_model_s *M);
 void ensure_vdel4(omega_model_s *M);
 void run_diag(omega_model_s *M, const Real *h, const Real *vn, const Real *tr, hipStream_t s);
-void run_stage(omega_model_s *M, StageArgs a, hipStream_t s);
+// fields: bit 0 h, bit 1 vn, bit 2 tracers (all by default)
+void run_stage(omega_model_s *M, StageArgs a, hipStream_t s, int fields = 7);
 FieldIO fio(const Real *in, Real *tend, Real *acc, const Real *yold, Real *out);
 void check_state(omega_model_s *M, omega_state_s *S, const char *what);
 void ensure3(DBuf<Real> (&b)[3], omega_state_s *S);
 void compute_tendency_ptrs(omega_model_s *M, Real *ht, Real *vt, Real *tt, const Real *h,
                            const Real *vn, const Real *tr, double t, int add_mode,
-                           hipStream_t s);
+                           hipStream_t s, int fields = 7);
 void step_once(omega_stepper_s *St, omega_state_s *S, double t, double dt, hipStream_t s);
 void do_step_maybe_graph(omega_stepper_s *St, omega_state_s *S, double t, double dt,
                          hipStream_t s);

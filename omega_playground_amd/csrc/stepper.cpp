@@ -80,13 +80,13 @@ void run_diag(omega_model_s *M, const Real *h, const Real *vn, const Real *tr, h
 }
 
 // One tendency evaluation (+ optional fused stage update).
-void run_stage(omega_model_s *M, StageArgs a, hipStream_t s) {
+void run_stage(omega_model_s *M, StageArgs a, hipStream_t s, int fields) {
   const MeshView mv = M->mesh->dm.view();
   const ModelCoef mc = M->coef();
   a.terms = M->terms;
-  a.do_h = !M->params.disable_h_tendency;
-  a.do_vn = !M->params.disable_vn_tendency;
-  a.ntr = M->params.ntracers;
+  a.do_h = !M->params.disable_h_tendency && (fields & 1);
+  a.do_vn = !M->params.disable_vn_tendency && (fields & 2);
+  a.ntr = (fields & 4) ? M->params.ntracers : 0;
   run_diag(M, a.h.in, a.vn.in, a.tr.in, s);
   const DiagOut d = M->diag();
   if (hex_usable(mv, M->lc)) launch_tend_hex(mv, M->lc, mc, a, d, s);
@@ -119,7 +119,7 @@ void ensure3(DBuf<Real> (&b)[3], omega_state_s *S) {
 // compute_tendency (shallow_water_model.cpp:313-347), writing tend buffers.
 void compute_tendency_ptrs(omega_model_s *M, Real *ht, Real *vt, Real *tt, const Real *h,
                            const Real *vn, const Real *tr, double t, int add_mode,
-                           hipStream_t s) {
+                           hipStream_t s, int fields) {
   StageArgs a{};
   a.epi = EPI_STORE;
   a.base_from_tend = add_mode == OMEGA_ADD_INCREMENT;
@@ -127,8 +127,10 @@ void compute_tendency_ptrs(omega_model_s *M, Real *ht, Real *vt, Real *tt, const
   a.h = fio(h, ht, nullptr, nullptr, nullptr);
   a.vn = fio(vn, vt, nullptr, nullptr, nullptr);
   a.tr = fio(tr, tt, nullptr, nullptr, nullptr);
-  run_stage(M, a, s);
-  call_hook(M, ht, vt, h, vn, t, s);
+  run_stage(M, a, s, fields);
+  // the hook runs after the built-in terms of a full evaluation only
+  // (shallow_water_model.cpp:344-345)
+  if (fields == 7) call_hook(M, ht, vt, h, vn, t, s);
 }
 
 // ---------------------------------------------------------------------------

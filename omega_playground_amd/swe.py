@@ -96,6 +96,8 @@ def lib():
         "omega_model_set_additional_tendency": ([vp, ADDFN, vp], C.c_int),
         "omega_model_compute_tendency": ([vp, vp, vp, C.c_double, C.c_int, vp], C.c_int),
         "omega_model_set_terms": ([vp, C.c_int], C.c_int),
+        "omega_model_compute_field_tendency": ([vp, C.c_int, vp, vp, C.c_int, vp], C.c_int),
+        "omega_stepper_coeffs": ([vp, ip, dp, dp, dp], C.c_int),
         "omega_model_get_terms": ([vp, ip], C.c_int),
         "omega_model_set_launch": ([vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
         "omega_model_get_launch": ([vp, ip, ip, ip, ip], C.c_int),
@@ -377,6 +379,22 @@ class ShallowWaterModel:
         check(lib().omega_model_compute_tendency(self._h, tend._h, state._h, float(t), int(add_mode),
                                                  _stream(stream)))
 
+    def _field_tendency(self, field, tend, state, add_mode, stream):
+        check(lib().omega_model_compute_field_tendency(self._h, field, tend._h, state._h,
+                                                       int(add_mode), _stream(stream)))
+
+    def compute_h_tendency(self, tend, state, add_mode: int = ADD_REPLACE, stream=None):
+        """compute_h_tendency (shallow_water_model.cpp:43-86): only tend's h."""
+        self._field_tendency(0, tend, state, add_mode, stream)
+
+    def compute_vn_tendency(self, tend, state, add_mode: int = ADD_REPLACE, stream=None):
+        """compute_vn_tendency (shallow_water_model.cpp:88-219): only tend's vn."""
+        self._field_tendency(1, tend, state, add_mode, stream)
+
+    def compute_tr_tendency(self, tend, state, add_mode: int = ADD_REPLACE, stream=None):
+        """compute_tr_tendency (shallow_water_model.cpp:221-311): only tend's tracers."""
+        self._field_tendency(2, tend, state, add_mode, stream)
+
     @property
     def terms(self) -> int:
         m = C.c_int()
@@ -436,6 +454,14 @@ class _Stepper:
 
     def set_graph(self, enable: bool = True):
         check(lib().omega_stepper_set_graph(self._h, int(bool(enable))))
+
+    def coeffs(self):
+        """(m_rka, m_rkb, m_rkc), nstages values each."""
+        n = C.c_int()
+        check(lib().omega_stepper_coeffs(self._h, C.byref(n), None, None, None))
+        a, b, c = (np.zeros(n.value) for _ in range(3))
+        check(lib().omega_stepper_coeffs(self._h, C.byref(n), _dp(a), _dp(b), _dp(c)))
+        return a, b, c
 
     def __del__(self):
         if getattr(self, "_h", None) is not None and _lib is not None:

@@ -222,3 +222,27 @@ def test_specialised_each_term_bitwise(terms):
     gmodel.compute_tendency(gt, gs, 0.0)
     for a, b in zip(gt.download(), tend_o):
         assert same_bits(a, b)
+
+
+@pytest.mark.parametrize("field", [0, 1, 2])
+def test_field_tendency_matches_full_tendency(field):
+    """compute_{h,vn,tr}_tendency write exactly their field of compute_tendency."""
+    om, omodel, gm, gmodel, state = make_pair(9, 9, 4, 2)
+    gs = upload(gm, 2, state)
+    full = swe.ShallowWaterState(gm, 2)
+    gmodel.compute_tendency(full, gs, 0.0)
+    rng = np.random.default_rng(field)
+    junk = tuple(rng.standard_normal(a.shape) for a in state)
+    part = upload(gm, 2, junk)
+    (gmodel.compute_h_tendency, gmodel.compute_vn_tendency, gmodel.compute_tr_tendency)[field](
+        part, gs)
+    for f, (a, b, j) in enumerate(zip(part.download(), full.download(), junk)):
+        assert same_bits(a, b if f == field else j), f
+
+
+def test_stepper_coefficients_match_oracle():
+    om, omodel, gm, gmodel, state = make_pair(4, 4, 1, 0)
+    for kind, n in (("rk4", 4), ("lsrk", 5), ("lsrk", 1)):
+        st = swe.RK4Stepper(gmodel) if kind == "rk4" else swe.LSRKStepper(gmodel, n)
+        for a, b in zip(st.coeffs(), po.Stepper(omodel, kind, n).coeffs()):
+            assert same_bits(a, b), kind
