@@ -438,3 +438,53 @@ def test_adjoint_rejects_hook():
     s = upload(gm, 1, state)
     with pytest.raises(swe.OmegaError, match="hook"):
         swe.tendency_vjp(m, s, s, swe.ShallowWaterState(gm, 1))
+
+
+# ---------------------------------------------------------------------------
+# torch.autograd.Function over the library (test harness only)
+# ---------------------------------------------------------------------------
+class LibraryStep(torch.autograd.Function):
+    """One library time step as an autograd node: forward runs the HIP step,
+    backward the library's step adjoint (CPU float64 tensors in and out,
+    original numbering)."""
+
+    @staticmethod
+    def forward(ctx, h, vn, tr, stepper, gm, dt):
+        ntr = tr.shape[0]
+        s = upload(gm, ntr, [x.detach().numpy() for x in (h, vn, tr)])
+        ctx.saved = (stepper, gm, dt, [x.detach().numpy().copy() for x in (h, vn, tr)])
+        stepper.do_step(0.0, dt, s)
+        return tuple(torch.from_numpy(a) for a in s.download())
+
+    @staticmethod
+    def backward(ctx, gh, gvn, gtr):
+        stepper, gm, dt, y = ctx.saved
+        ntr = y[2].shape[0]
+        cot = upload(gm, ntr, [g.contiguous().numpy() for g in (gh, gvn, gtr)])
+        swe.step_vjp(stepper, 0.0, dt, upload(gm, ntr, y), cot)
+        out = [torch.from_numpy(a) for a in cot.download()]
+        return out[0], out[1], out[2], None, None, None
+
+
+def test_autograd_function_wrapper_matches_torch_reference():
+    gm, gmodel, pd, state = setup(8, 8, 3, 2, seed=14)
+    st = swe.LSRKStepper(gmodel)
+    dt = 25.0
+    # two library steps chained through autograd, loss on the result
+    y = [torch.tensor(a, requires_grad=True) for a in state]
+    z = LibraryStep.apply(*y, st, gm, dt)
+    z = LibraryStep.apply(*z, st, gm, dt)
+    loss = (z[0] ** 2).sum() * 1e-6 + (z[1] * z[1]).sum() + (z[2] ** 3).sum() * 1e-9
+    g_lib = torch.autograd.grad(loss, y)
+    # the same through the torch restatement
+    tm = ts.TorchMesh(gm, 3)
+    om = po.Mesh(4, 4, DC, 1)
+    a, b, _ = po.Stepper(po.Model(om, ntracers=0), "lsrk", 5).coeffs()
+    y2 = [torch.tensor(s, requires_grad=True) for s in state]
+    w = ts.lsrk_step(tm, pd, ALL_TERMS, dt, a, b, *y2)
+    w = ts.lsrk_step(tm, pd, ALL_TERMS, dt, a, b, *w)
+    loss2 = (w[0] ** 2).sum() * 1e-6 + (w[1] * w[1]).sum() + (w[2] ** 3).sum() * 1e-9
+    g_ref = torch.autograd.grad(loss2, y2)
+    assert abs(float(loss) - float(loss2)) <= 1e-12 * abs(float(loss2))
+    for name, p, q in zip(("h", "vn", "tr"), g_lib, g_ref):
+        assert rel_err(p.numpy(), q.numpy()) <= TOL_REF, name
