@@ -344,6 +344,36 @@ int omega_adjoint_run(omega_stepper_t stepper, double t0, double dt, int nsteps,
                       omega_stream_t stream);
 
 /* ----------------------------------------------------------------------
+ * Ensembles.  No reference counterpart: the reference steps one model at a
+ * time.  Members are (model, state, dt) triples with their own meshes,
+ * layer counts, tracer counts and coefficients; one batch step advances
+ * every member by one do_step of the batch's stepper kind, with each stage
+ * phase (diagnostics, momentum del4 pre-pass, tendency + update) a single
+ * launch over all members.  Each member's state ends bit-identical to
+ * stepping it alone.  Members must not have an additional-tendency hook.
+ * -------------------------------------------------------------------- */
+typedef struct omega_batch_s *omega_batch_t;
+
+int omega_batch_create(int kind, int nstages, omega_batch_t *out);
+int omega_batch_destroy(omega_batch_t batch);
+/* Adds a member; *index (may be NULL) receives its position. */
+int omega_batch_add(omega_batch_t batch, omega_model_t model, omega_state_t state, double dt,
+                    int *index);
+int omega_batch_step(omega_batch_t batch, omega_stream_t stream);
+int omega_batch_run(omega_batch_t batch, int nsteps, omega_stream_t stream);
+/* Capture the ensemble step in a HIP graph (one per buffer parity). */
+int omega_batch_set_graph(omega_batch_t batch, int enable);
+/* Launch shape of the ensemble kernels: threads per block (64, 128, 256)
+ * and levels per lane (1 or 2). */
+int omega_batch_set_launch(omega_batch_t batch, int threads, int levels_per_thread);
+int omega_batch_get_launch(omega_batch_t batch, int *threads, int *levels_per_thread);
+/* Times reps steps per admissible launch shape, keeps the fastest and
+ * restores the members' states; *best_ms = that shape's time per step. */
+int omega_batch_autotune(omega_batch_t batch, int reps, double *best_ms, omega_stream_t stream);
+int omega_batch_info(omega_batch_t batch, int *nmembers, int64_t *blocks_per_launch,
+                     int64_t *cell_layers);
+
+/* ----------------------------------------------------------------------
  * Utilities
  * -------------------------------------------------------------------- */
 

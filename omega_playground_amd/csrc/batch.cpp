@@ -1,0 +1,403 @@
+// Ensembles (no reference counterpart): independent members -- each its own
+// mesh, layer count, tracer count, coefficients and time step -- advanced
+// together.  Every stage phase (diagnostics, momentum del4 pre-pass,
+// tendency + update) is one launch over all members' blocks, driven by a
+// table of per-member descriptors built once per (step parity, stage); the
+// step can be captured in a HIP graph.  The stage updates are the fused
+// epilogues of the single-model steppers (rk4.cpp:28-104, lsrk.cpp:36-89),
+// so each member advances bit for bit as it would alone.
+#include <algorithm>
+#include <map>
+
+#include "batch.hpp"
+#include "objects.hpp"
+
+using namespace omega_hip;
+
+struct omega_batch_s {
+  int kind = OMEGA_STEPPER_LSRK;
+  int nstages = 5;
+  std::vector<Real> rka, rkb, rkc;
+  struct Member {
+    omega_model_s *M;
+    omega_state_s *S;
+    double dt;
+    int cur0 = 0;                    // state buffer at parity 0
+    DBuf<Real> acc[3], p1[3], p2[3]; // RK4
+    DBuf<Real> tend[3];              // LSRK
+  };
+  std::vector<std::unique_ptr<Member>> mem;
+  int N = 2;         // levels per lane
+  int threads = 256; // threads per block
+  bool built = false;
+  int parity = 0;
+  int nblocks = 0;
+  bool any_vdel4 = false;
+  DBuf<int> blk;
+  DBuf<BatchMember> desc; // [parity][stage][member]
+  bool use_graph = false;
+  std::map<int, hipGraphExec_t> graphs;
+  hipStream_t cap_stream = nullptr;
+
+  ~omega_batch_s() {
+    drop_graphs();
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
+  }
+  void drop_graphs() {
+    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+  }
+};
+
+namespace {
+
+bool odd_flip(const omega_batch_s *B) {
+  return B->kind == OMEGA_STEPPER_LSRK && (B->nstages % 2 == 1);
+}
+
+// Buffers of state field f at buffer index c (allocating the second buffer).
+Real *state_buf(omega_state_s *S, int c, int f) {
+  if (c != S->cur) return S->alt(f);
+  return S->ptr(f);
+}
+
+void build_tables(omega_batch_s *B) {
+  OMEGA_REQUIRE(!B->mem.empty(), "batch: no members");
+  const int nm = (int)B->mem.size(), ns = B->nstages, N = B->N, T = B->threads;
+  std::vector<int> blk;
+  std::vector<BatchMember> geo(nm);
+  B->any_vdel4 = false;
+  for (int i = 0; i < nm; ++i) {
+    auto &mb = *B->mem[i];
+    omega_model_s *M = mb.M;
+    omega_state_s *S = mb.S;
+    const HostMesh &hm = M->mesh->dm.host;
+    OMEGA_REQUIRE(N == 1 || hm.nlayers % 2 == 0,
+                  "batch: two levels per lane need an even layer count in every member");
+    M->mesh->dm.upload();
+    if (M->terms & T_VDEL4) ensure_vdel4(M);
+    for (int f = 0; f < 3; ++f) (void)S->alt(f);
+    if (B->kind == OMEGA_STEPPER_RK4) {
+      ensure3(mb.acc, S);
+      ensure3(mb.p1, S);
+      ensure3(mb.p2, S);
+    } else {
+      ensure3(mb.tend, S);
+    }
+    BatchMember &g = geo[i];
+    int lanes = (hm.nlayers + N - 1) / N;
+    if (lanes > 64) lanes = 64;
+    int lx = 1;
+    while (lx < lanes) lx <<= 1;
+    g.lx = lx;
+    g.rows = std::max(1, T / lx);
+    g.chunks = (hm.nlayers + lx * N - 1) / (lx * N);
+    g.block0 = (int)blk.size();
+    const int nb = ((hm.ncells + g.rows - 1) / g.rows) * g.chunks;
+    blk.insert(blk.end(), nb, i);
+    const bool do_vn = !M->params.disable_vn_tendency;
+    const int ntr = M->params.ntracers;
+    const bool need_diag = do_vn || ((M->terms & T_TDEL4) && ntr > 0);
+    g.dterms = need_diag ? (do_vn ? M->terms : (M->terms & T_TDEL4)) : 0;
+    g.vdel4 = do_vn && (M->terms & T_VDEL4);
+    B->any_vdel4 = B->any_vdel4 || g.vdel4;
+    mb.cur0 = S->cur;
+  }
+  B->nblocks = (int)blk.size();
+  std::vector<BatchMember> tab((size_t)2 * ns * nm);
+  for (int p = 0; p < 2; ++p)
+    for (int s = 0; s < ns; ++s)
+      for (int i = 0; i < nm; ++i) {
+        auto &mb = *B->mem[i];
+        omega_model_s *M = mb.M;
+        omega_state_s *S = mb.S;
+        BatchMember g = geo[i];
+        g.m = M->mesh->dm.view();
+        g.mc = M->coef();
+        g.d = M->diag();
+        StageArgs a{};
+        a.terms = M->terms;
+        a.do_h = !M->params.disable_h_tendency;
+        a.do_vn = !M->params.disable_vn_tendency;
+        a.ntr = M->params.ntracers;
+        const double dt = mb.dt;
+        if (B->kind == OMEGA_STEPPER_RK4) {
+          // rk4_fused_step's buffer rotation
+          const int c = mb.cur0;
+          Real *Y[3], *A[3], *P1[3], *P2[3];
+          for (int f = 0; f < 3; ++f) {
+            Y[f] = state_buf(S, c, f);
+            A[f] = mb.acc[f].p;
+            P1[f] = mb.p1[f].p;
+            P2[f] = mb.p2[f].p;
+          }
+          a.base_from_tend = 0;
+          a.db = dt * B->rkb[s];
+          a.da = s < 3 ? dt * B->rka[s] : 0.0;
+          Real **in = s == 0 ? Y : (s == 2 ? P2 : P1);
+          Real **out = s == 0 ? P1 : (s == 1 ? P2 : (s == 2 ? P1 : Y));
+          a.epi = s == 0 ? EPI_RK4_FIRST : (s == 3 ? EPI_RK4_LAST : EPI_RK4_MID);
+          a.h = fio(in[0], nullptr, A[0], Y[0], out[0]);
+          a.vn = fio(in[1], nullptr, A[1], Y[1], out[1]);
+          a.tr = fio(in[2], nullptr, A[2], Y[2], out[2]);
+        } else {
+          // lsrk_fused_step: stage s reads buffer (cur0 ^ parity ^ s) and
+          // writes the other one
+          const int steps_parity = odd_flip(B) ? p : 0;
+          const int cin = (mb.cur0 ^ steps_parity ^ (s & 1)) & 1;
+          a.epi = EPI_LSRK;
+          a.a = B->rka[s];
+          a.base_from_tend = B->rka[s] != 0.0;
+          a.db = dt * B->rkb[s];
+          Real *in[3], *out[3];
+          for (int f = 0; f < 3; ++f) {
+            in[f] = state_buf(S, cin, f);
+            out[f] = state_buf(S, 1 - cin, f);
+          }
+          a.h = fio(in[0], mb.tend[0].p, nullptr, nullptr, out[0]);
+          a.vn = fio(in[1], mb.tend[1].p, nullptr, nullptr, out[1]);
+          a.tr = fio(in[2], mb.tend[2].p, nullptr, nullptr, out[2]);
+        }
+        g.a = a;
+        tab[((size_t)p * ns + s) * nm + i] = g;
+      }
+  B->blk.upload(blk);
+  if (B->desc.n != tab.size()) B->desc.alloc(tab.size());
+  OMEGA_HIP_CHECK(hipMemcpy(B->desc.p, tab.data(), tab.size() * sizeof(BatchMember),
+                            hipMemcpyHostToDevice));
+  B->parity = 0;
+  B->drop_graphs();
+  B->built = true;
+}
+
+// The tables hold buffer addresses for the states' current buffer parity;
+// a member stepped on its own in between (or re-allocated buffers) forces
+// a rebuild.
+void ensure_tables(omega_batch_s *B) {
+  bool ok = B->built;
+  if (ok)
+    for (auto &mbp : B->mem) {
+      const auto &mb = *mbp;
+      const int expect = odd_flip(B) ? (mb.cur0 ^ B->parity) : mb.cur0;
+      if (mb.S->cur != expect) ok = false;
+    }
+  if (!ok) build_tables(B);
+}
+
+void step_launches(omega_batch_s *B, int parity, hipStream_t s) {
+  const int nm = (int)B->mem.size();
+  for (int st = 0; st < B->nstages; ++st) {
+    const BatchMember *d = B->desc.p + ((size_t)parity * B->nstages + st) * nm;
+    launch_batch_diag(d, B->blk.p, B->nblocks, B->threads, B->N, s);
+    if (B->any_vdel4) launch_batch_vdel4(d, B->blk.p, B->nblocks, B->threads, B->N, s);
+    launch_batch_tend(d, B->blk.p, B->nblocks, B->threads, B->N, s);
+  }
+}
+
+void batch_step(omega_batch_s *B, hipStream_t s) {
+  ensure_tables(B);
+  const int p = B->parity;
+  if (B->use_graph && !stream_capturing(s)) {
+    auto it = B->graphs.find(p);
+    if (it == B->graphs.end()) {
+      if (!B->cap_stream)
+        OMEGA_HIP_CHECK(hipStreamCreateWithFlags(&B->cap_stream, hipStreamNonBlocking));
+      hipGraph_t g;
+      OMEGA_HIP_CHECK(hipStreamBeginCapture(B->cap_stream, hipStreamCaptureModeThreadLocal));
+      try {
+        step_launches(B, p, B->cap_stream);
+      } catch (...) {
+        hipGraph_t dummy;
+        (void)hipStreamEndCapture(B->cap_stream, &dummy);
+        throw;
+      }
+      OMEGA_HIP_CHECK(hipStreamEndCapture(B->cap_stream, &g));
+      hipGraphExec_t ex;
+      OMEGA_HIP_CHECK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+      OMEGA_HIP_CHECK(hipGraphDestroy(g));
+      it = B->graphs.emplace(p, ex).first;
+    }
+    OMEGA_HIP_CHECK(hipGraphLaunch(it->second, s));
+  } else {
+    step_launches(B, p, s);
+  }
+  if (odd_flip(B)) {
+    for (auto &mb : B->mem) mb->S->cur = 1 - mb->S->cur;
+    B->parity ^= 1;
+  }
+}
+
+double time_steps(omega_batch_s *B, int reps, hipStream_t s) {
+  hipEvent_t a, b;
+  OMEGA_HIP_CHECK(hipEventCreate(&a));
+  OMEGA_HIP_CHECK(hipEventCreate(&b));
+  batch_step(B, s); // warm-up (and graph capture)
+  OMEGA_HIP_CHECK(hipEventRecord(a, s));
+  for (int i = 0; i < reps; ++i) batch_step(B, s);
+  OMEGA_HIP_CHECK(hipEventRecord(b, s));
+  OMEGA_HIP_CHECK(hipEventSynchronize(b));
+  float ms = 0;
+  OMEGA_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms / std::max(reps, 1);
+}
+
+} // namespace
+
+extern "C" {
+
+int omega_batch_create(int kind, int nstages, omega_batch_t *out) {
+  return guarded([&] {
+    OMEGA_REQUIRE(out, "null argument");
+    OMEGA_REQUIRE(kind == OMEGA_STEPPER_RK4 || kind == OMEGA_STEPPER_LSRK, "bad stepper kind");
+    OMEGA_REQUIRE(kind == OMEGA_STEPPER_RK4 || nstages >= 1, "nstages must be >= 1");
+    auto B = std::make_unique<omega_batch_s>();
+    B->kind = kind;
+    B->nstages = kind == OMEGA_STEPPER_RK4 ? 4 : nstages;
+    stepper_coefficients(kind, B->nstages, B->rka, B->rkb, B->rkc);
+    *out = B.release();
+  });
+}
+
+int omega_batch_destroy(omega_batch_t B) {
+  return guarded([&] { delete B; });
+}
+
+int omega_batch_add(omega_batch_t B, omega_model_t M, omega_state_t S, double dt, int *index) {
+  return guarded([&] {
+    OMEGA_REQUIRE(B && M, "null argument");
+    check_state(M, S, "batch member");
+    OMEGA_REQUIRE(M->add_fn == nullptr,
+                  "batch: members cannot have an additional-tendency hook");
+    for (auto &mb : B->mem)
+      OMEGA_REQUIRE(mb->S != S, "batch: a state can belong to one member only");
+    // two levels per lane need an even layer count in every member
+    if (M->mesh->dm.host.nlayers % 2 != 0) B->N = 1;
+    auto mb = std::make_unique<omega_batch_s::Member>();
+    mb->M = M;
+    mb->S = S;
+    mb->dt = dt;
+    B->mem.push_back(std::move(mb));
+    B->built = false;
+    if (index) *index = (int)B->mem.size() - 1;
+  });
+}
+
+int omega_batch_step(omega_batch_t B, omega_stream_t stream) {
+  return guarded([&] {
+    OMEGA_REQUIRE(B, "null batch");
+    batch_step(B, as_stream(stream));
+  });
+}
+
+int omega_batch_run(omega_batch_t B, int nsteps, omega_stream_t stream) {
+  return guarded([&] {
+    OMEGA_REQUIRE(B, "null batch");
+    OMEGA_REQUIRE(nsteps >= 0, "nsteps must be >= 0");
+    for (int i = 0; i < nsteps; ++i) batch_step(B, as_stream(stream));
+  });
+}
+
+int omega_batch_set_graph(omega_batch_t B, int enable) {
+  return guarded([&] {
+    OMEGA_REQUIRE(B, "null batch");
+    B->use_graph = enable != 0;
+  });
+}
+
+int omega_batch_set_launch(omega_batch_t B, int threads, int levels_per_thread) {
+  return guarded([&] {
+    OMEGA_REQUIRE(B, "null batch");
+    OMEGA_REQUIRE(threads == 64 || threads == 128 || threads == 256,
+                  "threads must be 64, 128 or 256");
+    OMEGA_REQUIRE(levels_per_thread == 1 || levels_per_thread == 2,
+                  "levels_per_thread must be 1 or 2");
+    B->threads = threads;
+    B->N = levels_per_thread;
+    B->built = false;
+  });
+}
+
+int omega_batch_get_launch(omega_batch_t B, int *threads, int *levels_per_thread) {
+  return guarded([&] {
+    OMEGA_REQUIRE(B, "null batch");
+    if (threads) *threads = B->threads;
+    if (levels_per_thread) *levels_per_thread = B->N;
+  });
+}
+
+// Times `reps` steps for every launch shape the ensemble admits and keeps
+// the fastest.  The members' states are restored afterwards, so the
+// ensemble continues from where it was.
+int omega_batch_autotune(omega_batch_t B, int reps, double *best_ms, omega_stream_t stream) {
+  return guarded([&] {
+    OMEGA_REQUIRE(B && !B->mem.empty(), "batch: no members");
+    hipStream_t s = as_stream(stream);
+    reps = std::max(1, reps);
+    // save every member's state
+    std::vector<std::array<DBuf<Real>, 3>> saved(B->mem.size());
+    std::vector<int> cur(B->mem.size());
+    for (size_t i = 0; i < B->mem.size(); ++i) {
+      omega_state_s *S = B->mem[i]->S;
+      cur[i] = S->cur;
+      for (int f = 0; f < 3; ++f) {
+        saved[i][f].alloc(S->n[f]);
+        if (S->n[f])
+          OMEGA_HIP_CHECK(hipMemcpyAsync(saved[i][f].p, S->ptr(f), S->n[f] * sizeof(Real),
+                                         hipMemcpyDeviceToDevice, s));
+      }
+    }
+    bool even_layers = true;
+    for (auto &mb : B->mem) even_layers = even_layers && mb->M->mesh->dm.host.nlayers % 2 == 0;
+    const int keepN = B->N, keepT = B->threads;
+    double best = -1;
+    int bestN = keepN, bestT = keepT;
+    for (int N : {2, 1}) {
+      if (N == 2 && !even_layers) continue;
+      for (int T : {256, 128, 64}) {
+        B->N = N;
+        B->threads = T;
+        B->built = false;
+        const double ms = time_steps(B, reps, s);
+        if (best < 0 || ms < best) {
+          best = ms;
+          bestN = N;
+          bestT = T;
+        }
+      }
+    }
+    B->N = bestN;
+    B->threads = bestT;
+    B->built = false;
+    // restore the states
+    for (size_t i = 0; i < B->mem.size(); ++i) {
+      omega_state_s *S = B->mem[i]->S;
+      S->cur = cur[i];
+      for (int f = 0; f < 3; ++f)
+        if (S->n[f])
+          OMEGA_HIP_CHECK(hipMemcpyAsync(S->ptr(f), saved[i][f].p, S->n[f] * sizeof(Real),
+                                         hipMemcpyDeviceToDevice, s));
+    }
+    OMEGA_HIP_CHECK(hipStreamSynchronize(s));
+    if (best_ms) *best_ms = best;
+  });
+}
+
+int omega_batch_info(omega_batch_t B, int *nmembers, int64_t *blocks_per_launch,
+                     int64_t *cell_layers) {
+  return guarded([&] {
+    OMEGA_REQUIRE(B, "null batch");
+    if (!B->built && !B->mem.empty()) build_tables(B);
+    if (nmembers) *nmembers = (int)B->mem.size();
+    if (blocks_per_launch) *blocks_per_launch = B->nblocks;
+    if (cell_layers) {
+      int64_t n = 0;
+      for (auto &mb : B->mem)
+        n += (int64_t)mb->M->mesh->dm.host.ncells * mb->M->mesh->dm.host.nlayers;
+      *cell_layers = n;
+    }
+  });
+}
+
+} // extern "C"

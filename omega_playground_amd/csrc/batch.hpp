@@ -1,0 +1,32 @@
+// Ensembles: members of different meshes, layer counts, tracer counts and
+// coefficients advanced together, one launch per stage phase for the whole
+// ensemble, through per-member descriptor tables.
+#pragma once
+
+#include "internal.hpp"
+
+namespace omega_hip {
+
+// Everything one member needs for one stage; a table of these per
+// (step parity, stage) lives in device memory.
+struct BatchMember {
+  MeshView m;
+  ModelCoef mc;
+  DiagOut d;
+  StageArgs a;
+  int dterms;   // diagnostics terms (0: no diagnostics pass needed)
+  int vdel4;    // momentum del4 pre-pass needed
+  int lx, rows; // lanes per row and rows (entities) per block
+  int chunks;   // level chunks (blocks per cell segment)
+  int block0;   // first block of this member in the launch
+};
+
+// One launch each over all members' blocks; blk[b] = member of block b.
+void launch_batch_diag(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
+                       hipStream_t s);
+void launch_batch_vdel4(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
+                        hipStream_t s);
+void launch_batch_tend(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
+                       hipStream_t s);
+
+} // namespace omega_hip

@@ -581,32 +581,10 @@ int omega_stepper_create(omega_model_t M, int kind, int nstages, omega_stepper_t
     S->kind = kind;
     const HostMesh &h = M->mesh->dm.host;
     const size_t L = h.nlayers, ntr = M->params.ntracers;
-    if (kind == OMEGA_STEPPER_RK4) {
-      // rk4.cpp:14-25
-      S->nstages = 4;
-      S->rka = {1. / 2, 1. / 2, 1};
-      S->rkb = {1. / 6, 1. / 3, 1. / 3, 1. / 6};
-      S->rkc = {1. / 2, 1. / 2, 1};
-    } else {
-      OMEGA_REQUIRE(nstages >= 1, "nstages must be >= 1");
-      S->nstages = nstages;
-      S->rka.assign(nstages, 0.0);
-      S->rkb.assign(nstages, 0.0);
-      S->rkc.assign(nstages, 0.0);
-      if (nstages == 5) {
-        // Carpenter-Kennedy (5,4) 2N-storage coefficients (lsrk.cpp:16-28)
-        S->rka = {0., -567301805773. / 1357537059087., -2404267990393. / 2016746695238.,
-                  -3550918686646. / 2091501179385., -1275806237668. / 842570457699.};
-        S->rkb = {1432997174477. / 9575080441755., 5161836677717. / 13612068292357.,
-                  1720146321549. / 2090206949498., 3134564353537. / 4481467310338.,
-                  2277821191437. / 14882151754819.};
-        S->rkc = {0., 1432997174477. / 9575080441755., 2526269341429. / 6820363962896.,
-                  2006345519317. / 3224310063776., 2802321613138. / 2924317926251.};
-      } else if (nstages == 1) { // forward Euler (lsrk.cpp:29-33)
-        S->rka = {0};
-        S->rkb = {1};
-        S->rkc = {0};
-      }
+    OMEGA_REQUIRE(kind == OMEGA_STEPPER_RK4 || nstages >= 1, "nstages must be >= 1");
+    S->nstages = kind == OMEGA_STEPPER_RK4 ? 4 : nstages;
+    stepper_coefficients(kind, S->nstages, S->rka, S->rkb, S->rkc);
+    if (kind == OMEGA_STEPPER_LSRK) {
       // tendency storage starts at zero (lsrk.cpp:10-14)
       S->tend[0].alloc((size_t)h.ncells * L);
       S->tend[1].alloc((size_t)h.nedges * L);

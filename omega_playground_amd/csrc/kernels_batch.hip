@@ -1,0 +1,132 @@
+// Ensemble kernels: the forward operators of fwd_ops.hpp applied to every
+// member of an ensemble in one launch.  Block b belongs to member blk[b]
+// (wavefront-uniform, so the member's descriptor comes through scalar
+// loads); inside the member the block covers `rows` cells of its
+// space-filling-curve order and `lx` lanes x N levels of one level chunk,
+// exactly as the single-model kernels do, so each member's results are the
+// same bits as stepping it alone.
+#include "batch.hpp"
+#include "fwd_ops.hpp"
+
+namespace omega_hip {
+
+namespace {
+
+using namespace dev;
+using namespace fwd;
+
+struct Place {
+  int c0, c1, k, ty, R;
+  bool active;
+};
+
+__device__ __forceinline__ Place place(const BatchMember &B, int lb, int N) {
+  Place p;
+  const int chunk = lb % B.chunks, seg = lb / B.chunks;
+  const int tx = (int)threadIdx.x % B.lx;
+  p.ty = (int)threadIdx.x / B.lx;
+  p.R = B.rows;
+  p.c0 = seg * p.R;
+  p.c1 = min(p.c0 + p.R, B.m.nc);
+  p.k = (chunk * B.lx + tx) * N;
+  p.active = p.ty < p.R && p.c0 < B.m.nc && p.k < B.m.L;
+  return p;
+}
+
+template <int N>
+__global__ void __launch_bounds__(256)
+    k_batch_diag(const BatchMember *__restrict__ desc, const int *__restrict__ blk) {
+  const int id = uniform(blk[blockIdx.x]);
+  const BatchMember &B = desc[id];
+  if (!B.dterms) return;
+  const Place p = place(B, (int)blockIdx.x - B.block0, N);
+  if (!p.active) return;
+  const MeshView &m = B.m;
+  const DiagOut &d = B.d;
+  const int terms = B.dterms, ntr = B.a.ntr;
+  const double *h = B.a.h.in, *vn = B.a.vn.in, *tr = B.a.tr.in;
+  if (terms & (T_PV | T_VDEL2 | T_VDEL4)) {
+    const int vb = m.vofs[p.c0], ve = m.vofs[p.c1];
+    for (int v = vb + p.ty; v < ve; v += p.R) vertex_diag<N>(m, v, p.k, h, vn, d);
+  }
+  for (int c = p.c0 + p.ty; c < p.c1; c += p.R) cell_diag<N>(m, terms, ntr, c, p.k, h, vn, tr, d);
+}
+
+template <int N>
+__global__ void __launch_bounds__(256)
+    k_batch_vdel4_edge(const BatchMember *__restrict__ desc, const int *__restrict__ blk) {
+  const int id = uniform(blk[blockIdx.x]);
+  const BatchMember &B = desc[id];
+  if (!B.vdel4) return;
+  const Place p = place(B, (int)blockIdx.x - B.block0, N);
+  if (!p.active) return;
+  const MeshView &m = B.m;
+  const DiagOut &d = B.d;
+  const int eb = m.eofs[p.c0], ee = m.eofs[p.c1];
+  for (int e = eb + p.ty; e < ee; e += p.R) vdel4_edge<N>(m, d, e, p.k);
+}
+
+template <int N>
+__global__ void __launch_bounds__(256)
+    k_batch_vdel4_vc(const BatchMember *__restrict__ desc, const int *__restrict__ blk) {
+  const int id = uniform(blk[blockIdx.x]);
+  const BatchMember &B = desc[id];
+  if (!B.vdel4) return;
+  const Place p = place(B, (int)blockIdx.x - B.block0, N);
+  if (!p.active) return;
+  const MeshView &m = B.m;
+  const DiagOut &d = B.d;
+  const int vb = m.vofs[p.c0], ve = m.vofs[p.c1];
+  for (int v = vb + p.ty; v < ve; v += p.R) vdel4_vertex<N>(m, d, v, p.k);
+  for (int c = p.c0 + p.ty; c < p.c1; c += p.R) vdel4_cell<N>(m, d, c, p.k);
+}
+
+template <int N>
+__global__ void __launch_bounds__(256)
+    k_batch_tend(const BatchMember *__restrict__ desc, const int *__restrict__ blk) {
+  const int id = uniform(blk[blockIdx.x]);
+  const BatchMember &B = desc[id];
+  const Place p = place(B, (int)blockIdx.x - B.block0, N);
+  if (!p.active) return;
+  // the descriptor is read in place (global memory, wavefront-uniform
+  // address, scalar loads); copying it out would put it on the stack
+  const MeshView &m = B.m;
+  const ModelCoef &mc = B.mc;
+  const StageArgs &a = B.a;
+  const DiagOut &d = B.d;
+  const int eb = m.eofs[p.c0], ee = m.eofs[p.c1];
+  for (int e = eb + p.ty; e < ee; e += p.R) edge_tend<N>(m, mc, a, d, e, p.k);
+  for (int c = p.c0 + p.ty; c < p.c1; c += p.R) cell_tend<N>(m, mc, a, d, c, p.k);
+}
+
+} // namespace
+
+#define OMEGA_BATCH_LAUNCH(K)                                                               \
+  do {                                                                                      \
+    if (N == 2) hipLaunchKernelGGL(K<2>, dim3(nblocks), dim3(threads), 0, s, desc, blk);    \
+    else hipLaunchKernelGGL(K<1>, dim3(nblocks), dim3(threads), 0, s, desc, blk);           \
+    OMEGA_HIP_CHECK(hipGetLastError());                                                     \
+  } while (0)
+
+void launch_batch_diag(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
+                       hipStream_t s) {
+  TimerScope ts("batch_diag", s);
+  OMEGA_BATCH_LAUNCH(k_batch_diag);
+}
+
+void launch_batch_vdel4(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
+                        hipStream_t s) {
+  TimerScope ts("batch_vdel4", s);
+  OMEGA_BATCH_LAUNCH(k_batch_vdel4_edge);
+  OMEGA_BATCH_LAUNCH(k_batch_vdel4_vc);
+}
+
+void launch_batch_tend(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
+                       hipStream_t s) {
+  TimerScope ts("batch_tend", s);
+  OMEGA_BATCH_LAUNCH(k_batch_tend);
+}
+
+#undef OMEGA_BATCH_LAUNCH
+
+} // namespace omega_hip

@@ -139,6 +139,9 @@ void ensure3(DBuf<Real> (&b)[3], omega_state_s *S);
 void compute_tendency_ptrs(omega_model_s *M, Real *ht, Real *vt, Real *tt, const Real *h,
                            const Real *vn, const Real *tr, double t, int add_mode,
                            hipStream_t s, int fields = 7);
+// RK4 tableau (rk4.cpp:14-25) or the low-storage coefficients (lsrk.cpp:16-33)
+void stepper_coefficients(int kind, int nstages, std::vector<Real> &rka, std::vector<Real> &rkb,
+                          std::vector<Real> &rkc);
 void step_once(omega_stepper_s *St, omega_state_s *S, double t, double dt, hipStream_t s);
 void do_step_maybe_graph(omega_stepper_s *St, omega_state_s *S, double t, double dt,
                          hipStream_t s);

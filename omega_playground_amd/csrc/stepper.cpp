@@ -137,6 +137,34 @@ void compute_tendency_ptrs(omega_model_s *M, Real *ht, Real *vt, Real *tt, const
 // Steppers
 // ---------------------------------------------------------------------------
 
+void stepper_coefficients(int kind, int nstages, std::vector<Real> &rka, std::vector<Real> &rkb,
+                          std::vector<Real> &rkc) {
+  if (kind == OMEGA_STEPPER_RK4) {
+    // rk4.cpp:14-25
+    rka = {1. / 2, 1. / 2, 1};
+    rkb = {1. / 6, 1. / 3, 1. / 3, 1. / 6};
+    rkc = {1. / 2, 1. / 2, 1};
+    return;
+  }
+  rka.assign(nstages, 0.0);
+  rkb.assign(nstages, 0.0);
+  rkc.assign(nstages, 0.0);
+  if (nstages == 5) {
+    // Carpenter-Kennedy (5,4) 2N-storage coefficients (lsrk.cpp:16-28)
+    rka = {0., -567301805773. / 1357537059087., -2404267990393. / 2016746695238.,
+           -3550918686646. / 2091501179385., -1275806237668. / 842570457699.};
+    rkb = {1432997174477. / 9575080441755., 5161836677717. / 13612068292357.,
+           1720146321549. / 2090206949498., 3134564353537. / 4481467310338.,
+           2277821191437. / 14882151754819.};
+    rkc = {0., 1432997174477. / 9575080441755., 2526269341429. / 6820363962896.,
+           2006345519317. / 3224310063776., 2802321613138. / 2924317926251.};
+  } else if (nstages == 1) { // forward Euler (lsrk.cpp:29-33)
+    rka = {0};
+    rkb = {1};
+    rkc = {0};
+  }
+}
+
 // RK4 with a fused tendency + update per stage (rk4.cpp:28-104 reorganised:
 // the accumulator lives in acc until the last stage writes the state).
 void rk4_fused_step(omega_stepper_s *St, omega_state_s *S, double dt, hipStream_t s) {

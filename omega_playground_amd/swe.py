@@ -127,6 +127,16 @@ def lib():
         "omega_misfit": ([vp, vp, vp, dp, vp, dp, vp], C.c_int),
         "omega_adjoint_run": ([vp, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int64, vp, vp,
                                dp, dp, vp, dp, vp], C.c_int),
+        "omega_batch_create": ([C.c_int, C.c_int, pp], C.c_int),
+        "omega_batch_destroy": ([vp], C.c_int),
+        "omega_batch_add": ([vp, vp, vp, C.c_double, ip], C.c_int),
+        "omega_batch_step": ([vp, vp], C.c_int),
+        "omega_batch_run": ([vp, C.c_int, vp], C.c_int),
+        "omega_batch_set_graph": ([vp, C.c_int], C.c_int),
+        "omega_batch_set_launch": ([vp, C.c_int, C.c_int], C.c_int),
+        "omega_batch_get_launch": ([vp, ip, ip], C.c_int),
+        "omega_batch_autotune": ([vp, C.c_int, dp, vp], C.c_int),
+        "omega_batch_info": ([vp, ip, i64p, i64p], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -573,6 +583,58 @@ def adjoint_run(stepper: "_Stepper", t0: float, dt: float, nsteps: int, state: S
                                   int(byte_cap), state._h, obs._h, w, C.byref(J), grad._h, ptr,
                                   _stream(stream)))
     return J.value, np.array(buf[:])
+
+
+# ---------------------------------------------------------------------------
+# Ensembles (no reference counterpart)
+# ---------------------------------------------------------------------------
+class Batch:
+    """Ensemble of independent (model, state, dt) members advanced together,
+    one launch per stage phase for all members."""
+
+    def __init__(self, kind: str = "lsrk", nstages: int = 5):
+        h = C.c_void_p()
+        check(lib().omega_batch_create(0 if kind == "rk4" else 1, int(nstages), C.byref(h)))
+        self._h = h
+        self._members = []  # keeps models and states alive
+
+    def add(self, model: ShallowWaterModel, state: ShallowWaterState, dt: float) -> int:
+        idx = C.c_int()
+        check(lib().omega_batch_add(self._h, model._h, state._h, float(dt), C.byref(idx)))
+        self._members.append((model, state))
+        return idx.value
+
+    def step(self, stream=None):
+        check(lib().omega_batch_step(self._h, _stream(stream)))
+
+    def run(self, nsteps: int, stream=None):
+        check(lib().omega_batch_run(self._h, int(nsteps), _stream(stream)))
+
+    def set_graph(self, enable: bool = True):
+        check(lib().omega_batch_set_graph(self._h, int(bool(enable))))
+
+    def set_launch(self, threads: int = 256, levels_per_thread: int = 2):
+        check(lib().omega_batch_set_launch(self._h, int(threads), int(levels_per_thread)))
+
+    def get_launch(self):
+        t, n = C.c_int(), C.c_int()
+        check(lib().omega_batch_get_launch(self._h, C.byref(t), C.byref(n)))
+        return t.value, n.value
+
+    def autotune(self, reps: int = 3, stream=None) -> float:
+        ms = C.c_double()
+        check(lib().omega_batch_autotune(self._h, int(reps), C.byref(ms), _stream(stream)))
+        return ms.value
+
+    def info(self):
+        n, b, w = C.c_int(), C.c_int64(), C.c_int64()
+        check(lib().omega_batch_info(self._h, C.byref(n), C.byref(b), C.byref(w)))
+        return {"members": n.value, "blocks_per_launch": b.value, "cell_layers": w.value}
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _lib is not None:
+            _lib.omega_batch_destroy(self._h)
+            self._h = None
 
 
 def _integral(fn, state, model, stream=None):

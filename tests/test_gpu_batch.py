@@ -1,0 +1,131 @@
+"""GPU tests of ensembles: members with different meshes, layer counts,
+tracer counts, coefficients and time steps advanced together must end
+bit-identical to stepping each member alone (which the parity tests tie to
+the oracle), with and without graph capture, after autotuning, and when a
+member is also stepped on its own in between."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from omega_playground_amd import swe
+
+from helpers import params_dict, rich_state, same_bits
+
+pytestmark = pytest.mark.gpu
+
+# (nx, ny, nlayers, ntracers, dc, params overrides, dt)
+MEMBERS = [
+    (8, 6, 4, 1, 10e3, dict(eddy_diff2=0.0, eddy_diff4=0.0, visc_del4=0.0), 20.0),
+    (10, 10, 6, 3, 10e3, {}, 15.0),
+    (12, 8, 2, 0, 5e3, dict(visc_del4=0.0), 10.0),
+    (6, 6, 8, 2, 20e3, dict(disable_vn_tendency=True), 30.0),
+    (8, 8, 64, 1, 10e3, dict(visc_del2=0.0), 12.0),
+]
+
+
+def make_members(members, seed=0):
+    out = []
+    for i, (nx, ny, L, ntr, dc, over, dt) in enumerate(members):
+        pd = params_dict(dc, ntr)
+        pd.update(over)
+        mesh = swe.PlanarHexagonalMesh(nx, ny, dc, L)
+        model = swe.ShallowWaterModel(mesh, swe.ShallowWaterParams(**pd))
+        y0 = rich_state(mesh, L, ntr, seed + i)
+        out.append((mesh, model, ntr, y0, dt, pd))
+    return out
+
+
+def state_of(mesh, ntr, y):
+    s = swe.ShallowWaterState(mesh, ntr)
+    s.upload(*y)
+    return s
+
+
+def solo(members, kind, nstages, nsteps):
+    res = []
+    for mesh, model, ntr, y0, dt, _ in members:
+        st = swe.RK4Stepper(model) if kind == "rk4" else swe.LSRKStepper(model, nstages)
+        s = state_of(mesh, ntr, y0)
+        st.run(0.0, dt, nsteps, s)
+        res.append(s.download())
+    return res
+
+
+@pytest.mark.parametrize("kind,nstages", [("lsrk", 5), ("rk4", 4), ("lsrk", 1)])
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_matches_members_stepped_alone(kind, nstages, graph):
+    print("case", kind, nstages, graph, flush=True)
+    members = make_members(MEMBERS)
+    ref = solo(members, kind, nstages, 3)
+    b = swe.Batch(kind, nstages)
+    states = []
+    for mesh, model, ntr, y0, dt, _ in members:
+        s = state_of(mesh, ntr, y0)
+        b.add(model, s, dt)
+        states.append(s)
+    b.set_graph(graph)
+    b.run(3)
+    for i, (s, r) in enumerate(zip(states, ref)):
+        for name, a, c in zip(("h", "vn", "tr"), s.download(), r):
+            assert same_bits(a, c), (i, name)
+
+
+def test_batch_member_matches_oracle():
+    members = make_members(MEMBERS[1:2])
+    mesh, model, ntr, y0, dt, pd = members[0]
+    b = swe.Batch("lsrk", 5)
+    s = state_of(mesh, ntr, y0)
+    b.add(model, s, dt)
+    b.run(2)
+    om = po.Mesh(10, 10, 10e3, 6)
+    ref = tuple(a.copy() for a in y0)
+    po.Stepper(po.Model(om, **pd), "lsrk", 5).run(0.0, dt, 2, ref)
+    for name, a, c in zip(("h", "vn", "tr"), s.download(), ref):
+        assert same_bits(a, c), name
+
+
+def test_batch_autotune_and_interleaved_solo_steps():
+    members = make_members(MEMBERS, seed=4)
+    ref = solo(members, "lsrk", 5, 4)
+    b = swe.Batch("lsrk", 5)
+    b.set_graph(True)
+    states = []
+    for mesh, model, ntr, y0, dt, _ in members:
+        s = state_of(mesh, ntr, y0)
+        b.add(model, s, dt)
+        states.append(s)
+    ms = b.autotune(reps=2)
+    assert ms > 0
+    threads, lpt = b.get_launch()
+    assert threads in (64, 128, 256) and lpt in (1, 2)
+    b.step()
+    # the second step: member 1 alone, the others in a second ensemble; the
+    # first ensemble then finds the buffers flipped and rebuilds its tables
+    mesh1, model1, ntr1, _, dt1, _ = members[1]
+    swe.LSRKStepper(model1).do_step(0.0, dt1, states[1])
+    b2 = swe.Batch("lsrk", 5)
+    for i, (mesh, model, ntr, y0, dt, _) in enumerate(members):
+        if i != 1:
+            b2.add(model, states[i], dt)
+    b2.step()
+    b.run(2)   # the first ensemble rebuilds its tables for the new buffer parity
+    for i, (s, r) in enumerate(zip(states, ref)):
+        for name, a, c in zip(("h", "vn", "tr"), s.download(), r):
+            assert same_bits(a, c), (i, name)
+
+
+def test_batch_rejects_hook_and_shared_state():
+    members = make_members(MEMBERS[:1])
+    mesh, model, ntr, y0, dt, pd = members[0]
+
+    class Forced(swe.ShallowWaterModel):
+        def additional_tendency(self, h_tend, vn_tend, h, vn, t, stream):
+            return 0
+
+    b = swe.Batch()
+    s = state_of(mesh, ntr, y0)
+    with pytest.raises(swe.OmegaError, match="hook"):
+        b.add(Forced(mesh, swe.ShallowWaterParams(**pd)), s, dt)
+    b.add(model, s, dt)
+    with pytest.raises(swe.OmegaError, match="one member"):
+        b.add(model, s, dt)
