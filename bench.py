@@ -151,6 +151,84 @@ def adjoint_bench(n_snapshots: int = 10, nsteps: int | None = None):
     }
 
 
+def batched_bench(nmembers: int = 48, nsteps: int = 50):
+    """The ensemble workload: 48 seeded members (nx in {32..192}, 16/32/64
+    layers, 1-4 tracers, double vortex with offset centres and random dH),
+    LSRK5, 50 steps, del2 viscosity 1e3 and tracer del4 1e9 as the forward
+    workload; one autotuned, graph-captured launch per stage phase for the
+    whole ensemble against a sequential per-member loop (each member its own
+    graph-captured stepper) on the same GPU."""
+    specs = wl.ensemble_members(nmembers, seed=0)
+    members = []
+    for sp in specs:
+        mesh = swe.PlanarHexagonalMesh(sp["nx"], sp["nx"], sp["dc"], sp["nlayers"])
+        params = swe.ShallowWaterParams(f0=1e-4, grav=9.81, visc_del2=1e3, eddy_diff4=1e9,
+                                        ntracers=sp["ntracers"])
+        model = swe.ShallowWaterModel(mesh, params)
+        y0 = wl.gaussian_double_vortex(mesh, sp["nlayers"], sp["ntracers"], seed=sp["seed"],
+                                       dh=sp["dh"], centres=sp["centres"])
+        members.append((mesh, model, sp, y0, wl.dt_rule(sp["dc"])))
+    work = sum(m[0].ncells * m[2]["nlayers"] for m in members) * nsteps
+
+    def fresh():
+        out = []
+        for mesh, model, sp, y0, dt in members:
+            s = swe.ShallowWaterState(mesh, sp["ntracers"])
+            s.upload(*y0)
+            out.append(s)
+        return out
+
+    # batched
+    states_b = fresh()
+    batch = swe.Batch("lsrk", 5)
+    for (mesh, model, sp, y0, dt), s in zip(members, states_b):
+        batch.add(model, s, dt)
+    batch.set_graph(True)
+    tune_ms = batch.autotune(reps=3)
+    batch.run(2)   # capture the graphs of both buffer parities
+    swe.synchronize()
+    for (mesh, model, sp, y0, dt), s in zip(members, states_b):
+        s.upload(*y0)
+    t0 = time.perf_counter()
+    batch.run(nsteps)
+    swe.synchronize()
+    t_batch = time.perf_counter() - t0
+    # looped: one member after another, each with its own graph-captured stepper
+    states_l = fresh()
+    steppers = []
+    for mesh, model, sp, y0, dt in members:
+        st = swe.LSRKStepper(model, 5)
+        st.set_graph(True)
+        steppers.append(st)
+    for st, (mesh, model, sp, y0, dt), s in zip(steppers, members, states_l):
+        st.run(0.0, dt, 2, s)   # capture both buffer parities
+        s.upload(*y0)
+    swe.synchronize()
+    t0 = time.perf_counter()
+    for st, (mesh, model, sp, y0, dt), s in zip(steppers, members, states_l):
+        st.run(0.0, dt, nsteps, s)
+    swe.synchronize()
+    t_loop = time.perf_counter() - t0
+    identical = all(
+        all(np.array_equal(a, b) for a, b in zip(sb.download(), sl.download()))
+        for sb, sl in zip(states_b, states_l))
+    info = batch.info()
+    return {
+        "workload": f"{nmembers} members, nx in {{32..192}}, 16/32/64 layers, 1-4 tracers, "
+                    f"LSRK5, {nsteps} steps, double vortex, del2 visc 1e3, tracer del4 1e9",
+        "cell_layers": info["cell_layers"],
+        "blocks_per_launch": info["blocks_per_launch"],
+        "launch": list(batch.get_launch()),
+        "autotune_ms_per_step": tune_ms,
+        "batched_s": t_batch,
+        "looped_s": t_loop,
+        "speedup": t_loop / t_batch,
+        "batched_cell_layer_steps_per_s": work / t_batch,
+        "looped_cell_layer_steps_per_s": work / t_loop,
+        "bit_identical": bool(identical),
+    }
+
+
 def copy_calibration():
     """A plain device copy of known size, for scaling the HBM byte counters:
     1 GiB read + 1 GiB written per copy kernel."""
@@ -189,6 +267,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--skip-adjoint", action="store_true")
+    ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--launch", default="", help="rows,lpt,swizzle,levels_per_block")
     args = ap.parse_args()
@@ -260,6 +339,13 @@ def main():
         log(f"[bench] adjoint: step ratio {adj['ratio_per_step']:.2f}, trajectory ratio "
             f"{adj['ratio_total']:.2f} ({adj['adjoint_run_s']:.3f} s)")
 
+    bat = None
+    if not args.skip_batched:
+        log("[bench] ensemble workload")
+        bat = batched_bench()
+        log(f"[bench] ensemble: batched {bat['batched_s']:.3f} s vs looped {bat['looped_s']:.3f} s "
+            f"= {bat['speedup']:.2f}x, bit-identical {bat['bit_identical']}, launch {bat['launch']}")
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
@@ -300,7 +386,7 @@ def main():
                           "frac": step_achieved / HBM_PEAK_GBS, "diag_kernel_avg_ms": avg_diag_ms},
         "finite": finite,
         "adjoint": adj,
-        "batched": None,
+        "batched": bat,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

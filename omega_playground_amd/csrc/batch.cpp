@@ -31,9 +31,10 @@ struct omega_batch_s {
   int threads = 256; // threads per block
   bool built = false;
   int parity = 0;
-  int nblocks = 0;
+  // two launch groups: 0 general kernels, 1 pattern-specialised kernels
+  int nblocks[2] = {0, 0};
   bool any_vdel4 = false;
-  DBuf<int> blk;
+  DBuf<int> blk[2];
   DBuf<BatchMember> desc; // [parity][stage][member]
   bool use_graph = false;
   std::map<int, hipGraphExec_t> graphs;
@@ -64,7 +65,7 @@ Real *state_buf(omega_state_s *S, int c, int f) {
 void build_tables(omega_batch_s *B) {
   OMEGA_REQUIRE(!B->mem.empty(), "batch: no members");
   const int nm = (int)B->mem.size(), ns = B->nstages, N = B->N, T = B->threads;
-  std::vector<int> blk;
+  std::vector<int> blk[2];
   std::vector<BatchMember> geo(nm);
   B->any_vdel4 = false;
   for (int i = 0; i < nm; ++i) {
@@ -85,16 +86,25 @@ void build_tables(omega_batch_s *B) {
       ensure3(mb.tend, S);
     }
     BatchMember &g = geo[i];
-    int lanes = (hm.nlayers + N - 1) / N;
-    if (lanes > 64) lanes = 64;
-    int lx = 1;
-    while (lx < lanes) lx <<= 1;
-    g.lx = lx;
-    g.rows = std::max(1, T / lx);
-    g.chunks = (hm.nlayers + lx * N - 1) / (lx * N);
-    g.block0 = (int)blk.size();
+    g.hex = hex_usable(M->mesh->dm.view(), M->lc) ? 1 : 0;
+    if (g.hex) {
+      // one row of 64/N lanes covers 64 levels
+      g.lx = 64 / N;
+      g.rows = 256 / g.lx;
+      g.chunks = hm.nlayers / 64;
+    } else {
+      int lanes = (hm.nlayers + N - 1) / N;
+      if (lanes > 64) lanes = 64;
+      int lx = 1;
+      while (lx < lanes) lx <<= 1;
+      g.lx = lx;
+      g.rows = std::max(1, T / lx);
+      g.chunks = (hm.nlayers + lx * N - 1) / (lx * N);
+    }
+    std::vector<int> &bl = blk[g.hex];
+    g.block0 = (int)bl.size();
     const int nb = ((hm.ncells + g.rows - 1) / g.rows) * g.chunks;
-    blk.insert(blk.end(), nb, i);
+    bl.insert(bl.end(), nb, i);
     const bool do_vn = !M->params.disable_vn_tendency;
     const int ntr = M->params.ntracers;
     const bool need_diag = do_vn || ((M->terms & T_TDEL4) && ntr > 0);
@@ -103,7 +113,7 @@ void build_tables(omega_batch_s *B) {
     B->any_vdel4 = B->any_vdel4 || g.vdel4;
     mb.cur0 = S->cur;
   }
-  B->nblocks = (int)blk.size();
+  for (int q = 0; q < 2; ++q) B->nblocks[q] = (int)blk[q].size();
   std::vector<BatchMember> tab((size_t)2 * ns * nm);
   for (int p = 0; p < 2; ++p)
     for (int s = 0; s < ns; ++s)
@@ -161,7 +171,7 @@ void build_tables(omega_batch_s *B) {
         g.a = a;
         tab[((size_t)p * ns + s) * nm + i] = g;
       }
-  B->blk.upload(blk);
+  for (int q = 0; q < 2; ++q) B->blk[q].upload(blk[q]);
   if (B->desc.n != tab.size()) B->desc.alloc(tab.size());
   OMEGA_HIP_CHECK(hipMemcpy(B->desc.p, tab.data(), tab.size() * sizeof(BatchMember),
                             hipMemcpyHostToDevice));
@@ -188,9 +198,16 @@ void step_launches(omega_batch_s *B, int parity, hipStream_t s) {
   const int nm = (int)B->mem.size();
   for (int st = 0; st < B->nstages; ++st) {
     const BatchMember *d = B->desc.p + ((size_t)parity * B->nstages + st) * nm;
-    launch_batch_diag(d, B->blk.p, B->nblocks, B->threads, B->N, s);
-    if (B->any_vdel4) launch_batch_vdel4(d, B->blk.p, B->nblocks, B->threads, B->N, s);
-    launch_batch_tend(d, B->blk.p, B->nblocks, B->threads, B->N, s);
+    const int *g0 = B->blk[0].p, *g1 = B->blk[1].p;
+    const int n0 = B->nblocks[0], n1 = B->nblocks[1];
+    if (n0) launch_batch_diag(d, g0, n0, B->threads, B->N, s);
+    if (n1) launch_batch_diag_hex(d, g1, n1, B->N, s);
+    if (B->any_vdel4) {
+      if (n0) launch_batch_vdel4(d, g0, n0, B->threads, B->N, s);
+      if (n1) launch_batch_vdel4(d, g1, n1, 256, B->N, s);
+    }
+    if (n0) launch_batch_tend(d, g0, n0, B->threads, B->N, s);
+    if (n1) launch_batch_tend_hex(d, g1, n1, B->N, s);
   }
 }
 
@@ -390,7 +407,7 @@ int omega_batch_info(omega_batch_t B, int *nmembers, int64_t *blocks_per_launch,
     OMEGA_REQUIRE(B, "null batch");
     if (!B->built && !B->mem.empty()) build_tables(B);
     if (nmembers) *nmembers = (int)B->mem.size();
-    if (blocks_per_launch) *blocks_per_launch = B->nblocks;
+    if (blocks_per_launch) *blocks_per_launch = B->nblocks[0] + B->nblocks[1];
     if (cell_layers) {
       int64_t n = 0;
       for (auto &mb : B->mem)

@@ -18,7 +18,8 @@ struct BatchMember {
   int vdel4;    // momentum del4 pre-pass needed
   int lx, rows; // lanes per row and rows (entities) per block
   int chunks;   // level chunks (blocks per cell segment)
-  int block0;   // first block of this member in the launch
+  int block0;   // first block of this member in its launch
+  int hex;      // member runs the pattern-specialised kernels
 };
 
 // One launch each over all members' blocks; blk[b] = member of block b.
@@ -28,5 +29,12 @@ void launch_batch_vdel4(const BatchMember *desc, const int *blk, int nblocks, in
                         hipStream_t s);
 void launch_batch_tend(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
                        hipStream_t s);
+// The same phases for members on the regular hexagonal mesh with a layer
+// count that is a multiple of 64 (pattern-specialised operators; 256
+// threads, 64/N lanes per row).
+void launch_batch_diag_hex(const BatchMember *desc, const int *blk, int nblocks, int N,
+                           hipStream_t s);
+void launch_batch_tend_hex(const BatchMember *desc, const int *blk, int nblocks, int N,
+                           hipStream_t s);
 
 } // namespace omega_hip

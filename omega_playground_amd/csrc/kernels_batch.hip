@@ -7,6 +7,7 @@
 // same bits as stepping it alone.
 #include "batch.hpp"
 #include "fwd_ops.hpp"
+#include "hex_ops.hpp"
 
 namespace omega_hip {
 
@@ -99,6 +100,50 @@ __global__ void __launch_bounds__(256)
   for (int c = p.c0 + p.ty; c < p.c1; c += p.R) cell_tend<N>(m, mc, a, d, c, p.k);
 }
 
+// Pattern-specialised phases: one cell (its 3 owned edges and 2 owned
+// vertices) per row of 64/N lanes.
+template <int N>
+__global__ void __launch_bounds__(256)
+    k_batch_diag_hex(const BatchMember *__restrict__ desc, const int *__restrict__ blk) {
+  const int id = uniform(blk[blockIdx.x]);
+  const BatchMember &B = desc[id];
+  if (!B.dterms) return;
+  const int lb = (int)blockIdx.x - B.block0;
+  const int chunk = lb % B.chunks, seg = lb / B.chunks;
+  const int c = hexops::ent<N>(seg * B.rows + (int)threadIdx.x / B.lx);
+  if (c >= B.m.nc) return;
+  const int k = (chunk * B.lx + (int)threadIdx.x % B.lx) * N;
+  const MeshView &m = B.m;
+  const DiagOut &d = B.d;
+  const double *h = B.a.h.in, *vn = B.a.vn.in, *tr = B.a.tr.in;
+  if (B.dterms & (T_PV | T_VDEL2 | T_VDEL4)) {
+    hexops::vertex_diag_u<N>(m, 2 * c, k, h, vn, d);
+    hexops::vertex_diag_u<N>(m, 2 * c + 1, k, h, vn, d);
+  }
+  hexops::cell_diag_u<N>(m, B.dterms, B.a.ntr, c, k, h, vn, tr, d);
+}
+
+template <int N>
+__global__ void __launch_bounds__(256)
+    k_batch_tend_hex(const BatchMember *__restrict__ desc, const int *__restrict__ blk) {
+  const int id = uniform(blk[blockIdx.x]);
+  const BatchMember &B = desc[id];
+  const int lb = (int)blockIdx.x - B.block0;
+  const int chunk = lb % B.chunks, seg = lb / B.chunks;
+  const int c = hexops::ent<N>(seg * B.rows + (int)threadIdx.x / B.lx);
+  if (c >= B.m.nc) return;
+  const int k = (chunk * B.lx + (int)threadIdx.x % B.lx) * N;
+  const MeshView &m = B.m;
+  const ModelCoef &mc = B.mc;
+  const StageArgs &a = B.a;
+  const DiagOut &d = B.d;
+  const int e0 = 3 * c;
+  hexops::edge_tend_hex<N, 0>(m, mc, a, d, e0, k);
+  hexops::edge_tend_hex<N, 1>(m, mc, a, d, e0 + 1, k);
+  hexops::edge_tend_hex<N, 2>(m, mc, a, d, e0 + 2, k);
+  hexops::cell_tend_hex<N>(m, mc, a, d, c, k);
+}
+
 } // namespace
 
 #define OMEGA_BATCH_LAUNCH(K)                                                               \
@@ -125,6 +170,20 @@ void launch_batch_tend(const BatchMember *desc, const int *blk, int nblocks, int
                        hipStream_t s) {
   TimerScope ts("batch_tend", s);
   OMEGA_BATCH_LAUNCH(k_batch_tend);
+}
+
+void launch_batch_diag_hex(const BatchMember *desc, const int *blk, int nblocks, int N,
+                           hipStream_t s) {
+  TimerScope ts("batch_diag", s);
+  const int threads = 256;
+  OMEGA_BATCH_LAUNCH(k_batch_diag_hex);
+}
+
+void launch_batch_tend_hex(const BatchMember *desc, const int *blk, int nblocks, int N,
+                           hipStream_t s) {
+  TimerScope ts("batch_tend", s);
+  const int threads = 256;
+  OMEGA_BATCH_LAUNCH(k_batch_tend_hex);
 }
 
 #undef OMEGA_BATCH_LAUNCH

@@ -15,140 +15,14 @@
 // so results are the same bits as the general kernels.  The vector-memory
 // address path of the CU is what bounds these kernels, so the aim throughout
 // is the fewest load instructions per level.
-#include "device_common.hpp"
-#include "hex_pattern.hpp"
+#include "hex_ops.hpp"
 
 namespace omega_hip {
 
 namespace {
 
 using namespace dev;
-
-// Entity index, made wavefront-uniform when one entity fills the wavefront.
-template <int N> __device__ __forceinline__ int ent(int x) {
-  if constexpr (N == 1) return uniform(x);
-  else return x;
-}
-
-template <int N>
-__device__ __forceinline__ Vd<N> rowv(const double *__restrict__ p, int e, int L, int k) {
-  return ld<N>(p + (size_t)e * L + k);
-}
-
-// ---------------------------------------------------------------------------
-// Diagnostics
-// ---------------------------------------------------------------------------
-
-// VorticityAuxVars::compute_vort_vertex (vorticity_aux.hpp:85-116)
-template <int N>
-__device__ __forceinline__ void vertex_diag_u(const MeshView &m, int v, int k,
-                                              const double *__restrict__ h,
-                                              const double *__restrict__ vn, const DiagOut &d) {
-  const int L = m.L;
-  int e[3], c[3];
-  double crv[3], cth[3];
-#pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    e[j] = m.eov[3 * v + j];
-    c[j] = m.cov[3 * v + j];
-    crv[j] = m.crv[3 * v + j];
-    cth[j] = m.cth[3 * v + j];
-  }
-  Vd<N> ve[3], hc[3];
-#pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    ve[j] = rowv<N>(vn, e[j], L, k);
-    hc[j] = rowv<N>(h, c[j], L, k);
-  }
-  Vd<N> rv = zero<N>(), th = zero<N>();
-#pragma unroll
-  for (int j = 0; j < 3; ++j)
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      rv[i] = rv[i] + crv[j] * ve[j][i];
-      th[i] = th[i] + cth[j] * hc[j][i];
-    }
-  Vd<N> inv;
-#pragma unroll
-  for (int i = 0; i < N; ++i) inv[i] = 1.0 / th[i];
-  st<N>(d.rvort_v + (size_t)v * L + k, rv);
-  st<N>(d.inv_thick_v + (size_t)v * L + k, inv);
-}
-
-// KineticAuxVars::compute_kinetic_cell (kinetic_aux.hpp:44-73) and
-// TracerHyperDiffusionOnCell::compute_tracer_del2
-// (tracer_hyperdiff_cell.hpp:94-129), norm_tr = tr*(1/h) formed on the fly.
-template <int N>
-__device__ __forceinline__ void cell_diag_u(const MeshView &m, int terms, int ntr, int c, int k,
-                                            const double *__restrict__ h,
-                                            const double *__restrict__ vn,
-                                            const double *__restrict__ tr, const DiagOut &d) {
-  const int L = m.L, me = m.me;
-  int E[6];
-#pragma unroll
-  for (int j = 0; j < 6; ++j) E[j] = m.eoc[c * me + j];
-  if (terms & (T_KE | T_VDEL2 | T_VDEL4)) {
-    Vd<N> ve[6];
-#pragma unroll
-    for (int j = 0; j < 6; ++j) ve[j] = rowv<N>(vn, E[j], L, k);
-    Vd<N> ke = zero<N>(), div = zero<N>();
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-      const double cke = m.cke[c * me + j];
-      const double cd = m.cdiv[c * me + j];
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        ke[i] = ke[i] + cke * ve[j][i] * ve[j][i] * 0.25;
-        div[i] = div[i] + cd * ve[j][i];
-      }
-    }
-    st<N>(d.ke_c + (size_t)c * L + k, ke);
-    st<N>(d.div_c + (size_t)c * L + k, div);
-  }
-  if ((terms & T_TDEL4) && ntr > 0) {
-    int NB[6];
-#pragma unroll
-    for (int j = 0; j < 6; ++j) NB[j] = m.coc[c * 6 + j];
-    const Vd<N> hC = rowv<N>(h, c, L, k);
-    Vd<N> hN[6], invN[6], cm[6], invC;
-    double idc[6];
-#pragma unroll
-    for (int j = 0; j < 6; ++j) hN[j] = rowv<N>(h, NB[j], L, k);
-#pragma unroll
-    for (int i = 0; i < N; ++i) invC[i] = 1.0 / hC[i];
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-      const double cd = m.cdiv[c * me + j];
-      idc[j] = m.inv_dc[E[j]];
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        invN[j][i] = 1.0 / hN[j][i];
-        // h_mean = 0.5*(h[cell0] + h[cell1]); a two-term sum is the same in
-        // either order
-        cm[j][i] = cd * (0.5 * (hN[j][i] + hC[i]));
-      }
-    }
-    for (int l = 0; l < ntr; ++l) {
-      const double *trl = tr + (size_t)l * m.nc * L;
-      const Vd<N> tC = rowv<N>(trl, c, L, k);
-      Vd<N> tN[6];
-#pragma unroll
-      for (int j = 0; j < 6; ++j) tN[j] = rowv<N>(trl, NB[j], L, k);
-      Vd<N> d2 = zero<N>();
-#pragma unroll
-      for (int j = 0; j < 6; ++j)
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-          const double nC = tC[i] * invC[i];
-          const double nN = tN[j][i] * invN[j][i];
-          // grad = (norm_tr[cell1] - norm_tr[cell0]) * inv_dc
-          const double grad = (hex::cell_is_second(j) ? (nC - nN) : (nN - nC)) * idc[j];
-          d2[i] = d2[i] + cm[j][i] * grad;
-        }
-      st<N>(d.trdel2_c + ((size_t)l * m.nc + c) * L + k, d2);
-    }
-  }
-}
+using namespace hexops;
 
 template <int N>
 __global__ void __launch_bounds__(256)
@@ -164,218 +38,6 @@ __global__ void __launch_bounds__(256)
     vertex_diag_u<N>(m, 2 * c + 1, k, h, vn, d);
   }
   cell_diag_u<N>(m, terms, ntr, c, k, h, vn, tr, d);
-}
-
-// ---------------------------------------------------------------------------
-// Tendencies + stage update
-// ---------------------------------------------------------------------------
-
-// compute_vn_tendency for an edge of class K (shallow_water_model.cpp:186-217;
-// pv_flux_edge.hpp:56-84, ke_grad_edge.hpp:37-51, ssh_grad_edge.hpp:37-50,
-// vel_diff_edge.hpp:61-86, vel_hyperdiff_edge.hpp:212-237)
-template <int N, int K>
-__device__ __forceinline__ void edge_tend_hex(const MeshView &m, const ModelCoef &mc,
-                                              const StageArgs &a, const DiagOut &d, int e, int k) {
-  const int L = m.L;
-  const size_t idx = (size_t)e * L + k;
-  if (!a.do_vn) {
-    if (a.epi == EPI_STORE) return;
-    epilogue<N>(a, a.vn, idx, tend_base<N>(a, a.vn, idx));
-    return;
-  }
-  const double *__restrict__ h = a.h.in;
-  const double *__restrict__ vn = a.vn.in;
-  const int *rec = m.hexrec + (size_t)e * hex::kRecInts;
-  int ej[10], uv[10], uc[10];
-#pragma unroll
-  for (int j = 0; j < 10; ++j) {
-    ej[j] = rec[j];
-    uv[j] = rec[10 + j];
-    uc[j] = rec[20 + j];
-  }
-  constexpr int o0 = hex::OWNV[K][0], o1 = hex::OWNV[K][1];
-  constexpr int q0 = hex::OWNC[K][0], q1 = hex::OWNC[K][1];
-  const bool pv = a.terms & T_PV;
-  Vd<N> hU[10], rvU[10], itU[10], vJ[10];
-#pragma unroll
-  for (int u = 0; u < 10; ++u) hU[u] = rowv<N>(h, uc[u], L, k);
-  if (pv) {
-#pragma unroll
-    for (int j = 0; j < 10; ++j) vJ[j] = rowv<N>(vn, ej[j], L, k);
-#pragma unroll
-    for (int u = 0; u < 10; ++u) {
-      rvU[u] = rowv<N>(d.rvort_v, uv[u], L, k);
-      itU[u] = rowv<N>(d.inv_thick_v, uv[u], L, k);
-    }
-  } else if (a.terms & T_VDEL2) {
-    rvU[o0] = rowv<N>(d.rvort_v, uv[o0], L, k);
-    rvU[o1] = rowv<N>(d.rvort_v, uv[o1], L, k);
-  }
-  const int c0 = uc[q0], c1 = uc[q1];
-  Vd<N> T = tend_base<N>(a, a.vn, idx);
-  const double idc = m.inv_dc[e];
-  if (pv) {
-    Vd<N> nrvU[10], npvU[10];
-#pragma unroll
-    for (int u = 0; u < 10; ++u) {
-      const double f = mc.f_v[uv[u]];
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        nrvU[u][i] = itU[u][i] * rvU[u][i];
-        npvU[u][i] = itU[u][i] * f;
-      }
-    }
-    double w[10];
-#pragma unroll
-    for (int j = 0; j < 10; ++j) w[j] = m.w[e * 10 + j];
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      const double qs = 0.5 * (nrvU[o0][i] + nrvU[o1][i]) + 0.5 * (npvU[o0][i] + npvU[o1][i]);
-      double qt = 0.0;
-#pragma unroll
-      for (int j = 0; j < 10; ++j) {
-        const double fh = 0.5 * (hU[hex::LC0[K][j]][i] + hU[hex::LC1[K][j]][i]);
-        const double nrv = 0.5 * (nrvU[hex::LV0[K][j]][i] + nrvU[hex::LV1[K][j]][i]);
-        const double npv = 0.5 * (npvU[hex::LV0[K][j]][i] + npvU[hex::LV1[K][j]][i]);
-        const double nv = (qs + nrv + npv) * 0.5;
-        qt = qt + w[j] * fh * vJ[j][i] * nv;
-      }
-      T[i] = T[i] + qt;
-    }
-  }
-  if (a.terms & T_KE) {
-    const Vd<N> k0 = rowv<N>(d.ke_c, c0, L, k), k1 = rowv<N>(d.ke_c, c1, L, k);
-#pragma unroll
-    for (int i = 0; i < N; ++i) T[i] = T[i] - (k1[i] - k0[i]) * idc;
-  }
-  if (a.terms & T_SSH) {
-#pragma unroll
-    for (int i = 0; i < N; ++i) T[i] = T[i] - mc.grav * (hU[q1][i] - hU[q0][i]) * idc;
-  }
-  if (a.terms & (T_VDEL2 | T_VDEL4)) {
-    const int ml = m.maxlev_e[e];
-    const double idv = m.inv_dv[e];
-    if (a.terms & T_VDEL2) {
-      const Vd<N> d0 = rowv<N>(d.div_c, c0, L, k), d1 = rowv<N>(d.div_c, c1, L, k);
-      const double cv = mc.cv2[e];
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        const double del2u = (d1[i] - d0[i]) * idc - (rvU[o1][i] - rvU[o0][i]) * idv;
-        T[i] = T[i] + ((k + i < ml) ? cv : 0.0) * del2u;
-      }
-    }
-    if (a.terms & T_VDEL4) {
-      const int b0 = uv[o0], b1 = uv[o1];
-      const Vd<N> d0 = rowv<N>(d.vdel2_div_c, c0, L, k), d1 = rowv<N>(d.vdel2_div_c, c1, L, k);
-      const Vd<N> r0 = rowv<N>(d.vdel2_rv_v, b0, L, k), r1 = rowv<N>(d.vdel2_rv_v, b1, L, k);
-      const double cv = mc.cv4[e];
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        const double del2u = (d1[i] - d0[i]) * idc - (r1[i] - r0[i]) * idv;
-        T[i] = T[i] - ((k + i < ml) ? cv : 0.0) * del2u;
-      }
-    }
-  }
-  epilogue<N>(a, a.vn, idx, T);
-}
-
-// compute_h_tendency / compute_tr_tendency for one cell
-// (thickness_hadv_cell.hpp:48-72, tracer_hadv_cell.hpp:60-90,
-// tracer_diff_cell.hpp:61-92, tracer_hyperdiff_cell.hpp:131-161)
-template <int N>
-__device__ __forceinline__ void cell_tend_hex(const MeshView &m, const ModelCoef &mc,
-                                              const StageArgs &a, const DiagOut &d, int c, int k) {
-  const int L = m.L, me = m.me, NC = m.nc;
-  const double *__restrict__ h = a.h.in;
-  const double *__restrict__ vn = a.vn.in;
-  int E[6], NB[6];
-#pragma unroll
-  for (int j = 0; j < 6; ++j) {
-    E[j] = m.eoc[c * me + j];
-    NB[j] = m.coc[c * 6 + j];
-  }
-  const Vd<N> hC = rowv<N>(h, c, L, k);
-  Vd<N> hN[6], vE[6], fh[6];
-  double cd[6];
-#pragma unroll
-  for (int j = 0; j < 6; ++j) {
-    hN[j] = rowv<N>(h, NB[j], L, k);
-    vE[j] = rowv<N>(vn, E[j], L, k);
-  }
-#pragma unroll
-  for (int j = 0; j < 6; ++j) {
-    cd[j] = m.cdiv[c * me + j];
-#pragma unroll
-    for (int i = 0; i < N; ++i) fh[j][i] = 0.5 * (hN[j][i] + hC[i]);
-  }
-  const size_t hidx = (size_t)c * L + k;
-  if (a.do_h) {
-    Vd<N> T = tend_base<N>(a, a.h, hidx);
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      double acc = 0.0;
-#pragma unroll
-      for (int j = 0; j < 6; ++j) acc = acc - cd[j] * fh[j][i] * vE[j][i];
-      T[i] = T[i] + acc;
-    }
-    epilogue<N>(a, a.h, hidx, T);
-  } else if (a.epi != EPI_STORE) {
-    epilogue<N>(a, a.h, hidx, tend_base<N>(a, a.h, hidx));
-  }
-  const int ntr = a.ntr;
-  if (ntr <= 0) return;
-  const bool td2 = a.terms & T_TDEL2, td4 = a.terms & T_TDEL4;
-  Vd<N> invC, invN[6];
-  double idc[6], c2[6], s2[6], c4[6];
-#pragma unroll
-  for (int i = 0; i < N; ++i) invC[i] = 1.0 / hC[i];
-#pragma unroll
-  for (int j = 0; j < 6; ++j) {
-#pragma unroll
-    for (int i = 0; i < N; ++i) invN[j][i] = 1.0 / hN[j][i];
-    idc[j] = m.inv_dc[E[j]];
-    c2[j] = td2 ? mc.ct2[c * me + j] : 0.0;
-    s2[j] = td2 ? m.s2[E[j]] : 0.0;
-    c4[j] = td4 ? mc.ct4[c * me + j] : 0.0;
-  }
-  for (int l = 0; l < ntr; ++l) {
-    const double *trl = a.tr.in + (size_t)l * NC * L;
-    const Vd<N> tC = rowv<N>(trl, c, L, k);
-    Vd<N> tN[6], dN[6], dC = zero<N>();
-#pragma unroll
-    for (int j = 0; j < 6; ++j) tN[j] = rowv<N>(trl, NB[j], L, k);
-    if (td4) {
-      const double *dl = d.trdel2_c + (size_t)l * NC * L;
-      dC = rowv<N>(dl, c, L, k);
-#pragma unroll
-      for (int j = 0; j < 6; ++j) dN[j] = rowv<N>(dl, NB[j], L, k);
-    }
-    const size_t tidx = ((size_t)l * NC + c) * L + k;
-    Vd<N> T = tend_base<N>(a, a.tr, tidx);
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      const double nC = tC[i] * invC[i];
-      double adv = 0.0, dif = 0.0, hyp = 0.0;
-#pragma unroll
-      for (int j = 0; j < 6; ++j) {
-        const double nN = tN[j][i] * invN[j][i];
-        const bool second = hex::cell_is_second(j);
-        adv = adv - cd[j] * fh[j][i] * ((nN + nC) * 0.5) * vE[j][i];
-        if (td2) {
-          const double grad = (second ? (nC - nN) : (nN - nC)) * idc[j];
-          dif = dif + c2[j] * fh[j][i] * s2[j] * grad;
-        }
-        if (td4) {
-          const double grad = (second ? (dC[i] - dN[j][i]) : (dN[j][i] - dC[i])) * idc[j];
-          hyp = hyp - c4[j] * grad;
-        }
-      }
-      T[i] = T[i] + adv;
-      if (td2) T[i] = T[i] + dif;
-      if (td4) T[i] = T[i] + hyp;
-    }
-    epilogue<N>(a, a.tr, tidx, T);
-  }
 }
 
 template <int N>

@@ -268,6 +268,24 @@ def adjoint_observations(mesh, cfg, seed=0):
     return np.ascontiguousarray(h + pert[:, None]), vn, tr
 
 
+def ensemble_members(nmembers=48, seed=0, nx_choices=(32, 48, 64, 96, 128, 192),
+                     layer_choices=(16, 32, 64), max_tracers=4, dc=10e3):
+    """Seeded member list of the ensemble workload: nx from nx_choices,
+    layers from layer_choices, 1..max_tracers tracers, double-vortex centres
+    offset by up to +-0.05 of the domain and dH uniform in [50, 100] m."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(nmembers):
+        nx = int(rng.choice(nx_choices))
+        out.append(dict(
+            nx=nx, dc=dc, nlayers=int(rng.choice(layer_choices)),
+            ntracers=int(rng.integers(1, max_tracers + 1)),
+            centres=((0.4 + rng.uniform(-0.05, 0.05), 0.5 + rng.uniform(-0.05, 0.05)),
+                     (0.6 + rng.uniform(-0.05, 0.05), 0.5 + rng.uniform(-0.05, 0.05))),
+            dh=float(rng.uniform(50.0, 100.0)), seed=seed * 1000 + i))
+    return out
+
+
 # Workload table of the throughput metric -----------------------------------
 CONFIGS = {
     "c1": dict(nx=64, dc=10e3, nlayers=64, ntracers=1, stepper="rk4", nsteps=10,

@@ -51,11 +51,15 @@ def solo(members, kind, nstages, nsteps):
     return res
 
 
+ODD = [(7, 5, 5, 2, 10e3, {}, 25.0)]
+
+
 @pytest.mark.parametrize("kind,nstages", [("lsrk", 5), ("rk4", 4), ("lsrk", 1)])
 @pytest.mark.parametrize("graph", [False, True])
-def test_batch_matches_members_stepped_alone(kind, nstages, graph):
-    print("case", kind, nstages, graph, flush=True)
-    members = make_members(MEMBERS)
+@pytest.mark.parametrize("odd", [False, True])
+def test_batch_matches_members_stepped_alone(kind, nstages, graph, odd):
+    print("case", kind, nstages, graph, odd, flush=True)
+    members = make_members(MEMBERS + (ODD if odd else []))
     ref = solo(members, kind, nstages, 3)
     b = swe.Batch(kind, nstages)
     states = []
@@ -65,6 +69,7 @@ def test_batch_matches_members_stepped_alone(kind, nstages, graph):
         states.append(s)
     b.set_graph(graph)
     b.run(3)
+    assert b.get_launch()[1] == (1 if odd else 2)
     for i, (s, r) in enumerate(zip(states, ref)):
         for name, a, c in zip(("h", "vn", "tr"), s.download(), r):
             assert same_bits(a, c), (i, name)
