@@ -270,6 +270,9 @@ def main():
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--launch", default="", help="rows,lpt,swizzle,levels_per_block")
+    ap.add_argument("--variant", type=int, default=None,
+                    help="0 specialised if possible, 1 general, 2 LDS-tiled")
+    ap.add_argument("--tiling", default="", help="cells_per_tile,levels_per_block")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -285,6 +288,10 @@ def main():
     cfg, mesh, params, model, state0 = build_case(args.config, args.ordering)
     if args.launch:
         model.set_launch(*(int(x) for x in args.launch.split(",")))
+    if args.variant is not None:
+        model.set_variant(args.variant)
+    if args.tiling:
+        model.set_tiling(*(int(x) for x in args.tiling.split(",")))
     ntr, L, nc = cfg["ntracers"], cfg["nlayers"], mesh.ncells
     dt = wl.dt_rule(cfg["dc"], cfg["grav"])
     state = swe.ShallowWaterState(mesh, ntr)
@@ -292,7 +299,7 @@ def main():
     stepper = swe.RK4Stepper(model) if cfg["stepper"] == "rk4" else swe.LSRKStepper(model, 5)
     stepper.set_graph(not args.no_graph)
     log(f"[bench] {args.config}: nx={cfg['nx']} cells={nc} L={L} ntr={ntr} dt={dt:.4f}s "
-        f"launch={model.get_launch()} graph={not args.no_graph}")
+        f"launch={model.get_launch()} kernels={model.kernel_variant()} graph={not args.no_graph}")
 
     # warm-up
     stepper.run(0.0, dt, args.warmup, state)

@@ -234,12 +234,21 @@ int omega_model_get_launch(omega_model_t model, int *rows_per_block, int *levels
 
 /* Kernel family: 0 (default) uses the pattern-specialised kernels when the
  * mesh is a regular hexagonal mesh verified against the built-in stencil
- * pattern and nlayers is a multiple of 64, 1 forces the general kernels.
- * omega_model_kernel_variant reports what runs: 2 specialised, 1 general.
- * No reference counterpart (performance control only; both give the same
- * bits). */
+ * pattern and nlayers is a multiple of 64, 1 forces the general kernels,
+ * 2 selects the LDS-tiled tendency kernels (any mesh; every row a tile's
+ * stencils read is staged once in LDS).  omega_model_kernel_variant reports
+ * what runs: 3 tiled, 2 specialised, 1 general.  No reference counterpart
+ * (performance control only; all give the same bits). */
 int omega_model_set_variant(omega_model_t model, int variant);
 int omega_model_kernel_variant(omega_model_t model, int *in_use);
+/* Tiled kernels: cells per tile (1..256, default 32) and levels per block
+ * (8 or 16, default 8); the tile's staged rows must fit the LDS a block
+ * may allocate (160 KiB on gfx950). */
+int omega_model_set_tiling(omega_model_t model, int cells_per_tile, int levels_per_block);
+/* Set sizes of the tiling of a mesh (host only): stats[0..6] = tiles, and
+ * the largest per-tile counts of union edges, union cells, cells of owned
+ * edges, union vertices, edges of owned cells, owned + neighbour cells. */
+int omega_mesh_tile_stats(omega_mesh_t mesh, int cells_per_tile, int *stats);
 
 /* ----------------------------------------------------------------------
  * Time steppers (src/time_integration/time_stepper.hpp:8-38)

@@ -549,15 +549,37 @@ int omega_model_set_launch(omega_model_t M, int rows, int lpt, int swizzle, int 
 int omega_model_set_variant(omega_model_t M, int variant) {
   return guarded([&] {
     OMEGA_REQUIRE(M, "null model");
-    OMEGA_REQUIRE(variant >= 0 && variant <= 1, "variant must be 0 or 1");
+    OMEGA_REQUIRE(variant >= 0 && variant <= 2, "variant must be 0, 1 or 2");
     M->lc.variant = variant;
+  });
+}
+
+int omega_mesh_tile_stats(omega_mesh_t mesh, int cells_per_tile, int *stats) {
+  return guarded([&] {
+    OMEGA_REQUIRE(mesh && stats, "null argument");
+    omega_hip::TileTables t;
+    build_tile_tables(mesh->dm, cells_per_tile, t, false);
+    const int v[7] = {t.ntiles, t.max_ue, t.max_uc, t.max_nkc, t.max_uv, t.max_ce, t.max_cc};
+    for (int i = 0; i < 7; ++i) stats[i] = v[i];
+  });
+}
+
+int omega_model_set_tiling(omega_model_t M, int cells_per_tile, int levels_per_block) {
+  return guarded([&] {
+    OMEGA_REQUIRE(M, "null model");
+    OMEGA_REQUIRE(cells_per_tile >= 1 && cells_per_tile <= 256,
+                  "cells_per_tile must be in 1..256");
+    OMEGA_REQUIRE(levels_per_block == 8 || levels_per_block == 16,
+                  "levels_per_block must be 8 or 16");
+    M->lc.tile = cells_per_tile;
+    M->lc.tile_k = levels_per_block;
   });
 }
 
 int omega_model_kernel_variant(omega_model_t M, int *in_use) {
   return guarded([&] {
     OMEGA_REQUIRE(M && in_use, "null argument");
-    *in_use = hex_usable(M->mesh->dm.view(), M->lc) ? 2 : 1;
+    *in_use = M->lc.variant == 2 ? 3 : hex_usable(M->mesh->dm.view(), M->lc) ? 2 : 1;
   });
 }
 

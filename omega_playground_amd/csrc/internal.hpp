@@ -164,7 +164,10 @@ struct Launch {
   int lpt = 1;       // levels per thread (1 or 2)
   int swizzle = 1;   // XCD-contiguous block order
   int chunk = 0;     // levels per block (0: all levels, up to 64 lanes)
-  int variant = 0;   // 0: pattern-specialised kernels when the mesh allows, 1: general
+  int variant = 0;   // 0: pattern-specialised kernels when the mesh allows, 1: general,
+                     // 2: LDS-tiled tendency kernels
+  int tile = 32;     // tiled kernels: cells per tile
+  int tile_k = 8;    // tiled kernels: levels per block (8 or 16)
 };
 
 // Enabled-term bits (same meaning as omega_model_set_terms).

@@ -19,11 +19,13 @@ using omega_hip::Real;
 // Opaque objects
 // ---------------------------------------------------------------------------
 #include "adjoint.hpp"
+#include "tiles.hpp"
 
 struct omega_mesh_s {
   DeviceMesh dm;
   DBuf<Real> staging;             // device scratch for omega_add_host_field
   omega_hip::AdjLayout adj;       // transposed tables, built on first adjoint use
+  omega_hip::TileTables tiles;    // LDS tile tables, built on first tiled launch
 };
 
 struct omega_state_s {
@@ -130,6 +132,10 @@ bool stream_capturing(hipStream_t s);
 
 void model_build_coefficients(omega_model_s *M);
 void ensure_vdel4(omega_model_s *M);
+// Cells per tile of the tiled kernels for this model's launch shape, and
+// the tables for it (built before any capture).
+int tile_cells(const omega_model_s *M);
+void ensure_tiles(omega_model_s *M);
 void run_diag(omega_model_s *M, const Real *h, const Real *vn, const Real *tr, hipStream_t s);
 // fields: bit 0 h, bit 1 vn, bit 2 tracers (all by default)
 void run_stage(omega_model_s *M, StageArgs a, hipStream_t s, int fields = 7);

@@ -62,6 +62,15 @@ void ensure_vdel4(omega_model_s *M) {
   }
 }
 
+int tile_cells(const omega_model_s *M) { return M->lc.tile; }
+
+void ensure_tiles(omega_model_s *M) {
+  if (M->lc.variant != 2) return;
+  omega_mesh_s *mesh = M->mesh;
+  const int T = tile_cells(M);
+  if (mesh->tiles.tcells != T) build_tile_tables(mesh->dm, T, mesh->tiles);
+}
+
 // Diagnostics of a state (vertex vorticity and thickness, cell kinetic
 // energy and divergence, tracer Laplacian, momentum del4 pre-pass) into the
 // model's buffers; only what the enabled terms read.
@@ -89,8 +98,14 @@ void run_stage(omega_model_s *M, StageArgs a, hipStream_t s, int fields) {
   a.ntr = (fields & 4) ? M->params.ntracers : 0;
   run_diag(M, a.h.in, a.vn.in, a.tr.in, s);
   const DiagOut d = M->diag();
-  if (hex_usable(mv, M->lc)) launch_tend_hex(mv, M->lc, mc, a, d, s);
-  else launch_tend(mv, M->lc, mc, a, d, s);
+  if (M->lc.variant == 2) {
+    ensure_tiles(M);
+    launch_tend_tile(mv, M->mesh->tiles.view(), M->lc, mc, a, d, s);
+  } else if (hex_usable(mv, M->lc)) {
+    launch_tend_hex(mv, M->lc, mc, a, d, s);
+  } else {
+    launch_tend(mv, M->lc, mc, a, d, s);
+  }
 }
 
 FieldIO fio(const Real *in, Real *tend, Real *acc, const Real *yold, Real *out) {
@@ -284,11 +299,13 @@ void do_step_maybe_graph(omega_stepper_s *St, omega_state_s *S, double t, double
   }
   omega_model_s *M = St->model;
   if (M->terms & T_VDEL4) ensure_vdel4(M);
+  ensure_tiles(M);
   GraphKey key{dt, S->cur, {}};
   for (int f = 0; f < 3; ++f)
     key.bufs.insert(key.bufs.end(), {S->buf[0][f].p, S->buf[1][f].p, St->acc[f].p,
                                      St->p1[f].p, St->p2[f].p, St->tend[f].p});
-  key.bufs.insert(key.bufs.end(), {M->vdel2_e.p, M->trdel2_c.p});
+  key.bufs.insert(key.bufs.end(), {M->vdel2_e.p, M->trdel2_c.p, M->mesh->tiles.ue.p,
+                                   M->mesh->tiles.cc.p});
   auto it = St->graphs.find(key);
   if (it == St->graphs.end()) {
     // bound the cache: states created and destroyed in a loop (the adjoint's

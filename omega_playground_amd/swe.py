@@ -103,6 +103,8 @@ def lib():
         "omega_model_get_launch": ([vp, ip, ip, ip, ip], C.c_int),
         "omega_model_set_variant": ([vp, C.c_int], C.c_int),
         "omega_model_kernel_variant": ([vp, ip], C.c_int),
+        "omega_model_set_tiling": ([vp, C.c_int, C.c_int], C.c_int),
+        "omega_mesh_tile_stats": ([vp, C.c_int, ip], C.c_int),
         "omega_stepper_create": ([vp, C.c_int, C.c_int, pp], C.c_int),
         "omega_stepper_destroy": ([vp], C.c_int),
         "omega_stepper_do_step": ([vp, C.c_double, C.c_double, vp, vp], C.c_int),
@@ -202,6 +204,14 @@ class MPASMesh:
             check(lib().omega_mesh_permutation(self._h, kind, out.ctypes.data_as(C.POINTER(C.c_int))))
             self._perm[kind] = out
         return self._perm[kind]
+
+    def tile_stats(self, cells_per_tile: int) -> dict:
+        """Per-tile set sizes of the LDS-tiled kernels (host only)."""
+        st = (C.c_int * 7)()
+        check(lib().omega_mesh_tile_stats(self._h, int(cells_per_tile), st))
+        keys = ("tiles", "union_edges", "union_cells", "owned_edge_cells", "union_vertices",
+                "cell_edges", "cell_cells")
+        return dict(zip(keys, list(st)))
 
     @property
     def regular_hex(self) -> bool:
@@ -426,11 +436,16 @@ class ShallowWaterModel:
         return a.value, b.value, c.value, d.value
 
     def set_variant(self, variant: int):
-        """0: pattern-specialised kernels when possible, 1: general kernels."""
+        """0: pattern-specialised kernels when possible, 1: general kernels,
+        2: LDS-tiled kernels."""
         check(lib().omega_model_set_variant(self._h, int(variant)))
 
+    def set_tiling(self, cells_per_tile: int = 32, levels_per_block: int = 8):
+        """Tile shape of the LDS-tiled kernels (variant 2)."""
+        check(lib().omega_model_set_tiling(self._h, int(cells_per_tile), int(levels_per_block)))
+
     def kernel_variant(self) -> int:
-        """2 if the pattern-specialised kernels run, 1 for the general ones."""
+        """3 tiled, 2 pattern-specialised, 1 general kernels."""
         v = C.c_int()
         check(lib().omega_model_kernel_variant(self._h, C.byref(v)))
         return v.value
