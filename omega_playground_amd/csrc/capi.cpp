@@ -549,7 +549,7 @@ int omega_model_set_launch(omega_model_t M, int rows, int lpt, int swizzle, int 
 int omega_model_set_variant(omega_model_t M, int variant) {
   return guarded([&] {
     OMEGA_REQUIRE(M, "null model");
-    OMEGA_REQUIRE(variant >= 0 && variant <= 2, "variant must be 0, 1 or 2");
+    OMEGA_REQUIRE(variant >= 0 && variant <= 3, "variant must be 0, 1, 2 or 3");
     M->lc.variant = variant;
   });
 }
@@ -579,7 +579,11 @@ int omega_model_set_tiling(omega_model_t M, int cells_per_tile, int levels_per_b
 int omega_model_kernel_variant(omega_model_t M, int *in_use) {
   return guarded([&] {
     OMEGA_REQUIRE(M && in_use, "null argument");
-    *in_use = M->lc.variant == 2 ? 3 : hex_usable(M->mesh->dm.view(), M->lc) ? 2 : 1;
+    const MeshView mv = M->mesh->dm.view();
+    *in_use = M->lc.variant == 2        ? 3
+              : hexu_usable(mv, M->lc) ? 4
+              : hex_usable(mv, M->lc)  ? 2
+                                       : 1;
   });
 }
 

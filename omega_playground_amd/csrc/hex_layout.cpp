@@ -74,6 +74,52 @@ bool build_hex_records(DeviceMesh &dm) {
     }
   }
   dm.hex_ok = true;
+  // per-cell union stencil: every cell must reproduce the hexu tables
+  dm.hexu_ok = false;
+  dm.hexu_h.assign((size_t)NC * hexu::kRec, 0);
+  std::vector<int> ue, uv2, uc2;
+  auto check = [](bool ok) {
+    if (!ok) throw 0;
+  };
+  try {
+    for (int c = 0; c < NC; ++c) {
+      ue = {dm.eofs_h[c], dm.eofs_h[c] + 1, dm.eofs_h[c] + 2};
+      uv2.clear();
+      uc2.clear();
+      for (int k = 0; k < 3; ++k) {
+        const int e = dm.eofs_h[c] + k;
+        for (int j = 0; j < hex::kSlots; ++j) {
+          const int ej = dm.eoe_h[(size_t)e * dm.me2 + j];
+          check(local_index(ue, ej) == hexu::TE[k][j]);
+          check(local_index(uv2, dm.voe_h[2 * (size_t)ej]) == hexu::TV0[k][j]);
+          check(local_index(uv2, dm.voe_h[2 * (size_t)ej + 1]) == hexu::TV1[k][j]);
+          check(local_index(uc2, dm.coe_h[2 * (size_t)ej]) == hexu::TC0[k][j]);
+          check(local_index(uc2, dm.coe_h[2 * (size_t)ej + 1]) == hexu::TC1[k][j]);
+        }
+        for (int s = 0; s < 2; ++s) {
+          check(local_index(uv2, dm.voe_h[2 * (size_t)e + s]) == hexu::OV[k][s]);
+          check(local_index(uc2, dm.coe_h[2 * (size_t)e + s]) == hexu::OC[k][s]);
+        }
+      }
+      for (int j = 0; j < 6; ++j) {
+        const int e = dm.eoc_h[(size_t)c * ME + j];
+        check(local_index(ue, e) == hexu::CE[j]);
+        check(local_index(uc2, dm.coe_h[2 * (size_t)e]) == hexu::CC0[j]);
+        check(local_index(uc2, dm.coe_h[2 * (size_t)e + 1]) == hexu::CC1[j]);
+      }
+      check(local_index(uc2, c) == hexu::SELF);
+      check((int)ue.size() == hexu::kNE && (int)uv2.size() == hexu::kNV &&
+            (int)uc2.size() == hexu::kNC);
+      int *rec = &dm.hexu_h[(size_t)c * hexu::kRec];
+      std::copy(ue.begin(), ue.end(), rec);
+      std::copy(uv2.begin(), uv2.end(), rec + hexu::kNE);
+      std::copy(uc2.begin(), uc2.end(), rec + hexu::kNE + hexu::kNV);
+    }
+  } catch (int) {
+    dm.hexu_h.clear();
+    return true;
+  }
+  dm.hexu_ok = true;
   return true;
 }
 

@@ -45,3 +45,52 @@ constexpr bool cell_is_second(int j) { return j < 3; }
 
 } // namespace hex
 } // namespace omega_hip
+
+namespace omega_hip {
+namespace hexu {
+
+// Union stencil of one cell on the regular hexagonal mesh: the cell's three
+// owned edges (classes 0..2) read 19 distinct edges, 16 distinct vertices
+// and 14 distinct cells.  Numbering them in first-appearance order (owned
+// edges first; then, per owned edge in class order, its edges-on-edge and
+// their vertices 0/1 and cells 0/1 slot by slot, then its own vertices and
+// cells; then the cell's six edges and their cells), the maps below are the
+// same for every cell; the host verifies that before these kernels run.
+constexpr int kNE = 19, kNV = 16, kNC = 14;
+constexpr int kRec = kNE + kNV + kNC; // ints per cell record
+
+// [class][slot] union index of edges_on_edge and of its vertices / cells
+constexpr int TE[3][10] = {{3, 4, 5, 6, 7, 1, 2, 8, 9, 10},
+                           {7, 11, 12, 13, 14, 2, 8, 9, 10, 0},
+                           {14, 15, 16, 17, 18, 8, 9, 10, 0, 1}};
+constexpr int TV0[3][10] = {{0, 1, 3, 4, 5, 6, 7, 7, 8, 9},
+                            {5, 10, 11, 12, 12, 7, 7, 8, 9, 5},
+                            {12, 13, 14, 14, 15, 7, 8, 9, 5, 6}};
+constexpr int TV1[3][10] = {{1, 2, 2, 3, 4, 5, 6, 8, 9, 0},
+                            {4, 4, 10, 11, 6, 6, 8, 9, 0, 0},
+                            {6, 12, 13, 15, 7, 8, 9, 0, 0, 5}};
+constexpr int TC0[3][10] = {{0, 0, 3, 4, 5, 5, 7, 6, 6, 6},
+                            {5, 4, 10, 11, 5, 7, 6, 6, 6, 0},
+                            {5, 11, 12, 7, 7, 6, 6, 6, 0, 5}};
+constexpr int TC1[3][10] = {{1, 2, 0, 0, 0, 6, 6, 8, 9, 1},
+                            {0, 5, 5, 5, 7, 6, 8, 9, 1, 6},
+                            {7, 7, 7, 13, 8, 8, 9, 1, 6, 6}};
+// [class][0/1] union index of the owned edge's own vertices and cells
+constexpr int OV[3][2] = {{5, 0}, {6, 5}, {7, 6}};
+constexpr int OC[3][2] = {{0, 6}, {5, 6}, {7, 6}};
+// the cell's six edges (edges_on_cell order) and their cells 0/1
+constexpr int CE[6] = {0, 1, 2, 8, 9, 10};
+constexpr int CC0[6] = {0, 5, 7, 6, 6, 6};
+constexpr int CC1[6] = {6, 6, 6, 8, 9, 1};
+constexpr int SELF = 6; // union index of the cell itself
+
+// the seven cells of the tracer stencil (the cell and its neighbours), as
+// positions into a compact array, and the union index of each position
+constexpr int kNT = 7;
+constexpr int TCELL[kNT] = {0, 1, 5, 6, 7, 8, 9};
+constexpr int tpos(int u) {
+  return u == 0 ? 0 : u == 1 ? 1 : u == 5 ? 2 : u == 6 ? 3 : u == 7 ? 4 : u == 8 ? 5 : 6;
+}
+
+} // namespace hexu
+} // namespace omega_hip

@@ -119,8 +119,9 @@ struct MeshView {
   const Real *crv;              // (dc*inv_area_triangle)*sign [nv][3]
   const Real *cth;              // kite*inv_area_triangle     [nv][3]
   // regular-hexagon records (valid when hex_ok, see hex_pattern.hpp)
-  int hex_ok;
+  int hex_ok, hexu_ok;
   const int *hexrec;            // [ne][30]
+  const int *hexu;              // [nc][49] per-cell union stencil (hex_pattern.hpp)
   const int *coc;               // cells_on_cell [nc][6]
 };
 
@@ -141,8 +142,9 @@ struct DeviceMesh {
   std::vector<Real> cdiv_h, cke_h, w_h, inv_dc_h, inv_dv_h, inv_dv4_h, s2_h, s4_h, crv_h, cth_h;
   // regular-hexagon records for the pattern-specialised kernels
   bool hex_ok = false;
-  std::vector<int> hexrec_h, coc_h;
-  DBuf<int> hexrec, coc;
+  bool hexu_ok = false; // per-cell union records valid (hexu)
+  std::vector<int> hexrec_h, coc_h, hexu_h;
+  DBuf<int> hexrec, coc, hexu;
 
   bool uploaded = false;
 
@@ -165,7 +167,7 @@ struct Launch {
   int swizzle = 1;   // XCD-contiguous block order
   int chunk = 0;     // levels per block (0: all levels, up to 64 lanes)
   int variant = 0;   // 0: pattern-specialised kernels when the mesh allows, 1: general,
-                     // 2: LDS-tiled tendency kernels
+                     // 2: LDS-tiled tendency kernels, 3: union-stencil specialised kernel
   int tile = 32;     // tiled kernels: cells per tile
   int tile_k = 8;    // tiled kernels: levels per block (8 or 16)
 };
@@ -235,6 +237,10 @@ void launch_diag_hex(const MeshView &m, const Launch &lc, int terms, int ntr, co
                      const Real *vn, const Real *tr, const DiagOut &d, hipStream_t s);
 void launch_tend_hex(const MeshView &m, const Launch &lc, const ModelCoef &mc, const StageArgs &a,
                      const DiagOut &d, hipStream_t s);
+// Union-stencil variant (one wavefront per cell, shared rows loaded once)
+bool hexu_usable(const MeshView &m, const Launch &lc);
+void launch_tend_hexu(const MeshView &m, const Launch &lc, const ModelCoef &mc,
+                      const StageArgs &a, const DiagOut &d, hipStream_t s);
 // y += alpha*x elementwise
 void launch_axpy(Real *y, Real alpha, const Real *x, size_t n, hipStream_t s);
 // y = x + alpha*z elementwise

@@ -427,6 +427,7 @@ void DeviceMesh::upload() {
   eofs.upload(eofs_h); vofs.upload(vofs_h);
   if (hex_ok) {
     hexrec.upload(hexrec_h);
+    if (hexu_ok) hexu.upload(hexu_h);
     coc.upload(coc_h);
   }
 
@@ -479,7 +480,9 @@ MeshView DeviceMesh::view() const {
   v.cdiv = cdiv.p; v.cke = cke.p; v.w = w.p; v.inv_dc = inv_dc.p; v.inv_dv = inv_dv.p;
   v.inv_dv4 = inv_dv4.p; v.s2 = s2.p; v.s4 = s4.p; v.crv = crv.p; v.cth = cth.p;
   v.hex_ok = hex_ok ? 1 : 0;
+  v.hexu_ok = hexu_ok ? 1 : 0;
   v.hexrec = hexrec.p;
+  v.hexu = hexu.p;
   v.coc = coc.p;
   return v;
 }

@@ -101,6 +101,8 @@ void run_stage(omega_model_s *M, StageArgs a, hipStream_t s, int fields) {
   if (M->lc.variant == 2) {
     ensure_tiles(M);
     launch_tend_tile(mv, M->mesh->tiles.view(), M->lc, mc, a, d, s);
+  } else if (hexu_usable(mv, M->lc)) {
+    launch_tend_hexu(mv, M->lc, mc, a, d, s);
   } else if (hex_usable(mv, M->lc)) {
     launch_tend_hex(mv, M->lc, mc, a, d, s);
   } else {
