@@ -209,6 +209,17 @@ def batched_bench(nmembers: int = 48, nsteps: int = 50):
         st.run(0.0, dt, nsteps, s)
     swe.synchronize()
     t_loop = time.perf_counter() - t0
+    # looped without graphs: each member's steps launched kernel by kernel,
+    # as a loop over the reference's do_step calls would run them
+    states_e = fresh()
+    for st in steppers:
+        st.set_graph(False)
+    swe.synchronize()
+    t0 = time.perf_counter()
+    for st, (mesh, model, sp, y0, dt), s in zip(steppers, members, states_e):
+        st.run(0.0, dt, nsteps, s)
+    swe.synchronize()
+    t_eager = time.perf_counter() - t0
     identical = all(
         all(np.array_equal(a, b) for a, b in zip(sb.download(), sl.download()))
         for sb, sl in zip(states_b, states_l))
@@ -223,9 +234,86 @@ def batched_bench(nmembers: int = 48, nsteps: int = 50):
         "batched_s": t_batch,
         "looped_s": t_loop,
         "speedup": t_loop / t_batch,
+        "looped_eager_s": t_eager,
+        "speedup_vs_eager_loop": t_eager / t_batch,
         "batched_cell_layer_steps_per_s": work / t_batch,
         "looped_cell_layer_steps_per_s": work / t_loop,
         "bit_identical": bool(identical),
+    }
+
+
+def ensemble_adjoint_bench(nsteps: int | None = None, n_snapshots: int = 10):
+    """The ensemble adjoint workload: 16 members on one nx=128 mesh, 64
+    layers, 2 tracers, RK4, shared del2 viscosity 1e3 and tracer del4 1e9;
+    per-member seeded initial states and observations (forward runs from
+    initial h perturbed by correlated Gaussian random fields); gradient of
+    the ensemble-mean misfit with respect to every member's initial state
+    and the shared coefficients, checkpoints capped at 8 GB.  One stacked
+    sweep for the ensemble against a loop of per-member adjoint runs with
+    the same snapshot count and cap."""
+    cfg = dict(wl.CONFIGS["c5"])
+    nsteps = nsteps or cfg["nsteps"]
+    M, ntr, L = cfg["nmembers"], cfg["ntracers"], cfg["nlayers"]
+    mesh = swe.PlanarHexagonalMesh(cfg["nx"], cfg["nx"], cfg["dc"], L)
+    params = swe.ShallowWaterParams(f0=cfg["f0"], grav=cfg["grav"], visc_del2=cfg["visc_del2"],
+                                    eddy_diff4=cfg["eddy_diff4"], ntracers=ntr)
+    dt = wl.dt_rule(cfg["dc"], cfg["grav"])
+    w = (1.0, 0.0, 1.0)
+    models = [swe.ShallowWaterModel(mesh, params) for _ in range(M)]
+    y0 = [wl.make_state(mesh, cfg, seed=m) for m in range(M)]
+    obs, states, obatch, batch = [], [], swe.Batch("rk4", 4), swe.Batch("rk4", 4)
+    for m in range(M):
+        o = swe.ShallowWaterState(mesh, ntr)
+        o.upload(*wl.adjoint_observations(mesh, cfg, seed=m))
+        obatch.add(models[m], o, dt)
+        obs.append(o)
+        s = swe.ShallowWaterState(mesh, ntr)
+        s.upload(*y0[m])
+        batch.add(models[m], s, dt)
+        states.append(s)
+    obatch.run(nsteps)
+    swe.synchronize()
+    grads = [swe.ShallowWaterState(mesh, ntr) for _ in range(M)]
+    t0 = time.perf_counter()
+    J, pgrad = batch.adjoint_run(0.0, nsteps, obs, grads, w, n_snapshots=n_snapshots,
+                                 byte_cap=cfg["byte_cap"])
+    swe.synchronize()
+    t_batch = time.perf_counter() - t0
+    ens_bytes = M * (mesh.ncells * L * (1 + ntr) + mesh.nedges * L) * 8
+    snaps = max(1, min(n_snapshots, cfg["byte_cap"] // ens_bytes))
+    # per-member loop, same snapshot count and cap
+    J_loop, pg_loop, max_diff = 0.0, np.zeros(4), 0.0
+    t0 = time.perf_counter()
+    for m in range(M):
+        st = swe.RK4Stepper(models[m])
+        st.set_graph(True)
+        g = swe.ShallowWaterState(mesh, ntr)
+        Jm, pgm = swe.adjoint_run(st, 0.0, dt, nsteps, states[m], obs[m], g,
+                                  tuple(x / M for x in w), n_snapshots=n_snapshots,
+                                  byte_cap=cfg["byte_cap"])
+        J_loop += Jm
+        pg_loop += pgm
+        if m == 0:
+            a, b = g.download(), grads[0].download()
+            max_diff = max(float(np.max(np.abs(x - y))) for x, y in zip(a, b))
+    swe.synchronize()
+    t_loop = time.perf_counter() - t0
+    return {
+        "workload": f"ensemble adjoint, {M} members, planar hex nx={cfg['nx']} ({mesh.ncells} "
+                    f"cells), {L} layers, {ntr} tracers, RK4, {nsteps} steps, del2 visc "
+                    f"{cfg['visc_del2']:g} and tracer del4 {cfg['eddy_diff4']:g} shared, "
+                    f"checkpoints <= {cfg['byte_cap'] / 1e9:g} GB",
+        "snapshots": int(snaps),
+        "forward_steps_recomputed": swe.revolve_cost(nsteps, int(snaps)),
+        "batched_s": t_batch,
+        "looped_s": t_loop,
+        "speedup": t_loop / t_batch,
+        "J": J,
+        "J_loop_rel_diff": abs(J - J_loop) / abs(J_loop),
+        "dJ_dvisc_del2": float(pgrad[0]),
+        "dJ_deddy_diff4": float(pgrad[3]),
+        "pgrad_loop_rel_diff": float(np.max(np.abs(pgrad - pg_loop) / np.maximum(np.abs(pg_loop), 1e-300))),
+        "member0_grad_max_abs_diff": max_diff,
     }
 
 
@@ -268,6 +356,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--skip-batched", action="store_true")
+    ap.add_argument("--skip-ensemble-adjoint", action="store_true")
+    ap.add_argument("--c5-steps", type=int, default=100,
+                    help="steps of the ensemble adjoint leg (the workload's full length is 500)")
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--launch", default="", help="rows,lpt,swizzle,levels_per_block")
     ap.add_argument("--variant", type=int, default=None,
@@ -353,6 +444,13 @@ def main():
         log(f"[bench] ensemble: batched {bat['batched_s']:.3f} s vs looped {bat['looped_s']:.3f} s "
             f"= {bat['speedup']:.2f}x, bit-identical {bat['bit_identical']}, launch {bat['launch']}")
 
+    eadj = None
+    if not args.skip_ensemble_adjoint:
+        log("[bench] ensemble adjoint workload")
+        eadj = ensemble_adjoint_bench(args.c5_steps)
+        log(f"[bench] ensemble adjoint: batched {eadj['batched_s']:.3f} s vs looped "
+            f"{eadj['looped_s']:.3f} s = {eadj['speedup']:.2f}x, {eadj['snapshots']} snapshots")
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
@@ -394,6 +492,7 @@ def main():
         "finite": finite,
         "adjoint": adj,
         "batched": bat,
+        "ensemble_adjoint": eadj,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

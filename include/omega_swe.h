@@ -236,14 +236,23 @@ int omega_model_get_launch(omega_model_t model, int *rows_per_block, int *levels
  * mesh is a regular hexagonal mesh verified against the built-in stencil
  * pattern and nlayers is a multiple of 64, 1 forces the general kernels,
  * 2 selects the LDS-tiled tendency kernels (any mesh; every row a tile's
- * stencils read is staged once in LDS).  omega_model_kernel_variant reports
- * what runs: 3 tiled, 2 specialised, 1 general.  No reference counterpart
- * (performance control only; all give the same bits). */
+ * stencils read is staged once in LDS), 3 the union-stencil specialised
+ * kernel (one wavefront per cell, rows shared by its three edges loaded
+ * once), 4 the hexagonal LDS-tiled edge kernel (velocity, thickness and
+ * vertex PV rows of a tile staged in LDS, stencil slots from the built-in
+ * pattern) followed by the specialised cell kernel; 3 and 4 fall back to
+ * the specialised kernels where they do not apply.
+ * omega_model_kernel_variant reports what runs: 5 hexagonal tiled,
+ * 4 union-stencil, 3 tiled, 2 specialised, 1 general.  No reference
+ * counterpart (performance control only; all give the same bits). */
 int omega_model_set_variant(omega_model_t model, int variant);
 int omega_model_kernel_variant(omega_model_t model, int *in_use);
 /* Tiled kernels: cells per tile (1..256, default 32) and levels per block
- * (8 or 16, default 8); the tile's staged rows must fit the LDS a block
- * may allocate (160 KiB on gfx950). */
+ * (8, 16 or 32, default 8; the general tiled kernels use 8 or 16 and take
+ * 32 as 16, the hexagonal tiled kernel uses 16 or 32 and takes 8 as 16);
+ * the tile's staged rows must fit the LDS a block may allocate (160 KiB on
+ * gfx950).  The hexagonal tiled kernel needs at most 256 union edges,
+ * cells and vertices per tile (one-byte stencil records). */
 int omega_model_set_tiling(omega_model_t model, int cells_per_tile, int levels_per_block);
 /* Set sizes of the tiling of a mesh (host only): stats[0..6] = tiles, and
  * the largest per-tile counts of union edges, union cells, cells of owned
@@ -381,6 +390,21 @@ int omega_batch_get_launch(omega_batch_t batch, int *threads, int *levels_per_th
 int omega_batch_autotune(omega_batch_t batch, int reps, double *best_ms, omega_stream_t stream);
 int omega_batch_info(omega_batch_t batch, int *nmembers, int64_t *blocks_per_launch,
                      int64_t *cell_layers);
+/* Ensemble adjoint with checkpointing under a memory cap.  The members
+ * (all on one mesh, with one layer count, tracer count, parameter set and
+ * dt; the batch's stepper kind) are stacked along the layer axis, which no
+ * operator couples, into one model of nmembers*nlayers layers, so each
+ * stage of the forward, recompute and reverse sweeps is one launch per
+ * kernel for the whole ensemble.  J = (1/M) sum_m J_m, J_m the misfit of
+ * omega_misfit of member m against obs[m] after nsteps steps; grads[m] =
+ * dJ/d(member m's initial state) (overwritten); param_grad[0..4) (may be NULL)
+ * += dJ/d(shared coefficients).  At most n_snapshots ensemble states and,
+ * when byte_cap > 0, byte_cap bytes of them are stored; the members' states
+ * are left unchanged.  Synchronises. */
+int omega_batch_adjoint_run(omega_batch_t batch, double t0, int nsteps, int n_snapshots,
+                            int64_t byte_cap, const omega_state_t *obs, const double *weights,
+                            double *J, omega_state_t *grads, double *param_grad,
+                            omega_stream_t stream);
 
 /* ----------------------------------------------------------------------
  * Utilities

@@ -418,3 +418,127 @@ int omega_batch_info(omega_batch_t B, int *nmembers, int64_t *blocks_per_launch,
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------
+// Ensemble adjoint.  No operator couples one layer to another, so members
+// that share a mesh, layer and tracer counts and coefficients are the
+// layers of one model: member m occupies layers [m*L, (m+1)*L) of a
+// stacked model with M*L layers.  The stacked model runs the checkpointed
+// adjoint once, so every stage of the forward, recompute and reverse sweeps
+// is one launch per kernel for the whole ensemble, the per-level arithmetic
+// is that of each member alone, and the gradients of the shared
+// coefficients are summed over all members by the same reductions.
+// ---------------------------------------------------------------------------
+namespace {
+
+void ok(int rc) {
+  if (rc != OMEGA_OK) throw Error(rc, g_last_error);
+}
+
+bool same_params(const omega_params &a, const omega_params &b) {
+  return a.f0 == b.f0 && a.grav == b.grav && a.drag_coeff == b.drag_coeff &&
+         a.visc_del2 == b.visc_del2 && a.visc_del4 == b.visc_del4 &&
+         a.eddy_diff2 == b.eddy_diff2 && a.eddy_diff4 == b.eddy_diff4 &&
+         a.ntracers == b.ntracers && a.disable_h_tendency == b.disable_h_tendency &&
+         a.disable_vn_tendency == b.disable_vn_tendency;
+}
+
+// dst rows of width M*L (member m at column m*L) <-> src rows of width L
+void stack_fields(omega_state_s *big, omega_state_s *mem, int m, int L, int M, bool to_big,
+                  hipStream_t s) {
+  for (int f = 0; f < 3; ++f) {
+    const size_t rows = mem->n[f] / (size_t)L;
+    if (!rows) continue;
+    const size_t wb = (size_t)L * sizeof(Real), pb = (size_t)M * L * sizeof(Real);
+    if (to_big)
+      OMEGA_HIP_CHECK(hipMemcpy2DAsync(big->ptr(f) + (size_t)m * L, pb, mem->ptr(f), wb, wb, rows,
+                                       hipMemcpyDeviceToDevice, s));
+    else
+      OMEGA_HIP_CHECK(hipMemcpy2DAsync(mem->ptr(f), wb, big->ptr(f) + (size_t)m * L, pb, wb, rows,
+                                       hipMemcpyDeviceToDevice, s));
+  }
+}
+
+struct Owned {
+  omega_mesh_s *mesh = nullptr;
+  omega_model_s *model = nullptr;
+  omega_stepper_s *st = nullptr;
+  omega_state_s *y = nullptr, *obs = nullptr, *grad = nullptr;
+  ~Owned() {
+    (void)omega_state_destroy(grad);
+    (void)omega_state_destroy(obs);
+    (void)omega_state_destroy(y);
+    (void)omega_stepper_destroy(st);
+    (void)omega_model_destroy(model);
+    delete mesh;
+  }
+};
+
+} // namespace
+
+extern "C" {
+
+int omega_batch_adjoint_run(omega_batch_t B, double t0, int nsteps, int n_snapshots,
+                            int64_t byte_cap, const omega_state_t *obs, const double *weights,
+                            double *J, omega_state_t *grads, double *param_grad,
+                            omega_stream_t stream) {
+  return guarded([&] {
+    OMEGA_REQUIRE(B && obs && weights && J && grads, "null argument");
+    const int M = (int)B->mem.size();
+    OMEGA_REQUIRE(M >= 1, "batch adjoint: the batch has no members");
+    const omega_model_s *M0 = B->mem[0]->M;
+    const omega_mesh_s *mesh0 = M0->mesh;
+    const HostMesh &h0 = mesh0->dm.host;
+    const int L = h0.nlayers, ntr = M0->params.ntracers;
+    const double dt = B->mem[0]->dt;
+    for (int m = 0; m < M; ++m) {
+      const auto &mb = *B->mem[m];
+      OMEGA_REQUIRE(mb.M->mesh == mesh0,
+                    "batch adjoint: all members must share one mesh (layers are stacked)");
+      OMEGA_REQUIRE(same_params(mb.M->params, M0->params) && mb.M->terms == M0->terms,
+                    "batch adjoint: members must share the model parameters and terms");
+      OMEGA_REQUIRE(mb.dt == dt, "batch adjoint: members must share dt");
+      OMEGA_REQUIRE(!mb.M->add_fn, "batch adjoint: members must not have an additional tendency");
+      OMEGA_REQUIRE(mb.S->ntr == ntr, "batch adjoint: member state tracer count differs");
+      OMEGA_REQUIRE(obs[m] && obs[m]->mesh == mesh0 && obs[m]->ntr == ntr,
+                    "batch adjoint: observation state does not match its member");
+      OMEGA_REQUIRE(grads[m] && grads[m]->mesh == mesh0 && grads[m]->ntr == ntr,
+                    "batch adjoint: gradient state does not match its member");
+    }
+    for (int e = 0; e < h0.nedges; ++e)
+      OMEGA_REQUIRE(h0.max_level_edge_top[e] >= L,
+                    "batch adjoint: stacking needs every layer active on every edge");
+    OMEGA_REQUIRE((int64_t)M * L <= (1 << 20), "batch adjoint: too many stacked layers");
+    hipStream_t s = as_stream(stream);
+
+    // stacked mesh, model and stepper
+    Owned o;
+    o.mesh = new omega_mesh_s();
+    o.mesh->dm.host = h0;
+    HostMesh &hb = o.mesh->dm.host;
+    hb.nlayers = M * L;
+    for (auto *v : {&hb.max_level_cell, &hb.max_level_edge_bot, &hb.max_level_edge_top,
+                    &hb.max_level_vertex_bot, &hb.max_level_vertex_top})
+      for (int &x : *v)
+        if (x >= L) x = M * L;
+    o.mesh->dm.build_device(mesh0->dm.ordering);
+    ok(omega_model_create(o.mesh, &M0->params, &o.model));
+    o.model->terms = M0->terms;
+    ok(omega_stepper_create(o.model, B->kind, B->nstages, &o.st));
+    ok(omega_state_create(o.mesh, ntr, &o.y));
+    ok(omega_state_create(o.mesh, ntr, &o.obs));
+    ok(omega_state_create(o.mesh, ntr, &o.grad));
+    for (int m = 0; m < M; ++m) {
+      stack_fields(o.y, B->mem[m]->S, m, L, M, true, s);
+      stack_fields(o.obs, obs[m], m, L, M, true, s);
+    }
+    // ensemble-mean misfit
+    const double w[3] = {weights[0] / M, weights[1] / M, weights[2] / M};
+    ok(omega_adjoint_run(o.st, t0, dt, nsteps, n_snapshots, byte_cap, o.y, o.obs, w, J, o.grad,
+                         param_grad, stream));
+    for (int m = 0; m < M; ++m) stack_fields(o.grad, grads[m], m, L, M, false, s);
+    OMEGA_HIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+} // extern "C"

@@ -549,7 +549,7 @@ int omega_model_set_launch(omega_model_t M, int rows, int lpt, int swizzle, int 
 int omega_model_set_variant(omega_model_t M, int variant) {
   return guarded([&] {
     OMEGA_REQUIRE(M, "null model");
-    OMEGA_REQUIRE(variant >= 0 && variant <= 3, "variant must be 0, 1, 2 or 3");
+    OMEGA_REQUIRE(variant >= 0 && variant <= 4, "variant must be 0, 1, 2, 3 or 4");
     M->lc.variant = variant;
   });
 }
@@ -569,8 +569,8 @@ int omega_model_set_tiling(omega_model_t M, int cells_per_tile, int levels_per_b
     OMEGA_REQUIRE(M, "null model");
     OMEGA_REQUIRE(cells_per_tile >= 1 && cells_per_tile <= 256,
                   "cells_per_tile must be in 1..256");
-    OMEGA_REQUIRE(levels_per_block == 8 || levels_per_block == 16,
-                  "levels_per_block must be 8 or 16");
+    OMEGA_REQUIRE(levels_per_block == 8 || levels_per_block == 16 || levels_per_block == 32,
+                  "levels_per_block must be 8, 16 or 32");
     M->lc.tile = cells_per_tile;
     M->lc.tile_k = levels_per_block;
   });
@@ -580,8 +580,10 @@ int omega_model_kernel_variant(omega_model_t M, int *in_use) {
   return guarded([&] {
     OMEGA_REQUIRE(M && in_use, "null argument");
     const MeshView mv = M->mesh->dm.view();
-    *in_use = M->lc.variant == 2        ? 3
-              : hexu_usable(mv, M->lc) ? 4
+    ensure_tiles(M);
+    *in_use = M->lc.variant == 2                          ? 3
+              : hextile_usable(mv, M->mesh->tiles, M->lc) ? 5
+              : hexu_usable(mv, M->lc)                    ? 4
               : hex_usable(mv, M->lc)  ? 2
                                        : 1;
   });

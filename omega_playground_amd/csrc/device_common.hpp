@@ -114,6 +114,92 @@ __device__ __forceinline__ void epilogue(const StageArgs &a, const FieldIO &f, s
   }
 }
 
+// The stage epilogue split in two so that its loads can be issued together
+// with a kernel's stencil loads: epi_prefetch reads the operands (and the
+// starting value of the tendency), epi_finish applies the same update as
+// epilogue().
+template <int N> struct EpiIn {
+  Vd<N> base; // starting value of the tendency (tend_base)
+  Vd<N> x, y; // FIRST/LSRK: x = y^s; MID: x = acc, y = yold; LAST: x = acc; PROVIS: y = yold
+};
+
+template <int N>
+__device__ __forceinline__ EpiIn<N> epi_prefetch(const StageArgs &a, const FieldIO &f, size_t idx) {
+  EpiIn<N> p;
+  p.base = zero<N>();
+  p.x = zero<N>();
+  p.y = zero<N>();
+  if (a.base_from_tend) p.base = ld<N>(f.tend + idx);
+  switch (a.epi) {
+  case EPI_RK4_FIRST:
+  case EPI_LSRK:
+    p.x = ld<N>(f.in + idx);
+    break;
+  case EPI_RK4_MID:
+    p.x = ld<N>(f.acc + idx);
+    p.y = ld<N>(f.yold + idx);
+    break;
+  case EPI_RK4_LAST:
+    p.x = ld<N>(f.acc + idx);
+    break;
+  case EPI_PROVIS:
+    p.y = ld<N>(f.yold + idx);
+    break;
+  default:
+    break;
+  }
+  if (a.base_from_tend) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) p.base[i] = p.base[i] * a.a;
+  }
+  return p;
+}
+
+template <int N>
+__device__ __forceinline__ void epi_finish(const StageArgs &a, const FieldIO &f, size_t idx,
+                                           const Vd<N> &T, const EpiIn<N> &p) {
+  Vd<N> u, w;
+  switch (a.epi) {
+  case EPI_STORE:
+    st<N>(f.tend + idx, T);
+    break;
+  case EPI_RK4_FIRST:
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      u[i] = p.x[i] + a.db * T[i];
+      w[i] = p.x[i] + a.da * T[i];
+    }
+    st<N>(f.acc + idx, u);
+    st<N>(f.out + idx, w);
+    break;
+  case EPI_RK4_MID:
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      u[i] = p.x[i] + a.db * T[i];
+      w[i] = p.y[i] + a.da * T[i];
+    }
+    st<N>(f.acc + idx, u);
+    st<N>(f.out + idx, w);
+    break;
+  case EPI_RK4_LAST:
+#pragma unroll
+    for (int i = 0; i < N; ++i) u[i] = p.x[i] + a.db * T[i];
+    st<N>(f.out + idx, u);
+    break;
+  case EPI_LSRK:
+    st<N>(f.tend + idx, T);
+#pragma unroll
+    for (int i = 0; i < N; ++i) u[i] = p.x[i] + a.db * T[i];
+    st<N>(f.out + idx, u);
+    break;
+  case EPI_PROVIS:
+#pragma unroll
+    for (int i = 0; i < N; ++i) u[i] = p.y[i] + a.da * T[i];
+    st<N>(f.out + idx, u);
+    break;
+  }
+}
+
 // Starting value of a tendency: 0 (replace) or a*tend (increment mode after
 // the low-storage scaling tend *= a, lsrk.cpp:47-63).
 template <int N>

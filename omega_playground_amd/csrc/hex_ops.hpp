@@ -351,5 +351,247 @@ __device__ __forceinline__ void cell_tend_hex(const MeshView &m, const ModelCoef
   }
 }
 
+// ---------------------------------------------------------------------------
+// Two cells per wavefront with wavefront-uniform tables (N = 2 layout)
+// ---------------------------------------------------------------------------
+// In the N = 2 layout lanes 0-31 of a wavefront carry cell cA and lanes 32-63
+// cell cB = cA + 1 (both at the same 64 levels).  Indices and coefficients
+// are then uniform per half-wavefront: the variants below read the tables of
+// both cells at wavefront-uniform addresses (scalar loads) and pick per lane,
+// instead of issuing one vector load per table entry.  Same expressions as
+// edge_tend_hex / cell_tend_hex, so the same bits.
+
+// Read-only table entry through the constant address space: the tables are
+// never written while a kernel runs, and loads from that space at uniform
+// addresses go through the scalar cache even after the kernel has stored
+// to other global memory.
+template <class T>
+__device__ __forceinline__ T cld(const T *p, long i) {
+  return ((const __attribute__((address_space(4))) T *)p)[i];
+}
+
+// both entries read at uniform offsets, then the lane's one picked
+template <class T>
+__device__ __forceinline__ T pick2(const T *p, long iA, long iB, bool hi) {
+  const T a = cld(p, iA);
+  const T b = cld(p, iB);
+  return hi ? b : a;
+}
+
+template <int K>
+__device__ __forceinline__ void edge_tend_hex2(const MeshView &m, const ModelCoef &mc,
+                                               const StageArgs &a, const DiagOut &d, int eA,
+                                               int eB, bool hi, int k) {
+  constexpr int N = 2;
+  const int L = m.L;
+  const int e = hi ? eB : eA;
+  const size_t idx = (size_t)e * L + k;
+  if (!a.do_vn) {
+    if (a.epi == EPI_STORE) return;
+    epilogue<N>(a, a.vn, idx, tend_base<N>(a, a.vn, idx));
+    return;
+  }
+  const double *__restrict__ h = a.h.in;
+  const double *__restrict__ vn = a.vn.in;
+  const int *rA = m.hexrec + (size_t)eA * hex::kRecInts;
+  const int *rB = m.hexrec + (size_t)eB * hex::kRecInts;
+  int ej[10], uv[10], uc[10], uvA[10], uvB[10];
+#pragma unroll
+  for (int j = 0; j < 10; ++j) {
+    const int ea = cld(rA, j), eb = cld(rB, j), ca = cld(rA, 20 + j), cb = cld(rB, 20 + j);
+    uvA[j] = cld(rA, 10 + j);
+    uvB[j] = cld(rB, 10 + j);
+    ej[j] = hi ? eb : ea;
+    uv[j] = hi ? uvB[j] : uvA[j];
+    uc[j] = hi ? cb : ca;
+  }
+  constexpr int o0 = hex::OWNV[K][0], o1 = hex::OWNV[K][1];
+  constexpr int q0 = hex::OWNC[K][0], q1 = hex::OWNC[K][1];
+  const bool pv = a.terms & T_PV;
+  Vd<N> hU[10], rvU[10], itU[10], vJ[10];
+#pragma unroll
+  for (int u = 0; u < 10; ++u) hU[u] = rowv<N>(h, uc[u], L, k);
+  if (pv) {
+#pragma unroll
+    for (int j = 0; j < 10; ++j) vJ[j] = rowv<N>(vn, ej[j], L, k);
+#pragma unroll
+    for (int u = 0; u < 10; ++u) {
+      rvU[u] = rowv<N>(d.rvort_v, uv[u], L, k);
+      itU[u] = rowv<N>(d.inv_thick_v, uv[u], L, k);
+    }
+  } else if (a.terms & T_VDEL2) {
+    rvU[o0] = rowv<N>(d.rvort_v, uv[o0], L, k);
+    rvU[o1] = rowv<N>(d.rvort_v, uv[o1], L, k);
+  }
+  const int c0 = uc[q0], c1 = uc[q1];
+  Vd<N> T = tend_base<N>(a, a.vn, idx);
+  const double idc = pick2(m.inv_dc, eA, eB, hi);
+  if (pv) {
+    Vd<N> nrvU[10], npvU[10];
+#pragma unroll
+    for (int u = 0; u < 10; ++u) {
+      const double fa = cld(mc.f_v, uvA[u]);
+      const double fb = cld(mc.f_v, uvB[u]);
+      const double f = hi ? fb : fa;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        nrvU[u][i] = itU[u][i] * rvU[u][i];
+        npvU[u][i] = itU[u][i] * f;
+      }
+    }
+    double w[10];
+#pragma unroll
+    for (int j = 0; j < 10; ++j) w[j] = pick2(m.w, (long)eA * 10 + j, (long)eB * 10 + j, hi);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const double qs = 0.5 * (nrvU[o0][i] + nrvU[o1][i]) + 0.5 * (npvU[o0][i] + npvU[o1][i]);
+      double qt = 0.0;
+#pragma unroll
+      for (int j = 0; j < 10; ++j) {
+        const double fh = 0.5 * (hU[hex::LC0[K][j]][i] + hU[hex::LC1[K][j]][i]);
+        const double nrv = 0.5 * (nrvU[hex::LV0[K][j]][i] + nrvU[hex::LV1[K][j]][i]);
+        const double npv = 0.5 * (npvU[hex::LV0[K][j]][i] + npvU[hex::LV1[K][j]][i]);
+        const double nv = (qs + nrv + npv) * 0.5;
+        qt = qt + w[j] * fh * vJ[j][i] * nv;
+      }
+      T[i] = T[i] + qt;
+    }
+  }
+  if (a.terms & T_KE) {
+    const Vd<N> k0 = rowv<N>(d.ke_c, c0, L, k), k1 = rowv<N>(d.ke_c, c1, L, k);
+#pragma unroll
+    for (int i = 0; i < N; ++i) T[i] = T[i] - (k1[i] - k0[i]) * idc;
+  }
+  if (a.terms & T_SSH) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) T[i] = T[i] - mc.grav * (hU[q1][i] - hU[q0][i]) * idc;
+  }
+  if (a.terms & (T_VDEL2 | T_VDEL4)) {
+    const int ml = pick2(m.maxlev_e, eA, eB, hi);
+    const double idv = pick2(m.inv_dv, eA, eB, hi);
+    if (a.terms & T_VDEL2) {
+      const Vd<N> d0 = rowv<N>(d.div_c, c0, L, k), d1 = rowv<N>(d.div_c, c1, L, k);
+      const double cv = pick2(mc.cv2, eA, eB, hi);
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const double del2u = (d1[i] - d0[i]) * idc - (rvU[o1][i] - rvU[o0][i]) * idv;
+        T[i] = T[i] + ((k + i < ml) ? cv : 0.0) * del2u;
+      }
+    }
+    if (a.terms & T_VDEL4) {
+      const int b0 = uv[o0], b1 = uv[o1];
+      const Vd<N> d0 = rowv<N>(d.vdel2_div_c, c0, L, k), d1 = rowv<N>(d.vdel2_div_c, c1, L, k);
+      const Vd<N> r0 = rowv<N>(d.vdel2_rv_v, b0, L, k), r1 = rowv<N>(d.vdel2_rv_v, b1, L, k);
+      const double cv = pick2(mc.cv4, eA, eB, hi);
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const double del2u = (d1[i] - d0[i]) * idc - (r1[i] - r0[i]) * idv;
+        T[i] = T[i] - ((k + i < ml) ? cv : 0.0) * del2u;
+      }
+    }
+  }
+  epilogue<N>(a, a.vn, idx, T);
+}
+
+__device__ __forceinline__ void cell_tend_hex2(const MeshView &m, const ModelCoef &mc,
+                                               const StageArgs &a, const DiagOut &d, int cA,
+                                               int cB, bool hi, int k) {
+  constexpr int N = 2;
+  const int L = m.L, me = m.me, NC = m.nc;
+  const int c = hi ? cB : cA;
+  const double *__restrict__ h = a.h.in;
+  const double *__restrict__ vn = a.vn.in;
+  int E[6], NB[6], EA[6], EB[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    EA[j] = cld(m.eoc, (long)cA * me + j);
+    EB[j] = cld(m.eoc, (long)cB * me + j);
+    E[j] = hi ? EB[j] : EA[j];
+    NB[j] = pick2(m.coc, (long)cA * 6 + j, (long)cB * 6 + j, hi);
+  }
+  const Vd<N> hC = rowv<N>(h, c, L, k);
+  Vd<N> hN[6], vE[6], fh[6];
+  double cd[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    hN[j] = rowv<N>(h, NB[j], L, k);
+    vE[j] = rowv<N>(vn, E[j], L, k);
+  }
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    cd[j] = pick2(m.cdiv, (long)cA * me + j, (long)cB * me + j, hi);
+#pragma unroll
+    for (int i = 0; i < N; ++i) fh[j][i] = 0.5 * (hN[j][i] + hC[i]);
+  }
+  const size_t hidx = (size_t)c * L + k;
+  if (a.do_h) {
+    Vd<N> T = tend_base<N>(a, a.h, hidx);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) acc = acc - cd[j] * fh[j][i] * vE[j][i];
+      T[i] = T[i] + acc;
+    }
+    epilogue<N>(a, a.h, hidx, T);
+  } else if (a.epi != EPI_STORE) {
+    epilogue<N>(a, a.h, hidx, tend_base<N>(a, a.h, hidx));
+  }
+  const int ntr = a.ntr;
+  if (ntr <= 0) return;
+  const bool td2 = a.terms & T_TDEL2, td4 = a.terms & T_TDEL4;
+  Vd<N> invC, invN[6];
+  double idc[6], c2[6], s2[6], c4[6];
+#pragma unroll
+  for (int i = 0; i < N; ++i) invC[i] = 1.0 / hC[i];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) invN[j][i] = 1.0 / hN[j][i];
+    idc[j] = pick2(m.inv_dc, EA[j], EB[j], hi);
+    c2[j] = td2 ? pick2(mc.ct2, (long)cA * me + j, (long)cB * me + j, hi) : 0.0;
+    s2[j] = td2 ? pick2(m.s2, EA[j], EB[j], hi) : 0.0;
+    c4[j] = td4 ? pick2(mc.ct4, (long)cA * me + j, (long)cB * me + j, hi) : 0.0;
+  }
+  for (int l = 0; l < ntr; ++l) {
+    const double *trl = a.tr.in + (size_t)l * NC * L;
+    const Vd<N> tC = rowv<N>(trl, c, L, k);
+    Vd<N> tN[6], dN[6], dC = zero<N>();
+#pragma unroll
+    for (int j = 0; j < 6; ++j) tN[j] = rowv<N>(trl, NB[j], L, k);
+    if (td4) {
+      const double *dl = d.trdel2_c + (size_t)l * NC * L;
+      dC = rowv<N>(dl, c, L, k);
+#pragma unroll
+      for (int j = 0; j < 6; ++j) dN[j] = rowv<N>(dl, NB[j], L, k);
+    }
+    const size_t tidx = ((size_t)l * NC + c) * L + k;
+    Vd<N> T = tend_base<N>(a, a.tr, tidx);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const double nC = tC[i] * invC[i];
+      double adv = 0.0, dif = 0.0, hyp = 0.0;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        const double nN = tN[j][i] * invN[j][i];
+        const bool second = hex::cell_is_second(j);
+        adv = adv - cd[j] * fh[j][i] * ((nN + nC) * 0.5) * vE[j][i];
+        if (td2) {
+          const double grad = (second ? (nC - nN) : (nN - nC)) * idc[j];
+          dif = dif + c2[j] * fh[j][i] * s2[j] * grad;
+        }
+        if (td4) {
+          const double grad = (second ? (dC[i] - dN[j][i]) : (dN[j][i] - dC[i])) * idc[j];
+          hyp = hyp - c4[j] * grad;
+        }
+      }
+      T[i] = T[i] + adv;
+      if (td2) T[i] = T[i] + dif;
+      if (td4) T[i] = T[i] + hyp;
+    }
+    epilogue<N>(a, a.tr, tidx, T);
+  }
+}
+
 } // namespace hexops
 } // namespace omega_hip

@@ -16,6 +16,7 @@
 // address path of the CU is what bounds these kernels, so the aim throughout
 // is the fewest load instructions per level.
 #include "hex_ops.hpp"
+#include "tiles.hpp"
 
 namespace omega_hip {
 
@@ -52,6 +53,38 @@ __global__ void __launch_bounds__(256)
   edge_tend_hex<N, 0>(m, mc, a, d, e0, k);
   edge_tend_hex<N, 1>(m, mc, a, d, e0 + 1, k);
   edge_tend_hex<N, 2>(m, mc, a, d, e0 + 2, k);
+  cell_tend_hex<N>(m, mc, a, d, c, k);
+}
+
+// N = 2 with two cells per wavefront and their tables read through the
+// scalar path (hex_ops.hpp: edge_tend_hex2 / cell_tend_hex2); the block's
+// row count is even, so the cells of one wavefront are cA and cA + 1
+__global__ void __launch_bounds__(256)
+    k_tend_hex2(MeshView m, ModelCoef mc, StageArgs a, DiagOut d, int swz) {
+  const int s = logical_block(blockIdx.x, gridDim.x, swz);
+  const int R = blockDim.y;
+  const int cA = uniform(s * R + ((int)threadIdx.y & ~1));
+  if (cA >= m.nc) return;
+  const int cB = min(cA + 1, m.nc - 1);
+  const bool hi = (threadIdx.y & 1) != 0;
+  if (s * R + (int)threadIdx.y >= m.nc) return;
+  const int k = (blockIdx.y * blockDim.x + threadIdx.x) * 2;
+  edge_tend_hex2<0>(m, mc, a, d, 3 * cA, 3 * cB, hi, k);
+  edge_tend_hex2<1>(m, mc, a, d, 3 * cA + 1, 3 * cB + 1, hi, k);
+  edge_tend_hex2<2>(m, mc, a, d, 3 * cA + 2, 3 * cB + 2, hi, k);
+  cell_tend_hex2(m, mc, a, d, cA, cB, hi, k);
+}
+
+// the cell part alone (thickness and tracer tendencies), for launchers that
+// evaluate the edges elsewhere
+template <int N>
+__global__ void __launch_bounds__(256)
+    k_tend_hex_cells(MeshView m, ModelCoef mc, StageArgs a, DiagOut d, int swz) {
+  const int s = logical_block(blockIdx.x, gridDim.x, swz);
+  const int R = blockDim.y;
+  const int c = ent<N>(s * R + (int)threadIdx.y);
+  if (c >= m.nc) return;
+  const int k = (blockIdx.y * blockDim.x + threadIdx.x) * N;
   cell_tend_hex<N>(m, mc, a, d, c, k);
 }
 
@@ -269,6 +302,8 @@ dim3 hex_grid(const MeshView &m, const Launch &lc, int N, dim3 &block) {
   const int lanes = 64 / N;
   int rows = lc.rows > 0 ? lc.rows : 256 / lanes;
   if (rows * lanes > 256) rows = 256 / lanes;
+  // N = 2: an even row count keeps the two cells of a wavefront adjacent
+  if (N == 2 && rows % 2) rows = rows > 1 ? rows - 1 : 2;
   block = dim3(lanes, rows);
   return dim3((m.nc + rows - 1) / rows, m.L / 64);
 }
@@ -306,6 +341,18 @@ void launch_diag_hex(const MeshView &m, const Launch &lc, int terms, int ntr, co
   OMEGA_HIP_CHECK(hipGetLastError());
 }
 
+void launch_tend_hex_cells(const MeshView &m, const Launch &lc, const ModelCoef &mc,
+                           const StageArgs &a, const DiagOut &d, hipStream_t s) {
+  dim3 block;
+  const int N = hex_n(m, lc);
+  const dim3 grid = hex_grid(m, lc, N, block);
+  if (N == 2)
+    hipLaunchKernelGGL(k_tend_hex_cells<2>, grid, block, 0, s, m, mc, a, d, lc.swizzle);
+  else
+    hipLaunchKernelGGL(k_tend_hex_cells<1>, grid, block, 0, s, m, mc, a, d, lc.swizzle);
+  OMEGA_HIP_CHECK(hipGetLastError());
+}
+
 void launch_tend_hex(const MeshView &m, const Launch &lc, const ModelCoef &mc, const StageArgs &a,
                      const DiagOut &d, hipStream_t s) {
   TimerScope ts("tend", s);
@@ -313,7 +360,7 @@ void launch_tend_hex(const MeshView &m, const Launch &lc, const ModelCoef &mc, c
   const int N = hex_n(m, lc);
   const dim3 grid = hex_grid(m, lc, N, block);
   if (N == 2)
-    hipLaunchKernelGGL(k_tend_hex<2>, grid, block, 0, s, m, mc, a, d, lc.swizzle);
+    hipLaunchKernelGGL(k_tend_hex2, grid, block, 0, s, m, mc, a, d, lc.swizzle);
   else
     hipLaunchKernelGGL(k_tend_hex<1>, grid, block, 0, s, m, mc, a, d, lc.swizzle);
   OMEGA_HIP_CHECK(hipGetLastError());

@@ -65,7 +65,7 @@ void ensure_vdel4(omega_model_s *M) {
 int tile_cells(const omega_model_s *M) { return M->lc.tile; }
 
 void ensure_tiles(omega_model_s *M) {
-  if (M->lc.variant != 2) return;
+  if (M->lc.variant != 2 && M->lc.variant != 4) return;
   omega_mesh_s *mesh = M->mesh;
   const int T = tile_cells(M);
   if (mesh->tiles.tcells != T) build_tile_tables(mesh->dm, T, mesh->tiles);
@@ -98,9 +98,11 @@ void run_stage(omega_model_s *M, StageArgs a, hipStream_t s, int fields) {
   a.ntr = (fields & 4) ? M->params.ntracers : 0;
   run_diag(M, a.h.in, a.vn.in, a.tr.in, s);
   const DiagOut d = M->diag();
+  if (M->lc.variant == 2 || M->lc.variant == 4) ensure_tiles(M);
   if (M->lc.variant == 2) {
-    ensure_tiles(M);
     launch_tend_tile(mv, M->mesh->tiles.view(), M->lc, mc, a, d, s);
+  } else if (hextile_usable(mv, M->mesh->tiles, M->lc)) {
+    launch_tend_hextile(mv, M->mesh->tiles.view(), M->lc, mc, a, d, s);
   } else if (hexu_usable(mv, M->lc)) {
     launch_tend_hexu(mv, M->lc, mc, a, d, s);
   } else if (hex_usable(mv, M->lc)) {
@@ -307,7 +309,7 @@ void do_step_maybe_graph(omega_stepper_s *St, omega_state_s *S, double t, double
     key.bufs.insert(key.bufs.end(), {S->buf[0][f].p, S->buf[1][f].p, St->acc[f].p,
                                      St->p1[f].p, St->p2[f].p, St->tend[f].p});
   key.bufs.insert(key.bufs.end(), {M->vdel2_e.p, M->trdel2_c.p, M->mesh->tiles.ue.p,
-                                   M->mesh->tiles.cc.p});
+                                   M->mesh->tiles.cc.p, M->mesh->tiles.hexl.p});
   auto it = St->graphs.find(key);
   if (it == St->graphs.end()) {
     // bound the cache: states created and destroyed in a loop (the adjoint's

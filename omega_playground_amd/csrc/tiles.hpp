@@ -34,6 +34,11 @@ struct TileView {
   const int4 *eoc_l;      // [nc][me] local (edge, cell0, cell1, 0) of each cell edge
   // largest set sizes (LDS allocation); max_eo: owned edges
   int max_ue, max_uc, max_nkc, max_uv, max_ce, max_cc, max_eo;
+  // regular hexagonal mesh: per owned edge, its ten edges-on-edge, ten
+  // distinct vertices and ten distinct cells (hex_pattern.hpp order) as
+  // one-byte indices into the tile's union edges / vertices / cells
+  // (32 bytes per edge, two uint4); null when unavailable
+  const uint4 *hexl;
 };
 
 struct TileTables {
@@ -41,6 +46,8 @@ struct TileTables {
   int ntiles = 0;
   DBuf<int> ue_ofs, ue, uc_ofs, uc, nkc, uv_ofs, uv, eoe_l, ce_ofs, ce, cc_ofs, cc;
   DBuf<int4> ue_cv, eoc_l;
+  DBuf<uint4> hexl;
+  bool hexl_ok = false;
   int max_ue = 0, max_uc = 0, max_nkc = 0, max_uv = 0, max_ce = 0, max_cc = 0, max_eo = 0;
   TileView view() const;
 };
@@ -53,5 +60,17 @@ void build_tile_tables(const DeviceMesh &dm, int tcells, TileTables &t, bool upl
 // edges, one for the owned cells.  levels: levels per block (8 or 16).
 void launch_tend_tile(const MeshView &m, const TileView &t, const Launch &lc,
                       const ModelCoef &mc, const StageArgs &a, const DiagOut &d, hipStream_t s);
+
+// Hexagonal-mesh variant: the vn tendency of the owned edges from LDS rows
+// (edges-on-edge velocities, cell thicknesses, vertex PV products) with the
+// compile-time stencil pattern, then the cell tendencies as in the
+// pattern-specialised kernels.  Usable when the tables carry hexl records.
+bool hextile_usable(const MeshView &m, const TileTables &t, const Launch &lc);
+void launch_tend_hextile(const MeshView &m, const TileView &t, const Launch &lc,
+                         const ModelCoef &mc, const StageArgs &a, const DiagOut &d,
+                         hipStream_t s);
+// Cell part of the pattern-specialised tendency kernel alone (kernels_hex.hip).
+void launch_tend_hex_cells(const MeshView &m, const Launch &lc, const ModelCoef &mc,
+                           const StageArgs &a, const DiagOut &d, hipStream_t s);
 
 } // namespace omega_hip

@@ -1,7 +1,10 @@
 // Host construction of the LDS tile tables (see tiles.hpp).
 #include <algorithm>
+#include <cstdint>
+#include <cstring>
 #include <unordered_map>
 
+#include "hex_pattern.hpp"
 #include "tiles.hpp"
 
 namespace omega_hip {
@@ -55,6 +58,7 @@ TileView TileTables::view() const {
   v.max_ce = max_ce;
   v.max_cc = max_cc;
   v.max_eo = max_eo;
+  v.hexl = hexl_ok ? hexl.p : nullptr;
   return v;
 }
 
@@ -66,6 +70,9 @@ void build_tile_tables(const DeviceMesh &dm, int tcells, TileTables &t, bool upl
   std::vector<int> ue_ofs{0}, ue, uc_ofs{0}, uc, nkc, uv_ofs{0}, uv, ce_ofs{0}, ce, cc_ofs{0}, cc;
   std::vector<int4> ue_cv, eoc_l((size_t)NC * ME, int4{0, 0, 0, 0});
   std::vector<int> eoe_l((size_t)NE * ME2, 0);
+  // one-byte local stencil records of the hexagonal pattern (8 words/edge)
+  bool hexl_ok = dm.hex_ok;
+  std::vector<uint32_t> hexl(hexl_ok ? (size_t)NE * 8 : 0, 0u);
   t.max_ue = t.max_uc = t.max_nkc = t.max_uv = t.max_ce = t.max_cc = t.max_eo = 0;
   for (int ti = 0; ti < ntiles; ++ti) {
     const int c0 = ti * tcells, c1 = std::min(NC, c0 + tcells);
@@ -108,6 +115,22 @@ void build_tile_tables(const DeviceMesh &dm, int tcells, TileTables &t, bool upl
                     "tile tables: cells of owned edges must lead the union");
       OMEGA_REQUIRE(q.z < (int)VV.size() && q.w < (int)VV.size(),
                     "tile tables: vertices of owned edges must lead the union");
+    }
+    if (hexl_ok) {
+      if (U.size() > 256 || UC.size() > 256 || UV.size() > 256) hexl_ok = false;
+      for (int e = e0; e < e1 && hexl_ok; ++e) {
+        const int *rec = &dm.hexrec_h[(size_t)e * hex::kRecInts];
+        uint32_t *dst = &hexl[(size_t)e * 8];
+        for (int b = 0; b < hex::kRecInts; ++b) {
+          const std::unordered_map<int, int> &mp = b < 10 ? lu : b < 20 ? lv : lc;
+          const auto it = mp.find(rec[b]);
+          if (it == mp.end()) {
+            hexl_ok = false;
+            break;
+          }
+          dst[b >> 2] |= (uint32_t)it->second << (8 * (b & 3));
+        }
+      }
     }
     ue.insert(ue.end(), U.begin(), U.end());
     ue_ofs.push_back((int)ue.size());
@@ -152,6 +175,7 @@ void build_tile_tables(const DeviceMesh &dm, int tcells, TileTables &t, bool upl
     t.max_cc = std::max(t.max_cc, (int)CC.size());
   }
   t.ntiles = ntiles;
+  t.hexl_ok = hexl_ok && upload;
   if (!upload) return;
   t.tcells = tcells;
   t.ue_ofs.upload(ue_ofs);
@@ -168,6 +192,13 @@ void build_tile_tables(const DeviceMesh &dm, int tcells, TileTables &t, bool upl
   t.cc_ofs.upload(cc_ofs);
   t.cc.upload(cc);
   t.eoc_l.upload(eoc_l);
+  if (t.hexl_ok) {
+    std::vector<uint4> h4(hexl.size() / 4);
+    std::memcpy(h4.data(), hexl.data(), hexl.size() * sizeof(uint32_t));
+    t.hexl.upload(h4);
+  } else {
+    t.hexl.release();
+  }
 }
 
 } // namespace omega_hip

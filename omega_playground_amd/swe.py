@@ -139,6 +139,8 @@ def lib():
         "omega_batch_get_launch": ([vp, ip, ip], C.c_int),
         "omega_batch_autotune": ([vp, C.c_int, dp, vp], C.c_int),
         "omega_batch_info": ([vp, ip, i64p, i64p], C.c_int),
+        "omega_batch_adjoint_run": ([vp, C.c_double, C.c_int, C.c_int, C.c_int64, pp, dp, dp, pp,
+                                     dp, vp], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -437,15 +439,18 @@ class ShallowWaterModel:
 
     def set_variant(self, variant: int):
         """0: pattern-specialised kernels when possible, 1: general kernels,
-        2: LDS-tiled kernels."""
+        2: LDS-tiled kernels, 3: union-stencil specialised kernel,
+        4: hexagonal LDS-tiled edge kernel + specialised cell kernel."""
         check(lib().omega_model_set_variant(self._h, int(variant)))
 
     def set_tiling(self, cells_per_tile: int = 32, levels_per_block: int = 8):
-        """Tile shape of the LDS-tiled kernels (variant 2)."""
+        """Tile shape of the LDS-tiled kernels (variants 2 and 4): cells per
+        tile and levels per block (8/16 for variant 2, 16/32 for variant 4)."""
         check(lib().omega_model_set_tiling(self._h, int(cells_per_tile), int(levels_per_block)))
 
     def kernel_variant(self) -> int:
-        """3 tiled, 2 pattern-specialised, 1 general kernels."""
+        """5 hexagonal tiled, 4 union-stencil, 3 tiled, 2 pattern-specialised,
+        1 general kernels."""
         v = C.c_int()
         check(lib().omega_model_kernel_variant(self._h, C.byref(v)))
         return v.value
@@ -566,6 +571,26 @@ class Tape:
         check(lib().omega_tape_reverse(self._h, cot._h, ptr, _stream(stream)))
         return np.array(buf[:])
 
+    def adjoint_run(self, t0: float, nsteps: int, obs, grads, weights=(1.0, 0.0, 1.0),
+                    n_snapshots: int = 10, byte_cap: int = 0, stream=None):
+        """Ensemble-mean misfit J = (1/M) sum_m J_m after nsteps steps and its
+        gradient: grads[m] = dJ/d(member m's initial state); returns (J,
+        gradient w.r.t. the shared coefficients, in the order of
+        PARAM_NAMES).  Members must share one mesh, layer and tracer counts,
+        parameters and dt; they are left unchanged."""
+        M = len(self._members)
+        if len(obs) != M or len(grads) != M:
+            raise ValueError("one observation and one gradient state per member")
+        o = (C.c_void_p * M)(*[x._h.value if isinstance(x._h, C.c_void_p) else x._h for x in obs])
+        g = (C.c_void_p * M)(*[x._h.value if isinstance(x._h, C.c_void_p) else x._h for x in grads])
+        w = (C.c_double * 3)(*[float(x) for x in weights])
+        J = C.c_double()
+        buf, ptr = _pgrad_out([0.0] * 4)
+        check(lib().omega_batch_adjoint_run(self._h, float(t0), int(nsteps), int(n_snapshots),
+                                            int(byte_cap), o, w, C.byref(J), g, ptr,
+                                            _stream(stream)))
+        return J.value, np.array(buf[:])
+
     def info(self):
         n, fs, nb = C.c_int(), C.c_int64(), C.c_int64()
         check(lib().omega_tape_info(self._h, C.byref(n), C.byref(fs), C.byref(nb)))
@@ -640,6 +665,26 @@ class Batch:
         ms = C.c_double()
         check(lib().omega_batch_autotune(self._h, int(reps), C.byref(ms), _stream(stream)))
         return ms.value
+
+    def adjoint_run(self, t0: float, nsteps: int, obs, grads, weights=(1.0, 0.0, 1.0),
+                    n_snapshots: int = 10, byte_cap: int = 0, stream=None):
+        """Ensemble-mean misfit J = (1/M) sum_m J_m after nsteps steps and its
+        gradient: grads[m] = dJ/d(member m's initial state); returns (J,
+        gradient w.r.t. the shared coefficients, in the order of
+        PARAM_NAMES).  Members must share one mesh, layer and tracer counts,
+        parameters and dt; they are left unchanged."""
+        M = len(self._members)
+        if len(obs) != M or len(grads) != M:
+            raise ValueError("one observation and one gradient state per member")
+        o = (C.c_void_p * M)(*[x._h.value if isinstance(x._h, C.c_void_p) else x._h for x in obs])
+        g = (C.c_void_p * M)(*[x._h.value if isinstance(x._h, C.c_void_p) else x._h for x in grads])
+        w = (C.c_double * 3)(*[float(x) for x in weights])
+        J = C.c_double()
+        buf, ptr = _pgrad_out([0.0] * 4)
+        check(lib().omega_batch_adjoint_run(self._h, float(t0), int(nsteps), int(n_snapshots),
+                                            int(byte_cap), o, w, C.byref(J), g, ptr,
+                                            _stream(stream)))
+        return J.value, np.array(buf[:])
 
     def info(self):
         n, b, w = C.c_int(), C.c_int64(), C.c_int64()

@@ -294,6 +294,11 @@ CONFIGS = {
                init="gaussian_double_vortex", visc_del2=1e3, eddy_diff4=1e9, f0=1e-4, grav=9.81),
     "c3": dict(nx=256, dc=10e3, nlayers=32, ntracers=2, stepper="rk4", nsteps=200,
                init="inertia_gravity_wave", visc_del2=1e3, eddy_diff4=0.0, f0=1e-4, grav=9.81),
+    # ensemble adjoint: 16 members on one mesh, shared del2 viscosity and
+    # tracer del4, checkpoints capped at 8 GB
+    "c5": dict(nx=128, dc=10e3, nlayers=64, ntracers=2, stepper="rk4", nsteps=500,
+               init="inertia_gravity_wave", visc_del2=1e3, eddy_diff4=1e9, f0=1e-4, grav=9.81,
+               nmembers=16, byte_cap=8_000_000_000),
 }
 
 

@@ -399,3 +399,43 @@ def test_union_stencil_steps_bitwise(kind, nstages, disable):
     st.run(0.0, 30.0, 3, gs)
     for name, a, b in zip(("h", "vn", "tr"), gs.download(), ref):
         assert same_bits(a, b), name
+
+
+@pytest.mark.parametrize("ordering", ["original", "hilbert", "morton"])
+@pytest.mark.parametrize("terms", [1, 2, 4, 8, 16, 32, 64, ALL_TERMS])
+@pytest.mark.parametrize("tiling", [(32, 16), (16, 32), (7, 16), (32, 8)])
+def test_hex_tiled_kernel_bitwise(ordering, terms, tiling):
+    """The hexagonal LDS-tiled edge kernel (tile rows staged once, stencil
+    slots from the built-in pattern, partial last tile) gives the oracle's
+    bits."""
+    om, omodel, gm, gmodel, state = make_pair(12, 10, 128, 3, ordering=ordering)
+    print("case", ordering, terms, tiling, flush=True)
+    omodel.terms = terms
+    gmodel.terms = terms
+    gmodel.set_variant(4)
+    gmodel.set_tiling(*tiling)
+    assert gmodel.kernel_variant() == 5
+    tend_o = omodel.new_state()
+    omodel.compute_tendency(tend_o, state, 0.0)
+    gt = swe.ShallowWaterState(gm, 3)
+    gmodel.compute_tendency(gt, upload(gm, 3, state), 0.0)
+    for name, a, b in zip(("h", "vn", "tr"), gt.download(), tend_o):
+        assert same_bits(a, b), name
+
+
+@pytest.mark.parametrize("kind,nstages", [("rk4", 4), ("lsrk", 5)])
+@pytest.mark.parametrize("disable", [(False, False), (True, False), (False, True)])
+def test_hex_tiled_steps_bitwise(kind, nstages, disable):
+    om, omodel, gm, gmodel, state = make_pair(12, 10, 64, 2, disable_h=disable[0],
+                                              disable_vn=disable[1])
+    print("case", kind, disable, flush=True)
+    gmodel.set_variant(4)
+    gmodel.set_tiling(32, 16)
+    ref = tuple(a.copy() for a in state)
+    po.Stepper(omodel, kind, nstages).run(0.0, 30.0, 3, ref)
+    gs = upload(gm, 2, state)
+    st = swe.RK4Stepper(gmodel) if kind == "rk4" else swe.LSRKStepper(gmodel, nstages)
+    st.set_graph(True)
+    st.run(0.0, 30.0, 3, gs)
+    for name, a, b in zip(("h", "vn", "tr"), gs.download(), ref):
+        assert same_bits(a, b), name
