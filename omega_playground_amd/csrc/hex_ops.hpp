@@ -493,6 +493,129 @@ __device__ __forceinline__ void edge_tend_hex2(const MeshView &m, const ModelCoe
   epilogue<N>(a, a.vn, idx, T);
 }
 
+// Vertex-union form of edge_tend_hex2: the PV products of the cell's 16
+// union vertices (hex_pattern.hpp, namespace hexu) are formed once per cell
+// by the caller and shared by its three edges; the rows per edge are then
+// 10 thicknesses and 10 velocities.  Same values, same expressions.
+template <int K>
+__device__ __forceinline__ void edge_tend_hex2v(const MeshView &m, const ModelCoef &mc,
+                                                const StageArgs &a, const DiagOut &d, int eA,
+                                                int eB, bool hi, int k, const Vd<2> (&P)[hexu::kNV],
+                                                const Vd<2> (&Q)[hexu::kNV]) {
+  constexpr int N = 2;
+  const int L = m.L;
+  const int e = hi ? eB : eA;
+  const size_t idx = (size_t)e * L + k;
+  if (!a.do_vn) {
+    if (a.epi == EPI_STORE) return;
+    epilogue<N>(a, a.vn, idx, tend_base<N>(a, a.vn, idx));
+    return;
+  }
+  const double *__restrict__ h = a.h.in;
+  const double *__restrict__ vn = a.vn.in;
+  const int *rA = m.hexrec + (size_t)eA * hex::kRecInts;
+  const int *rB = m.hexrec + (size_t)eB * hex::kRecInts;
+  int ej[10], uc[10];
+#pragma unroll
+  for (int j = 0; j < 10; ++j) {
+    const int ea = cld(rA, j), eb = cld(rB, j), ca = cld(rA, 20 + j), cb = cld(rB, 20 + j);
+    ej[j] = hi ? eb : ea;
+    uc[j] = hi ? cb : ca;
+  }
+  constexpr int o0 = hex::OWNV[K][0], o1 = hex::OWNV[K][1];
+  constexpr int q0 = hex::OWNC[K][0], q1 = hex::OWNC[K][1];
+  constexpr int v0 = hexu::OV[K][0], v1 = hexu::OV[K][1];
+  const bool pv = a.terms & T_PV;
+  Vd<N> hU[10], vJ[10];
+#pragma unroll
+  for (int u = 0; u < 10; ++u) hU[u] = rowv<N>(h, uc[u], L, k);
+  if (pv) {
+#pragma unroll
+    for (int j = 0; j < 10; ++j) vJ[j] = rowv<N>(vn, ej[j], L, k);
+  }
+  const int c0 = uc[q0], c1 = uc[q1];
+  Vd<N> T = tend_base<N>(a, a.vn, idx);
+  const double idc = pick2(m.inv_dc, eA, eB, hi);
+  if (pv) {
+    double w[10];
+#pragma unroll
+    for (int j = 0; j < 10; ++j) w[j] = pick2(m.w, (long)eA * 10 + j, (long)eB * 10 + j, hi);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const double qs = 0.5 * (P[v0][i] + P[v1][i]) + 0.5 * (Q[v0][i] + Q[v1][i]);
+      double qt = 0.0;
+#pragma unroll
+      for (int j = 0; j < 10; ++j) {
+        const double fh = 0.5 * (hU[hex::LC0[K][j]][i] + hU[hex::LC1[K][j]][i]);
+        const double nrv = 0.5 * (P[hexu::TV0[K][j]][i] + P[hexu::TV1[K][j]][i]);
+        const double npv = 0.5 * (Q[hexu::TV0[K][j]][i] + Q[hexu::TV1[K][j]][i]);
+        const double nv = (qs + nrv + npv) * 0.5;
+        qt = qt + w[j] * fh * vJ[j][i] * nv;
+      }
+      T[i] = T[i] + qt;
+    }
+  }
+  if (a.terms & T_KE) {
+    const Vd<N> k0 = rowv<N>(d.ke_c, c0, L, k), k1 = rowv<N>(d.ke_c, c1, L, k);
+#pragma unroll
+    for (int i = 0; i < N; ++i) T[i] = T[i] - (k1[i] - k0[i]) * idc;
+  }
+  if (a.terms & T_SSH) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) T[i] = T[i] - mc.grav * (hU[q1][i] - hU[q0][i]) * idc;
+  }
+  if (a.terms & (T_VDEL2 | T_VDEL4)) {
+    const int ml = pick2(m.maxlev_e, eA, eB, hi);
+    const double idv = pick2(m.inv_dv, eA, eB, hi);
+    const int b0 = pick2(rA, 10 + o0, (long)(rB - rA) + 10 + o0, hi);
+    const int b1 = pick2(rA, 10 + o1, (long)(rB - rA) + 10 + o1, hi);
+    if (a.terms & T_VDEL2) {
+      const Vd<N> d0 = rowv<N>(d.div_c, c0, L, k), d1 = rowv<N>(d.div_c, c1, L, k);
+      const Vd<N> r0 = rowv<N>(d.rvort_v, b0, L, k), r1 = rowv<N>(d.rvort_v, b1, L, k);
+      const double cv = pick2(mc.cv2, eA, eB, hi);
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const double del2u = (d1[i] - d0[i]) * idc - (r1[i] - r0[i]) * idv;
+        T[i] = T[i] + ((k + i < ml) ? cv : 0.0) * del2u;
+      }
+    }
+    if (a.terms & T_VDEL4) {
+      const Vd<N> d0 = rowv<N>(d.vdel2_div_c, c0, L, k), d1 = rowv<N>(d.vdel2_div_c, c1, L, k);
+      const Vd<N> r0 = rowv<N>(d.vdel2_rv_v, b0, L, k), r1 = rowv<N>(d.vdel2_rv_v, b1, L, k);
+      const double cv = pick2(mc.cv4, eA, eB, hi);
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const double del2u = (d1[i] - d0[i]) * idc - (r1[i] - r0[i]) * idv;
+        T[i] = T[i] - ((k + i < ml) ? cv : 0.0) * del2u;
+      }
+    }
+  }
+  epilogue<N>(a, a.vn, idx, T);
+}
+
+// PV products inv_thick*rel_vort and inv_thick*f of the 16 union vertices
+// of the wavefront's two cells (lane's cell picked per half-wavefront).
+__device__ __forceinline__ void union_pv2(const MeshView &m, const ModelCoef &mc,
+                                          const DiagOut &d, int cA, int cB, bool hi, int k,
+                                          Vd<2> (&P)[hexu::kNV], Vd<2> (&Q)[hexu::kNV]) {
+  const int L = m.L;
+  const int *uA = m.hexu + (size_t)cA * hexu::kRec + hexu::kNE;
+  const int *uB = m.hexu + (size_t)cB * hexu::kRec + hexu::kNE;
+#pragma unroll
+  for (int x = 0; x < hexu::kNV; ++x) {
+    const int va = cld(uA, x), vb = cld(uB, x);
+    const int v = hi ? vb : va;
+    const double fa = cld(mc.f_v, va), fb = cld(mc.f_v, vb);
+    const double f = hi ? fb : fa;
+    const Vd<2> rv = rowv<2>(d.rvort_v, v, L, k), it = rowv<2>(d.inv_thick_v, v, L, k);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      P[x][i] = it[i] * rv[i];
+      Q[x][i] = it[i] * f;
+    }
+  }
+}
+
 __device__ __forceinline__ void cell_tend_hex2(const MeshView &m, const ModelCoef &mc,
                                                const StageArgs &a, const DiagOut &d, int cA,
                                                int cB, bool hi, int k) {

@@ -11,7 +11,10 @@
 // vertices and cells they touch) once, issues all row loads of the 10-point
 // PV stencil back to back -- 10 distinct cells, 10 vertices, 10 edges
 // instead of 20 + 20 + 10 -- and indexes them through the compile-time slot
-// tables of hex_pattern.hpp.  Sums and products keep the reference's order,
+// tables of hex_pattern.hpp.  The default N = 2 kernel (k_tend_hex2v) puts
+// two cells in a wavefront, reads their tables through the scalar cache and
+// forms the vertex PV products of each cell's 16 union vertices once for
+// its three edges.  Sums and products keep the reference's order,
 // so results are the same bits as the general kernels.  The vector-memory
 // address path of the CU is what bounds these kernels, so the aim throughout
 // is the fewest load instructions per level.
@@ -72,6 +75,27 @@ __global__ void __launch_bounds__(256)
   edge_tend_hex2<0>(m, mc, a, d, 3 * cA, 3 * cB, hi, k);
   edge_tend_hex2<1>(m, mc, a, d, 3 * cA + 1, 3 * cB + 1, hi, k);
   edge_tend_hex2<2>(m, mc, a, d, 3 * cA + 2, 3 * cB + 2, hi, k);
+  cell_tend_hex2(m, mc, a, d, cA, cB, hi, k);
+}
+
+// vertex-union form: the PV products of the cell's 16 union vertices are
+// formed once and shared by its three edges (needs the per-cell union
+// records, hexu)
+__global__ void __launch_bounds__(256)
+    k_tend_hex2v(MeshView m, ModelCoef mc, StageArgs a, DiagOut d, int swz) {
+  const int s = logical_block(blockIdx.x, gridDim.x, swz);
+  const int R = blockDim.y;
+  const int cA = uniform(s * R + ((int)threadIdx.y & ~1));
+  if (cA >= m.nc) return;
+  const int cB = min(cA + 1, m.nc - 1);
+  const bool hi = (threadIdx.y & 1) != 0;
+  if (s * R + (int)threadIdx.y >= m.nc) return;
+  const int k = (blockIdx.y * blockDim.x + threadIdx.x) * 2;
+  Vd<2> P[hexu::kNV], Q[hexu::kNV];
+  if (a.do_vn && (a.terms & T_PV)) union_pv2(m, mc, d, cA, cB, hi, k, P, Q);
+  edge_tend_hex2v<0>(m, mc, a, d, 3 * cA, 3 * cB, hi, k, P, Q);
+  edge_tend_hex2v<1>(m, mc, a, d, 3 * cA + 1, 3 * cB + 1, hi, k, P, Q);
+  edge_tend_hex2v<2>(m, mc, a, d, 3 * cA + 2, 3 * cB + 2, hi, k, P, Q);
   cell_tend_hex2(m, mc, a, d, cA, cB, hi, k);
 }
 
@@ -359,7 +383,9 @@ void launch_tend_hex(const MeshView &m, const Launch &lc, const ModelCoef &mc, c
   dim3 block;
   const int N = hex_n(m, lc);
   const dim3 grid = hex_grid(m, lc, N, block);
-  if (N == 2)
+  if (N == 2 && m.hexu_ok)
+    hipLaunchKernelGGL(k_tend_hex2v, grid, block, 0, s, m, mc, a, d, lc.swizzle);
+  else if (N == 2)
     hipLaunchKernelGGL(k_tend_hex2, grid, block, 0, s, m, mc, a, d, lc.swizzle);
   else
     hipLaunchKernelGGL(k_tend_hex<1>, grid, block, 0, s, m, mc, a, d, lc.swizzle);
