@@ -424,8 +424,10 @@ int omega_synchronize(omega_stream_t stream);
  * every launch of the library's kernels.  Off by default. */
 int omega_timing_enable(int enable);
 int omega_timing_reset(void);
-/* name: "diag", "tend", "vdel4", "update", "reduce"; returns summed
- * milliseconds and launch count since the last reset (synchronises). */
+/* name: "diag", "tend", "vdel4", "update", "reduce" (and, for the
+ * hexagonal tiled variant, "tend_edges" / "tend_cells", the two launches
+ * inside "tend"); returns summed milliseconds and launch count since the
+ * last reset (synchronises). */
 int omega_timing_read(const char *name, double *total_ms, int64_t *launches);
 
 #ifdef __cplusplus
