@@ -493,6 +493,118 @@ __device__ __forceinline__ void edge_tend_hex2(const MeshView &m, const ModelCoe
   epilogue<N>(a, a.vn, idx, T);
 }
 
+// Pair forms of vertex_diag_u / cell_diag_u (N = 2, two cells per
+// wavefront, tables through the scalar cache); same expressions.
+__device__ __forceinline__ void vertex_diag_p(const MeshView &m, int vA, int vB, bool hi, int k,
+                                              const double *__restrict__ h,
+                                              const double *__restrict__ vn, const DiagOut &d) {
+  constexpr int N = 2;
+  const int L = m.L;
+  const int v = hi ? vB : vA;
+  int e[3], c[3];
+  double crv[3], cth[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    e[j] = pick2(m.eov, 3L * vA + j, 3L * vB + j, hi);
+    c[j] = pick2(m.cov, 3L * vA + j, 3L * vB + j, hi);
+    crv[j] = pick2(m.crv, 3L * vA + j, 3L * vB + j, hi);
+    cth[j] = pick2(m.cth, 3L * vA + j, 3L * vB + j, hi);
+  }
+  Vd<N> ve[3], hc[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    ve[j] = rowv<N>(vn, e[j], L, k);
+    hc[j] = rowv<N>(h, c[j], L, k);
+  }
+  Vd<N> rv = zero<N>(), th = zero<N>();
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      rv[i] = rv[i] + crv[j] * ve[j][i];
+      th[i] = th[i] + cth[j] * hc[j][i];
+    }
+  Vd<N> inv;
+#pragma unroll
+  for (int i = 0; i < N; ++i) inv[i] = 1.0 / th[i];
+  st<N>(d.rvort_v + (size_t)v * L + k, rv);
+  st<N>(d.inv_thick_v + (size_t)v * L + k, inv);
+}
+
+__device__ __forceinline__ void cell_diag_p(const MeshView &m, int terms, int ntr, int cA, int cB,
+                                            bool hi, int k, const double *__restrict__ h,
+                                            const double *__restrict__ vn,
+                                            const double *__restrict__ tr, const DiagOut &d) {
+  constexpr int N = 2;
+  const int L = m.L, me = m.me;
+  const int c = hi ? cB : cA;
+  int E[6], EA[6], EB[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    EA[j] = cld(m.eoc, (long)cA * me + j);
+    EB[j] = cld(m.eoc, (long)cB * me + j);
+    E[j] = hi ? EB[j] : EA[j];
+  }
+  if (terms & (T_KE | T_VDEL2 | T_VDEL4)) {
+    Vd<N> ve[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) ve[j] = rowv<N>(vn, E[j], L, k);
+    Vd<N> ke = zero<N>(), div = zero<N>();
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const double cke = pick2(m.cke, (long)cA * me + j, (long)cB * me + j, hi);
+      const double cd = pick2(m.cdiv, (long)cA * me + j, (long)cB * me + j, hi);
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        ke[i] = ke[i] + cke * ve[j][i] * ve[j][i] * 0.25;
+        div[i] = div[i] + cd * ve[j][i];
+      }
+    }
+    st<N>(d.ke_c + (size_t)c * L + k, ke);
+    st<N>(d.div_c + (size_t)c * L + k, div);
+  }
+  if ((terms & T_TDEL4) && ntr > 0) {
+    int NB[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) NB[j] = pick2(m.coc, (long)cA * 6 + j, (long)cB * 6 + j, hi);
+    const Vd<N> hC = rowv<N>(h, c, L, k);
+    Vd<N> hN[6], invN[6], cm[6], invC;
+    double idc[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) hN[j] = rowv<N>(h, NB[j], L, k);
+#pragma unroll
+    for (int i = 0; i < N; ++i) invC[i] = 1.0 / hC[i];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const double cd = pick2(m.cdiv, (long)cA * me + j, (long)cB * me + j, hi);
+      idc[j] = pick2(m.inv_dc, EA[j], EB[j], hi);
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        invN[j][i] = 1.0 / hN[j][i];
+        cm[j][i] = cd * (0.5 * (hN[j][i] + hC[i]));
+      }
+    }
+    for (int l = 0; l < ntr; ++l) {
+      const double *trl = tr + (size_t)l * m.nc * L;
+      const Vd<N> tC = rowv<N>(trl, c, L, k);
+      Vd<N> tN[6];
+#pragma unroll
+      for (int j = 0; j < 6; ++j) tN[j] = rowv<N>(trl, NB[j], L, k);
+      Vd<N> d2 = zero<N>();
+#pragma unroll
+      for (int j = 0; j < 6; ++j)
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          const double nC = tC[i] * invC[i];
+          const double nN = tN[j][i] * invN[j][i];
+          const double grad = (hex::cell_is_second(j) ? (nC - nN) : (nN - nC)) * idc[j];
+          d2[i] = d2[i] + cm[j][i] * grad;
+        }
+      st<N>(d.trdel2_c + ((size_t)l * m.nc + c) * L + k, d2);
+    }
+  }
+}
+
 // Vertex-union form of edge_tend_hex2: the PV products of the cell's 16
 // union vertices (hex_pattern.hpp, namespace hexu) are formed once per cell
 // by the caller and shared by its three edges; the rows per edge are then

@@ -44,6 +44,25 @@ __global__ void __launch_bounds__(256)
   cell_diag_u<N>(m, terms, ntr, c, k, h, vn, tr, d);
 }
 
+// N = 2, two cells per wavefront, tables through the scalar cache
+__global__ void __launch_bounds__(256)
+    k_diag_hex2(MeshView m, int terms, int ntr, const double *__restrict__ h,
+                const double *__restrict__ vn, const double *__restrict__ tr, DiagOut d, int swz) {
+  const int s = logical_block(blockIdx.x, gridDim.x, swz);
+  const int R = blockDim.y;
+  const int cA = uniform(s * R + ((int)threadIdx.y & ~1));
+  if (cA >= m.nc) return;
+  const int cB = min(cA + 1, m.nc - 1);
+  const bool hi = (threadIdx.y & 1) != 0;
+  if (s * R + (int)threadIdx.y >= m.nc) return;
+  const int k = (blockIdx.y * blockDim.x + threadIdx.x) * 2;
+  if (terms & (T_PV | T_VDEL2 | T_VDEL4)) {
+    vertex_diag_p(m, 2 * cA, 2 * cB, hi, k, h, vn, d);
+    vertex_diag_p(m, 2 * cA + 1, 2 * cB + 1, hi, k, h, vn, d);
+  }
+  cell_diag_p(m, terms, ntr, cA, cB, hi, k, h, vn, tr, d);
+}
+
 template <int N>
 __global__ void __launch_bounds__(256)
     k_tend_hex(MeshView m, ModelCoef mc, StageArgs a, DiagOut d, int swz) {
@@ -359,7 +378,7 @@ void launch_diag_hex(const MeshView &m, const Launch &lc, int terms, int ntr, co
   const int N = hex_n(m, lc);
   const dim3 grid = hex_grid(m, lc, N, block);
   if (N == 2)
-    hipLaunchKernelGGL(k_diag_hex<2>, grid, block, 0, s, m, terms, ntr, h, vn, tr, d, lc.swizzle);
+    hipLaunchKernelGGL(k_diag_hex2, grid, block, 0, s, m, terms, ntr, h, vn, tr, d, lc.swizzle);
   else
     hipLaunchKernelGGL(k_diag_hex<1>, grid, block, 0, s, m, terms, ntr, h, vn, tr, d, lc.swizzle);
   OMEGA_HIP_CHECK(hipGetLastError());
