@@ -367,8 +367,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); run with --gpus 1",
+        print("bench.py measures one GPU: there is no multi-GPU path (layers shard with no "
+              "exchange, the horizontal split needs one halo exchange per stage); run with "
+              "--gpus 1",
               file=sys.stderr)
         sys.exit(2)
 
@@ -461,6 +462,14 @@ def main():
 
     workload_key = f"{args.config}:{args.ordering}:" + ",".join(str(x) for x in model.get_launch())
     traffic = load_traffic("tend", workload_key)
+    dtraffic = load_traffic("diag", workload_key)
+    # measured HBM rate of the tend kernel (counter bytes per launch over this
+    # run's HIP-event duration) and of the whole step (both kernels per stage)
+    traffic_gbs = traffic["bytes_per_launch"] / (avg_tend_ms * 1e-3) / 1e9 if traffic else None
+    step_traffic_gbs = None
+    if traffic and dtraffic:
+        step_traffic_gbs = nlaunch * (traffic["bytes_per_launch"] + dtraffic["bytes_per_launch"]) \
+            / (ms_per_step * 1e-3) / 1e9
 
     out = {
         "metric": "cell-layer-steps/s FP64 fwd+adjoint at 1 GPU; adjoint/fwd ratio; batched speedup",
@@ -486,9 +495,14 @@ def main():
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "kernel": "k_tend (fused tendency + RK4 stage update)",
                      "kernel_avg_ms": avg_tend_ms,
-                     "alg_bytes_per_launch": alg_bytes_launch},
+                     "alg_bytes_per_launch": alg_bytes_launch,
+                     "traffic_GBs": traffic_gbs,
+                     "traffic_frac": traffic_gbs / HBM_PEAK_GBS if traffic_gbs else None},
         "step_roofline": {"B_alg_per_cell_layer_step": B, "achieved_GBs": step_achieved,
-                          "frac": step_achieved / HBM_PEAK_GBS, "diag_kernel_avg_ms": avg_diag_ms},
+                          "frac": step_achieved / HBM_PEAK_GBS, "diag_kernel_avg_ms": avg_diag_ms,
+                          "traffic_GBs": step_traffic_gbs,
+                          "traffic_frac": step_traffic_gbs / HBM_PEAK_GBS
+                          if step_traffic_gbs else None},
         "finite": finite,
         "adjoint": adj,
         "batched": bat,
