@@ -144,6 +144,68 @@ __global__ void __launch_bounds__(256)
   hexops::cell_tend_hex<N>(m, mc, a, d, c, k);
 }
 
+// N = 2 forms: the two rows of 32 lanes of a wavefront carry consecutive
+// cells cA, cA + 1 of the member, whose tables come through the scalar
+// cache (hex_ops.hpp pair operators), as in the single-model kernels.
+struct Pair {
+  int cA, cB, k;
+  bool hi, ok;
+};
+
+__device__ __forceinline__ Pair pair_of(const BatchMember &B, int lb) {
+  Pair q;
+  const int chunk = lb % B.chunks, seg = lb / B.chunks;
+  const int row = (int)threadIdx.x / B.lx;
+  q.cA = uniform(seg * B.rows + (row & ~1));
+  q.cB = min(q.cA + 1, B.m.nc - 1);
+  q.hi = (row & 1) != 0;
+  q.ok = q.cA < B.m.nc && seg * B.rows + row < B.m.nc;
+  q.k = (chunk * B.lx + (int)threadIdx.x % B.lx) * 2;
+  return q;
+}
+
+__global__ void __launch_bounds__(256)
+    k_batch_diag_hex2(const BatchMember *__restrict__ desc, const int *__restrict__ blk) {
+  const int id = uniform(blk[blockIdx.x]);
+  const BatchMember &B = desc[id];
+  if (!B.dterms) return;
+  const Pair q = pair_of(B, (int)blockIdx.x - B.block0);
+  if (!q.ok) return;
+  const MeshView &m = B.m;
+  const DiagOut &d = B.d;
+  const double *h = B.a.h.in, *vn = B.a.vn.in, *tr = B.a.tr.in;
+  if (B.dterms & (T_PV | T_VDEL2 | T_VDEL4)) {
+    hexops::vertex_diag_p(m, 2 * q.cA, 2 * q.cB, q.hi, q.k, h, vn, d);
+    hexops::vertex_diag_p(m, 2 * q.cA + 1, 2 * q.cB + 1, q.hi, q.k, h, vn, d);
+  }
+  hexops::cell_diag_p(m, B.dterms, B.a.ntr, q.cA, q.cB, q.hi, q.k, h, vn, tr, d);
+}
+
+__global__ void __launch_bounds__(256)
+    k_batch_tend_hex2(const BatchMember *__restrict__ desc, const int *__restrict__ blk) {
+  const int id = uniform(blk[blockIdx.x]);
+  const BatchMember &B = desc[id];
+  const Pair q = pair_of(B, (int)blockIdx.x - B.block0);
+  if (!q.ok) return;
+  const MeshView &m = B.m;
+  const ModelCoef &mc = B.mc;
+  const StageArgs &a = B.a;
+  const DiagOut &d = B.d;
+  const int eA = 3 * q.cA, eB = 3 * q.cB;
+  if (m.hexu_ok) {
+    Vd<2> P[hexu::kNV], Q[hexu::kNV];
+    if (a.do_vn && (a.terms & T_PV)) hexops::union_pv2(m, mc, d, q.cA, q.cB, q.hi, q.k, P, Q);
+    hexops::edge_tend_hex2v<0>(m, mc, a, d, eA, eB, q.hi, q.k, P, Q);
+    hexops::edge_tend_hex2v<1>(m, mc, a, d, eA + 1, eB + 1, q.hi, q.k, P, Q);
+    hexops::edge_tend_hex2v<2>(m, mc, a, d, eA + 2, eB + 2, q.hi, q.k, P, Q);
+  } else {
+    hexops::edge_tend_hex2<0>(m, mc, a, d, eA, eB, q.hi, q.k);
+    hexops::edge_tend_hex2<1>(m, mc, a, d, eA + 1, eB + 1, q.hi, q.k);
+    hexops::edge_tend_hex2<2>(m, mc, a, d, eA + 2, eB + 2, q.hi, q.k);
+  }
+  hexops::cell_tend_hex2(m, mc, a, d, q.cA, q.cB, q.hi, q.k);
+}
+
 } // namespace
 
 #define OMEGA_BATCH_LAUNCH(K)                                                               \
@@ -176,14 +238,24 @@ void launch_batch_diag_hex(const BatchMember *desc, const int *blk, int nblocks,
                            hipStream_t s) {
   TimerScope ts("batch_diag", s);
   const int threads = 256;
-  OMEGA_BATCH_LAUNCH(k_batch_diag_hex);
+  if (N == 2) {
+    hipLaunchKernelGGL(k_batch_diag_hex2, dim3(nblocks), dim3(threads), 0, s, desc, blk);
+    OMEGA_HIP_CHECK(hipGetLastError());
+  } else {
+    OMEGA_BATCH_LAUNCH(k_batch_diag_hex);
+  }
 }
 
 void launch_batch_tend_hex(const BatchMember *desc, const int *blk, int nblocks, int N,
                            hipStream_t s) {
   TimerScope ts("batch_tend", s);
   const int threads = 256;
-  OMEGA_BATCH_LAUNCH(k_batch_tend_hex);
+  if (N == 2) {
+    hipLaunchKernelGGL(k_batch_tend_hex2, dim3(nblocks), dim3(threads), 0, s, desc, blk);
+    OMEGA_HIP_CHECK(hipGetLastError());
+  } else {
+    OMEGA_BATCH_LAUNCH(k_batch_tend_hex);
+  }
 }
 
 #undef OMEGA_BATCH_LAUNCH
