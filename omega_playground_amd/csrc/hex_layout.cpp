@@ -28,6 +28,7 @@ bool build_hex_records(DeviceMesh &dm) {
   const HostMesh &m = dm.host;
   const int NC = m.ncells, NE = m.nedges, ME = m.max_edges;
   dm.hex_ok = false;
+  dm.hexd_ok = false;
   if (ME < 6 || dm.me2 != hex::kSlots) return false;
   // cells_on_cell in device numbering, derived from the cell's edges
   dm.coc_h.assign((size_t)NC * 6, 0);
@@ -74,6 +75,24 @@ bool build_hex_records(DeviceMesh &dm) {
     }
   }
   dm.hex_ok = true;
+  // diagnostics rows: every cell's owned vertices must follow the hexd maps
+  dm.hexd_ok = true;
+  for (int c = 0; c < NC && dm.hexd_ok; ++c) {
+    int E[8], Cl[7];
+    for (int j = 0; j < 6; ++j) E[j] = dm.eoc_h[(size_t)c * ME + j];
+    Cl[0] = c;
+    for (int j = 0; j < 6; ++j) Cl[1 + j] = dm.coc_h[(size_t)c * 6 + j];
+    const int v0 = dm.vofs_h[c];
+    E[6] = dm.eov_h[3 * (size_t)v0 + 2];
+    E[7] = dm.eov_h[3 * (size_t)(v0 + 1) + 1];
+    for (int i = 0; i < 2 && dm.hexd_ok; ++i)
+      for (int j = 0; j < 3; ++j) {
+        const int v = v0 + i;
+        if (dm.eov_h[3 * (size_t)v + j] != E[hexd::VE[i][j]] ||
+            dm.cov_h[3 * (size_t)v + j] != Cl[hexd::VC[i][j]])
+          dm.hexd_ok = false;
+      }
+  }
   // per-cell union stencil: every cell must reproduce the hexu tables
   dm.hexu_ok = false;
   dm.hexu_h.assign((size_t)NC * hexu::kRec, 0);

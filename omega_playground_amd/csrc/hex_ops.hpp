@@ -605,6 +605,127 @@ __device__ __forceinline__ void cell_diag_p(const MeshView &m, int terms, int nt
   }
 }
 
+// Union form of vertex_diag_p x 2 + cell_diag_p: the rows of the cell's
+// six edges and seven cells (itself and its neighbours) plus the two other
+// edges of its owned vertices are loaded once and shared by the vertex,
+// kinetic-energy/divergence and tracer-Laplacian parts (hexd maps).  Same
+// expressions, same bits.
+__device__ __forceinline__ void diag_union_p(const MeshView &m, int terms, int ntr, int cA,
+                                             int cB, bool hi, int k, const double *__restrict__ h,
+                                             const double *__restrict__ vn,
+                                             const double *__restrict__ tr, const DiagOut &d) {
+  constexpr int N = 2;
+  const int L = m.L, me = m.me;
+  const int c = hi ? cB : cA;
+  const bool vert = terms & (T_PV | T_VDEL2 | T_VDEL4);
+  const bool kediv = terms & (T_KE | T_VDEL2 | T_VDEL4);
+  const bool del2 = (terms & T_TDEL4) && ntr > 0;
+  int EA[6], EB[6], E[8], NB[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    EA[j] = cld(m.eoc, (long)cA * me + j);
+    EB[j] = cld(m.eoc, (long)cB * me + j);
+    E[j] = hi ? EB[j] : EA[j];
+    NB[j] = pick2(m.coc, (long)cA * 6 + j, (long)cB * 6 + j, hi);
+  }
+  // the two other edges of the owned vertices 2c (slot 2) and 2c+1 (slot 1)
+  E[6] = pick2(m.eov, 6L * cA + 2, 6L * cB + 2, hi);
+  E[7] = pick2(m.eov, 6L * cA + 4, 6L * cB + 4, hi);
+  Vd<N> ve[8], hR[7];
+  if (vert || kediv) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) ve[j] = rowv<N>(vn, E[j], L, k);
+  }
+  if (vert) {
+    ve[6] = rowv<N>(vn, E[6], L, k);
+    ve[7] = rowv<N>(vn, E[7], L, k);
+  }
+  if (vert || del2) {
+    hR[0] = rowv<N>(h, c, L, k);
+#pragma unroll
+    for (int j = 0; j < 6; ++j) hR[1 + j] = rowv<N>(h, NB[j], L, k);
+  }
+  if (vert) {
+    // VorticityAuxVars::compute_vort_vertex (vorticity_aux.hpp:85-116)
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int vA = 2 * cA + i, vB = 2 * cB + i;
+      const int v = hi ? vB : vA;
+      double crv[3], cth[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        crv[j] = pick2(m.crv, 3L * vA + j, 3L * vB + j, hi);
+        cth[j] = pick2(m.cth, 3L * vA + j, 3L * vB + j, hi);
+      }
+      Vd<N> rv = zero<N>(), th = zero<N>();
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+          rv[q] = rv[q] + crv[j] * ve[hexd::VE[i][j]][q];
+          th[q] = th[q] + cth[j] * hR[hexd::VC[i][j]][q];
+        }
+      Vd<N> inv;
+#pragma unroll
+      for (int q = 0; q < N; ++q) inv[q] = 1.0 / th[q];
+      st<N>(d.rvort_v + (size_t)v * L + k, rv);
+      st<N>(d.inv_thick_v + (size_t)v * L + k, inv);
+    }
+  }
+  if (kediv) {
+    // KineticAuxVars::compute_kinetic_cell (kinetic_aux.hpp:44-73)
+    Vd<N> ke = zero<N>(), div = zero<N>();
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const double cke = pick2(m.cke, (long)cA * me + j, (long)cB * me + j, hi);
+      const double cd = pick2(m.cdiv, (long)cA * me + j, (long)cB * me + j, hi);
+#pragma unroll
+      for (int q = 0; q < N; ++q) {
+        ke[q] = ke[q] + cke * ve[j][q] * ve[j][q] * 0.25;
+        div[q] = div[q] + cd * ve[j][q];
+      }
+    }
+    st<N>(d.ke_c + (size_t)c * L + k, ke);
+    st<N>(d.div_c + (size_t)c * L + k, div);
+  }
+  if (del2) {
+    // TracerHyperDiffusionOnCell::compute_tracer_del2
+    // (tracer_hyperdiff_cell.hpp:94-129)
+    Vd<N> invN[6], cm[6], invC;
+    double idc[6];
+#pragma unroll
+    for (int q = 0; q < N; ++q) invC[q] = 1.0 / hR[0][q];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const double cd = pick2(m.cdiv, (long)cA * me + j, (long)cB * me + j, hi);
+      idc[j] = pick2(m.inv_dc, EA[j], EB[j], hi);
+#pragma unroll
+      for (int q = 0; q < N; ++q) {
+        invN[j][q] = 1.0 / hR[1 + j][q];
+        cm[j][q] = cd * (0.5 * (hR[1 + j][q] + hR[0][q]));
+      }
+    }
+    for (int l = 0; l < ntr; ++l) {
+      const double *trl = tr + (size_t)l * m.nc * L;
+      const Vd<N> tC = rowv<N>(trl, c, L, k);
+      Vd<N> tN[6];
+#pragma unroll
+      for (int j = 0; j < 6; ++j) tN[j] = rowv<N>(trl, NB[j], L, k);
+      Vd<N> d2 = zero<N>();
+#pragma unroll
+      for (int j = 0; j < 6; ++j)
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+          const double nC = tC[q] * invC[q];
+          const double nN = tN[j][q] * invN[j][q];
+          const double grad = (hex::cell_is_second(j) ? (nC - nN) : (nN - nC)) * idc[j];
+          d2[q] = d2[q] + cm[j][q] * grad;
+        }
+      st<N>(d.trdel2_c + ((size_t)l * m.nc + c) * L + k, d2);
+    }
+  }
+}
+
 // Vertex-union form of edge_tend_hex2: the PV products of the cell's 16
 // union vertices (hex_pattern.hpp, namespace hexu) are formed once per cell
 // by the caller and shared by its three edges; the rows per edge are then

@@ -93,4 +93,17 @@ constexpr int tpos(int u) {
 }
 
 } // namespace hexu
+namespace hexd {
+
+// Rows of the diagnostics of one cell: the two vertices a cell owns touch
+// the cell's six edges plus two more edges, and only cells among the cell
+// and its six neighbours.  Per owned vertex (device vertices 2c, 2c+1):
+// [vertex][j] position of edges_on_vertex[j] in (edges_on_cell[0..6),
+// extra edge 6, extra edge 7) and of cells_on_vertex[j] in (cell,
+// cells_on_cell[0..6)).  The host verifies every cell before the kernels
+// that rely on these maps run.
+constexpr int VE[2][3] = {{0, 5, 6}, {0, 7, 1}};
+constexpr int VC[2][3] = {{0, 6, 1}, {1, 2, 0}};
+
+} // namespace hexd
 } // namespace omega_hip

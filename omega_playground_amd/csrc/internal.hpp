@@ -119,7 +119,7 @@ struct MeshView {
   const Real *crv;              // (dc*inv_area_triangle)*sign [nv][3]
   const Real *cth;              // kite*inv_area_triangle     [nv][3]
   // regular-hexagon records (valid when hex_ok, see hex_pattern.hpp)
-  int hex_ok, hexu_ok;
+  int hex_ok, hexu_ok, hexd_ok;
   const int *hexrec;            // [ne][30]
   const int *hexu;              // [nc][49] per-cell union stencil (hex_pattern.hpp)
   const int *coc;               // cells_on_cell [nc][6]
@@ -143,6 +143,7 @@ struct DeviceMesh {
   // regular-hexagon records for the pattern-specialised kernels
   bool hex_ok = false;
   bool hexu_ok = false; // per-cell union records valid (hexu)
+  bool hexd_ok = false; // owned vertices follow the diagnostics maps (hexd)
   std::vector<int> hexrec_h, coc_h, hexu_h;
   DBuf<int> hexrec, coc, hexu;
 

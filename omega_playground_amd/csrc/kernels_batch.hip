@@ -174,6 +174,10 @@ __global__ void __launch_bounds__(256)
   const MeshView &m = B.m;
   const DiagOut &d = B.d;
   const double *h = B.a.h.in, *vn = B.a.vn.in, *tr = B.a.tr.in;
+  if (m.hexd_ok) {
+    hexops::diag_union_p(m, B.dterms, B.a.ntr, q.cA, q.cB, q.hi, q.k, h, vn, tr, d);
+    return;
+  }
   if (B.dterms & (T_PV | T_VDEL2 | T_VDEL4)) {
     hexops::vertex_diag_p(m, 2 * q.cA, 2 * q.cB, q.hi, q.k, h, vn, d);
     hexops::vertex_diag_p(m, 2 * q.cA + 1, 2 * q.cB + 1, q.hi, q.k, h, vn, d);

@@ -56,6 +56,10 @@ __global__ void __launch_bounds__(256)
   const bool hi = (threadIdx.y & 1) != 0;
   if (s * R + (int)threadIdx.y >= m.nc) return;
   const int k = (blockIdx.y * blockDim.x + threadIdx.x) * 2;
+  if (m.hexd_ok) {
+    diag_union_p(m, terms, ntr, cA, cB, hi, k, h, vn, tr, d);
+    return;
+  }
   if (terms & (T_PV | T_VDEL2 | T_VDEL4)) {
     vertex_diag_p(m, 2 * cA, 2 * cB, hi, k, h, vn, d);
     vertex_diag_p(m, 2 * cA + 1, 2 * cB + 1, hi, k, h, vn, d);

@@ -481,6 +481,7 @@ MeshView DeviceMesh::view() const {
   v.inv_dv4 = inv_dv4.p; v.s2 = s2.p; v.s4 = s4.p; v.crv = crv.p; v.cth = cth.p;
   v.hex_ok = hex_ok ? 1 : 0;
   v.hexu_ok = hexu_ok ? 1 : 0;
+  v.hexd_ok = hex_ok && hexd_ok ? 1 : 0;
   v.hexrec = hexrec.p;
   v.hexu = hexu.p;
   v.coc = coc.p;
