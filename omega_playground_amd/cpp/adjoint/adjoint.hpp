@@ -15,6 +15,7 @@ inline CoefficientGradient tendency_vjp(const ShallowWaterModel &model,
                                         const ShallowWaterState &state,
                                         const ShallowWaterState &tend_cot,
                                         const ShallowWaterState &state_cot) {
+  model.settle_hook(state);
   CoefficientGradient g{};
   check(omega_model_tendency_vjp(model.handle(), state.handle(), tend_cot.handle(),
                                  state_cot.handle(), g.data(), model.stream()));
@@ -25,6 +26,7 @@ inline CoefficientGradient tendency_vjp(const ShallowWaterModel &model,
 inline CoefficientGradient step_vjp(const TimeStepper &stepper, Real t, Real dt,
                                     const ShallowWaterState &state,
                                     const ShallowWaterState &cot) {
+  stepper.m_shallow_water->settle_hook(state);
   CoefficientGradient g{};
   check(omega_stepper_step_vjp(stepper.handle(), t, dt, state.handle(), cot.handle(), g.data(),
                                stepper.m_shallow_water->stream()));
@@ -46,6 +48,7 @@ public:
   }
   // state: y_0 on entry, y_nsteps on exit
   void forward(Real t0, Real dt, const ShallowWaterState &state) const {
+    m_stepper->m_shallow_water->settle_hook(state);
     check(omega_tape_forward(m_handle, t0, dt, state.handle(), stream()));
   }
   // cot: dJ/dy_nsteps on entry, dJ/dy_0 on exit
@@ -74,6 +77,27 @@ inline Real misfit(const ShallowWaterModel &model, const ShallowWaterState &stat
   check(omega_misfit(model.handle(), state.handle(), obs.handle(), weights.data(), cot.handle(),
                      &J, model.stream()));
   return J;
+}
+
+// Misfit after nsteps steps from state (left unchanged) and its gradient:
+// grad = dJ/dy_0 (overwritten); binomial checkpointing with n_snapshots
+// stored states and an optional byte cap.
+struct AdjointResult {
+  Real J = 0;
+  CoefficientGradient coefficients{};
+};
+
+inline AdjointResult adjoint_run(const TimeStepper &stepper, Real t0, Real dt, Int nsteps,
+                                 const ShallowWaterState &state, const ShallowWaterState &obs,
+                                 const ShallowWaterState &grad,
+                                 const std::array<Real, 3> &weights = {1.0, 0.0, 1.0},
+                                 Int n_snapshots = 10, int64_t byte_cap = 0) {
+  stepper.m_shallow_water->settle_hook(state);
+  AdjointResult r;
+  check(omega_adjoint_run(stepper.handle(), t0, dt, nsteps, n_snapshots, byte_cap, state.handle(),
+                          obs.handle(), weights.data(), &r.J, grad.handle(),
+                          r.coefficients.data(), stepper.m_shallow_water->stream()));
+  return r;
 }
 
 } // namespace omega

@@ -95,11 +95,22 @@ struct ShallowWaterModel {
     }
   }
 
+  // For calls that need a model without a hook (adjoint, ensembles): one
+  // tendency evaluation of `state` into a scratch state lets the base
+  // implementation report itself, so only a real override stays registered.
+  void settle_hook(const ShallowWaterState &state) const {
+    if (m_hook_dropped || m_hook_settled) return;
+    ShallowWaterState scratch(m_mesh, state.ntracers());
+    compute_tendency(scratch, state, 0.0);
+    m_hook_settled = true;
+  }
+
 private:
   omega_model_t m_handle = nullptr;
   void *m_stream = nullptr;
   mutable bool m_base_hook_called = false;
   mutable bool m_hook_dropped = false;
+  mutable bool m_hook_settled = false;
 
   void field_tendency(int field, const ShallowWaterState &tend, const ShallowWaterState &state,
                       AddMode add_mode) const {

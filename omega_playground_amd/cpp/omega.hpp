@@ -11,3 +11,4 @@
 #include "model/shallow_water_state.hpp"
 #include "time_integration/time_stepper.hpp"
 #include "adjoint/adjoint.hpp"
+#include "ensemble/ensemble.hpp"

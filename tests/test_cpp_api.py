@@ -15,13 +15,14 @@ import pytest
 from omega_playground_amd import build as b
 
 BIN = os.path.join(os.path.dirname(__file__), "cpp", "bin", "reference_tests")
+API_BIN = os.path.join(os.path.dirname(__file__), "cpp", "bin", "api_tests")
 
 
-def _run(*args, timeout=240):
-    if not os.path.exists(BIN):
-        raise RuntimeError(f"{BIN} is missing: run __graft_entry__.build() first")
-    print("run", BIN, *args, flush=True)
-    r = subprocess.run([BIN, *args], capture_output=True, text=True, timeout=timeout)
+def _run(*args, timeout=240, exe=BIN):
+    if not os.path.exists(exe):
+        raise RuntimeError(f"{exe} is missing: run __graft_entry__.build() first")
+    print("run", exe, *args, flush=True)
+    r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=timeout)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stdout + r.stderr
     return r.stdout
@@ -31,6 +32,15 @@ def test_cpp_api_compiles_and_links(tmp_path, monkeypatch):
     monkeypatch.setattr(b, "CPP_TESTS", str(tmp_path))
     src = os.path.join(os.path.dirname(__file__), "cpp", "reference_tests.cpp")
     (tmp_path / "reference_tests.cpp").write_text(open(src).read())
+    exe, = b.build_cpp_programs()
+    assert os.path.exists(exe)
+
+
+def test_cpp_additions_compile_and_link(tmp_path, monkeypatch):
+    """The adjoint and ensemble classes (omega.hpp) in a driver program."""
+    monkeypatch.setattr(b, "CPP_TESTS", str(tmp_path))
+    src = os.path.join(os.path.dirname(__file__), "cpp", "api_tests.cpp")
+    (tmp_path / "api_tests.cpp").write_text(open(src).read())
     exe, = b.build_cpp_programs()
     assert os.path.exists(exe)
 
@@ -53,3 +63,9 @@ def test_cpp_nonlinear_swe_convergence():
 @pytest.mark.parametrize("kind", ["adv", "diff", "hyper"])
 def test_cpp_tracer_convergence(kind):
     _run("tracer", kind)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("what", ["adjoint", "ensemble", "ensemble_adjoint"])
+def test_cpp_adjoint_and_ensemble_classes(what):
+    _run(what, exe=API_BIN)
