@@ -125,6 +125,7 @@ void ensure_adjoint(omega_model_s *M) {
     M->ect4.upload(e4);
   }
   if (!M->a_sbar.p) {
+    M->a_se.alloc(NE * L);
     M->a_sbar.alloc(NE * L);
     M->a_fbar.alloc(NE * L);
     M->a_e2bar.alloc(NE * L);
@@ -145,7 +146,8 @@ void ensure_adjoint(omega_model_s *M) {
 AdjScratch scratch(omega_model_s *M) {
   return AdjScratch{M->a_sbar.p, M->a_zbar.p,  M->a_tbar.p,  M->a_kbar.p,
                     M->a_dbar.p, M->a_gbar.p,  M->a_fbar.p,  M->a_d2bar.p,
-                    M->a_z2bar.p, M->a_e2bar.p, M->a_partial.p, M->a_pgrad.p};
+                    M->a_z2bar.p, M->a_e2bar.p, M->a_partial.p, M->a_pgrad.p,
+                    M->a_se.p};
 }
 
 // out += J(y)^T k for the tendency at state y (pointers in device numbering);

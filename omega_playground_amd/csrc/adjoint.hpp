@@ -57,6 +57,7 @@ struct AdjScratch {
   Real *e2bar_e;  // [ne][L] momentum del4: del2 edge
   Real *partial;  // [4][nparts] parameter-gradient block partials
   Real *pgrad;    // [4] accumulated parameter gradient
+  Real *se_e;     // [ne][L] edge PV sum Q_e + P_e at the linearisation state
 };
 
 struct AdjArgs {

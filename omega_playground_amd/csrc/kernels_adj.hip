@@ -146,11 +146,25 @@ __global__ void __launch_bounds__(kAB)
 // ---------------------------------------------------------------------------
 template <int N>
 __global__ void __launch_bounds__(kAB)
-    k_adj_pv(MeshView m, AdjMeshView am, AdjArgs a, AdjScratch s, int swz) {
+    k_adj_pv(MeshView m, AdjMeshView am, ModelCoef mc, AdjArgs a, DiagOut d, AdjScratch s,
+             int swz) {
   const Seg g = segment<N>(m, swz);
   if (!g.active) return;
   const int L = m.L, k = g.k;
   for (int e = m.eofs[g.c0] + (int)threadIdx.y; e < m.eofs[g.c1]; e += blockDim.y) {
+    {
+      // edge PV sum S_e = Q_e + P_e, read by the edge pass for e and for
+      // every edge whose edges-on-edge list holds e
+      const int v0 = m.voe[2 * e], v1 = m.voe[2 * e + 1];
+      const double fv0 = mc.f_v[v0], fv1 = mc.f_v[v1];
+      const Vd<N> i0 = rd<N>(d.inv_thick_v, v0, L, k), i1 = rd<N>(d.inv_thick_v, v1, L, k);
+      const Vd<N> z0 = rd<N>(d.rvort_v, v0, L, k), z1 = rd<N>(d.rvort_v, v1, L, k);
+      Vd<N> Se;
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+        Se[i] = 0.5 * (i0[i] * z0[i] + i1[i] * z1[i]) + 0.5 * (i0[i] * fv0 + i1[i] * fv1);
+      st<N>(s.se_e + (size_t)e * L + k, Se);
+    }
     const Vd<N> ue = rd<N>(a.vn, e, L, k);
     const Vd<N> ute = rd<N>(a.c.vn, e, L, k);
     const Vd<N> he0 = rd<N>(a.h, m.coe[2 * e], L, k), he1 = rd<N>(a.h, m.coe[2 * e + 1], L, k);
@@ -325,26 +339,16 @@ __global__ void __launch_bounds__(kAB)
       }
       if (a.do_vn) {
         if (pv) {
-          // S_e = Q_e + P_e of this edge
-          const double fv0 = mc.f_v[v0], fv1 = mc.f_v[v1];
-          const Vd<N> i0 = rd<N>(d.inv_thick_v, v0, L, k), i1 = rd<N>(d.inv_thick_v, v1, L, k);
-          const Vd<N> z0 = rd<N>(d.rvort_v, v0, L, k), z1 = rd<N>(d.rvort_v, v1, L, k);
-          Vd<N> Se;
-#pragma unroll
-          for (int i = 0; i < N; ++i)
-            Se[i] = 0.5 * (i0[i] * z0[i] + i1[i] * z1[i]) + 0.5 * (i0[i] * fv0 + i1[i] * fv1);
+          // S_e = Q_e + P_e of this edge and S_t of each t (k_adj_pv)
+          const Vd<N> Se = rd<N>(s.se_e, e, L, k);
           for (int q = am.tofs[e]; q < am.tofs[e + 1]; ++q) {
             const int t = am.tedge[q];
             const double w = am.tw[q];
-            const int b0 = m.voe[2 * t], b1 = m.voe[2 * t + 1];
-            const double ft0 = mc.f_v[b0], ft1 = mc.f_v[b1];
-            const Vd<N> j0 = rd<N>(d.inv_thick_v, b0, L, k), j1 = rd<N>(d.inv_thick_v, b1, L, k);
-            const Vd<N> y0 = rd<N>(d.rvort_v, b0, L, k), y1 = rd<N>(d.rvort_v, b1, L, k);
+            const Vd<N> Stv = rd<N>(s.se_e, t, L, k);
             const Vd<N> ut = rd<N>(a.c.vn, t, L, k);
 #pragma unroll
             for (int i = 0; i < N; ++i) {
-              const double St =
-                  0.5 * (j0[i] * y0[i] + j1[i] * y1[i]) + 0.5 * (j0[i] * ft0 + j1[i] * ft1);
+              const double St = Stv[i];
               const double nv = (St + Se[i]) * 0.5;
               ub[i] += ut[i] * w * F[i] * nv;
               fb[i] += ut[i] * w * ue[i] * nv;
@@ -649,7 +653,7 @@ void launch_tendency_vjp(const MeshView &m, const AdjMeshView &am, const Launch 
     OMEGA_ADJ_LAUNCH(k_adj_v4a, m, mc, a, s, swz);
     OMEGA_ADJ_LAUNCH(k_adj_v4b, m, am, s, swz);
   }
-  if (pv) OMEGA_ADJ_LAUNCH(k_adj_pv, m, am, a, s, swz);
+  if (pv) OMEGA_ADJ_LAUNCH(k_adj_pv, m, am, mc, a, d, s, swz);
   OMEGA_ADJ_LAUNCH(k_adj_vc, m, mc, ac, a, d, s, swz);
   OMEGA_ADJ_LAUNCH(k_adj_edge, m, am, mc, ac, a, d, s, swz);
   OMEGA_ADJ_LAUNCH(k_adj_cell, m, am, mc, ac, a, d, s, swz);
