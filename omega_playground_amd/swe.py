@@ -282,6 +282,58 @@ class MPASMesh:
         self._init_info()
         return self
 
+    # MPAS mesh-file variables read by from_netcdf: (reference name, file
+    # name, kind); "index" variables are 1-based in the file, 0 = none
+    NETCDF_VARIABLES = (
+        ("nedges_on_cell", "nEdgesOnCell", "count"), ("edges_on_cell", "edgesOnCell", "index"),
+        ("vertices_on_cell", "verticesOnCell", "index"),
+        ("nedges_on_edge", "nEdgesOnEdge", "count"), ("edges_on_edge", "edgesOnEdge", "index"),
+        ("weights_on_edge", "weightsOnEdge", "real"), ("cells_on_edge", "cellsOnEdge", "index"),
+        ("vertices_on_edge", "verticesOnEdge", "index"),
+        ("edges_on_vertex", "edgesOnVertex", "index"), ("cells_on_vertex", "cellsOnVertex", "index"),
+        ("area_cell", "areaCell", "real"), ("dc_edge", "dcEdge", "real"),
+        ("dv_edge", "dvEdge", "real"), ("area_triangle", "areaTriangle", "real"),
+        ("kiteareas_on_vertex", "kiteAreasOnVertex", "real"),
+        ("x_cell", "xCell", "real"), ("y_cell", "yCell", "real"), ("x_edge", "xEdge", "real"),
+        ("y_edge", "yEdge", "real"), ("angle_edge", "angleEdge", "real"),
+        ("x_vertex", "xVertex", "real"), ("y_vertex", "yVertex", "real"),
+        ("mesh_scaling_del2", "meshScalingDel2", "real"),
+        ("mesh_scaling_del4", "meshScalingDel4", "real"),
+        ("max_level_edge_top", "maxLevelEdgeTop", "count"),
+    )
+
+    @classmethod
+    def from_netcdf(cls, path: str, nlayers: Optional[int] = None, ordering: str = "hilbert"):
+        """Reads an MPAS mesh file (NetCDF classic or 64-bit-offset format;
+        NetCDF-4/HDF5 files need converting first, e.g. `nccopy -k 2`).
+
+        Variables follow the MPAS mesh specification (nEdgesOnCell,
+        edgesOnCell, ..., kiteAreasOnVertex, weightsOnEdge, 1-based indices);
+        meshScalingDel2/Del4 and maxLevelEdgeTop are used when present.
+        nlayers defaults to the file's nVertLevels dimension, else 1.  The
+        reference builds meshes in memory only (mpas_mesh.hpp:7-71); this is
+        the file route to the same MPASMesh fields."""
+        from scipy.io import netcdf_file
+
+        arrays = {}
+        with netcdf_file(path, "r", mmap=False) as f:
+            var = f.variables
+            for name, ncname, kind in cls.NETCDF_VARIABLES:
+                if ncname not in var:
+                    continue
+                x = np.array(var[ncname].data)
+                if kind == "real":
+                    arrays[name] = x.astype(np.float64)
+                else:
+                    x = x.astype(np.int32)
+                    arrays[name] = x - 1 if kind == "index" else x
+            if nlayers is None:
+                nlayers = int(f.dimensions.get("nVertLevels") or 1)
+        missing = [nc for name, nc, kind in cls.NETCDF_VARIABLES[:15] if name not in arrays]
+        if missing:
+            raise OmegaError(f"{path}: MPAS mesh variables missing: {', '.join(missing)}")
+        return cls.from_arrays(arrays, nlayers, ordering)
+
 
 class PlanarHexagonalMesh(MPASMesh):
     """PlanarHexagonalMesh(nx, ny[, dc], nlayers=1)

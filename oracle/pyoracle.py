@@ -66,6 +66,8 @@ def lib():
         L.orc_mesh_array.restype = vp
         L.orc_mesh_array.argtypes = [vp, C.c_char_p, ip, lp, lp]
         L.orc_mesh_finalize.argtypes = [vp]
+        L.orc_mesh_alloc.restype = vp
+        L.orc_mesh_alloc.argtypes = [C.c_int] * 4
         L.orc_model_create.restype = vp
         L.orc_model_create.argtypes = [vp, C.POINTER(Params)]
         L.orc_model_free.argtypes = [vp]
@@ -143,6 +145,40 @@ class Mesh:
         if not p:
             raise AttributeError(name)
         return _view(p, dt.value, n0.value, n1.value)
+
+    @classmethod
+    def from_arrays(cls, arrays: dict, nlayers: int) -> "Mesh":
+        """A general MPASMesh filled from arrays (reference names, 0-based,
+        rows padded to maxedges / 2*maxedges columns), then finalize_mesh()
+        (src/mesh/mpas_mesh.hpp:7-71, mpas_mesh.cpp:5-57)."""
+        self = cls.__new__(cls)
+        nc = len(arrays["nedges_on_cell"])
+        ne = len(arrays["nedges_on_edge"])
+        nv = len(arrays["edges_on_vertex"])
+        self._h = lib().orc_mesh_alloc(nc, ne, nv, nlayers)
+        self.nx = self.ny = 0
+        self.nlayers, self.ncells, self.nedges, self.nvertices = nlayers, nc, ne, nv
+        self.dc = self.period_x = self.period_y = 0.0
+        for name, a in arrays.items():
+            if name in ("mesh_scaling_del2", "mesh_scaling_del4", "max_level_edge_top"):
+                continue
+            dst = getattr(self, name)
+            a = np.asarray(a)
+            if dst.ndim == 2 and a.ndim == 2 and a.shape[1] < dst.shape[1]:
+                dst[:, :a.shape[1]] = a
+            else:
+                dst[...] = a
+        self.finalize()
+        # optional per-edge scalings and level bounds replace finalize_mesh's
+        # unit defaults; edge_mask follows max_level_edge_top
+        for name in ("mesh_scaling_del2", "mesh_scaling_del4"):
+            if arrays.get(name) is not None:
+                getattr(self, name)[:] = arrays[name]
+        if arrays.get("max_level_edge_top") is not None:
+            top = np.asarray(arrays["max_level_edge_top"])
+            self.max_level_edge_top[:] = top
+            self.edge_mask[:] = (np.arange(nlayers)[None, :] < top[:, None]).astype(np.float64)
+        return self
 
     def finalize(self):
         lib().orc_mesh_finalize(self._h)

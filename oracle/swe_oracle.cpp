@@ -898,6 +898,17 @@ void *orc_mesh_array(void *mp, const char *name, int *dtype, long *n0, long *n1)
 // Re-derives the mesh's sign/kite arrays after a caller edited connectivity
 // (used to build general MPAS meshes in tests).
 void orc_mesh_finalize(void *mp) { static_cast<Mesh *>(mp)->finalize_mesh(); }
+// An empty MPASMesh of the given sizes (mpas_mesh.hpp:7-71): the caller fills
+// the primary arrays through orc_mesh_array and calls orc_mesh_finalize.
+void *orc_mesh_alloc(int ncells, int nedges, int nvertices, int nlayers) {
+  Mesh *m = new Mesh;
+  m->ncells = ncells;
+  m->nedges = nedges;
+  m->nvertices = nvertices;
+  m->nlayers = nlayers;
+  m->allocate();
+  return m;
+}
 
 void *orc_model_create(void *mesh, const Params *p) {
   return new Model(static_cast<Mesh *>(mesh), *p);
