@@ -178,6 +178,17 @@ typedef struct {
  * (src/model/shallow_water_model.cpp:5-41): terms are enabled from the
  * parameters exactly as there; f_vertex = f_edge = f0. */
 int omega_model_create(omega_mesh_t mesh, const omega_params *params, omega_model_t *out);
+/* Replaces LinearShallowWaterModel(MPASMesh*, const LinearShallowWaterParams&)
+ * (src/shallow_water_hierarchical.cpp:676-773, not compiled in the
+ * reference): the shallow-water equations linearised about a resting layer
+ * of thickness h0, h_t = -h0 div(vn) (:680-707) and vn_t = f_edge * (TRiSK
+ * tangential velocity) - g grad(h) (:709-741), with f_edge = f0 and no
+ * tracers (params->ntracers must be 0; the mixing coefficients are unused).
+ * Steppers, hooks, increment mode, field tendencies and the integrals
+ * (energy with h0*K, :743-773) work as for omega_model_create; the adjoint
+ * and ensembles do not take linear models. */
+int omega_model_create_linear(omega_mesh_t mesh, const omega_params *params, double h0,
+                              omega_model_t *out);
 int omega_model_destroy(omega_model_t model);
 
 /* Overwrites m_f_vertex (host array, original numbering). */
@@ -242,8 +253,8 @@ int omega_model_get_launch(omega_model_t model, int *rows_per_block, int *levels
  * vertex PV rows of a tile staged in LDS, stencil slots from the built-in
  * pattern) followed by the specialised cell kernel; 3 and 4 fall back to
  * the specialised kernels where they do not apply.
- * omega_model_kernel_variant reports what runs: 5 hexagonal tiled,
- * 4 union-stencil, 3 tiled, 2 specialised, 1 general.  No reference
+ * omega_model_kernel_variant reports what runs: 6 linear model,
+ * 5 hexagonal tiled, 4 union-stencil, 3 tiled, 2 specialised, 1 general.  No reference
  * counterpart (performance control only; all give the same bits). */
 int omega_model_set_variant(omega_model_t model, int variant);
 int omega_model_kernel_variant(omega_model_t model, int *in_use);

@@ -25,10 +25,18 @@ struct ShallowWaterModel {
   Real1d m_f_edge;
 
   ShallowWaterModel(MPASMesh *mesh, const ShallowWaterParams &params)
+      : ShallowWaterModel(mesh, params, nullptr) {}
+
+protected:
+  // linear_h0 != nullptr: LinearShallowWaterModel with that mean thickness
+  ShallowWaterModel(MPASMesh *mesh, const ShallowWaterParams &params, const Real *linear_h0)
       : m_params(params), m_mesh(mesh), m_f_vertex("f_vertex", mesh->m_nvertices),
         m_f_edge("f_edge", mesh->m_nedges) {
     const omega_params c = to_c(params);
-    check(omega_model_create(mesh->handle(), &c, &m_handle));
+    if (linear_h0)
+      check(omega_model_create_linear(mesh->handle(), &c, *linear_h0, &m_handle));
+    else
+      check(omega_model_create(mesh->handle(), &c, &m_handle));
     deep_copy(m_f_vertex, params.m_f0);
     deep_copy(m_f_edge, params.m_f0);
     // The hook is registered for every model; the base implementation
@@ -36,6 +44,8 @@ struct ShallowWaterModel {
     // without an override keep the fused stage kernels.
     check(omega_model_set_additional_tendency(m_handle, &ShallowWaterModel::hook, this));
   }
+
+public:
   ShallowWaterModel(const ShallowWaterModel &) = delete;
   ShallowWaterModel &operator=(const ShallowWaterModel &) = delete;
   virtual ~ShallowWaterModel() {

@@ -6,6 +6,7 @@
 #include "mesh/mpas_mesh.hpp"
 #include "mesh/planar_hexagonal_mesh.hpp"
 #include "model/shallow_water_diagnostics.hpp"
+#include "model/linear_shallow_water_model.hpp"
 #include "model/shallow_water_model.hpp"
 #include "model/shallow_water_params.hpp"
 #include "model/shallow_water_state.hpp"

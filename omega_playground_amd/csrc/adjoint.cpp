@@ -104,6 +104,7 @@ namespace {
 // Builds the mesh's transposed tables, the model's transposed tracer
 // coefficients and the cotangent scratch, once.
 void ensure_adjoint(omega_model_s *M) {
+  OMEGA_REQUIRE(!M->linear, "adjoint: not available for the linear model");
   OMEGA_REQUIRE(M->add_fn == nullptr,
                 "adjoint: not available while an additional-tendency hook is set");
   DeviceMesh &dm = M->mesh->dm;

@@ -287,6 +287,7 @@ int omega_batch_add(omega_batch_t B, omega_model_t M, omega_state_t S, double dt
     check_state(M, S, "batch member");
     OMEGA_REQUIRE(M->add_fn == nullptr,
                   "batch: members cannot have an additional-tendency hook");
+    OMEGA_REQUIRE(!M->linear, "batch: linear models step on their own (omega_stepper_*)");
     for (auto &mb : B->mem)
       OMEGA_REQUIRE(mb->S != S, "batch: a state can belong to one member only");
     // two levels per lane need an even layer count in every member

@@ -459,6 +459,29 @@ int omega_model_create(omega_mesh_t mesh, const omega_params *p, omega_model_t *
   });
 }
 
+int omega_model_create_linear(omega_mesh_t mesh, const omega_params *p, double h0,
+                              omega_model_t *out) {
+  return guarded([&] {
+    OMEGA_REQUIRE(mesh && p && out, "null argument");
+    OMEGA_REQUIRE(p->ntracers == 0, "the linear model carries no tracers (ntracers must be 0)");
+    OMEGA_REQUIRE(std::isfinite(h0) && h0 >= 0, "h0 must be finite and >= 0");
+    mesh->dm.upload();
+    auto M = std::make_unique<omega_model_s>();
+    M->mesh = mesh;
+    M->params = *p;
+    M->linear = true;
+    M->h0 = h0;
+    M->terms = 0;
+    const int L = mesh->dm.host.nlayers;
+    M->lc.lpt = (L % 2 == 0 && L >= 2) ? 2 : 1;
+    M->lc.rows = 0;
+    M->lc.swizzle = 1;
+    model_build_coefficients(M.get());
+    model_build_linear(M.get());
+    *out = M.release();
+  });
+}
+
 int omega_model_destroy(omega_model_t model) {
   return guarded([&] { delete model; });
 }
@@ -581,7 +604,8 @@ int omega_model_kernel_variant(omega_model_t M, int *in_use) {
     OMEGA_REQUIRE(M && in_use, "null argument");
     const MeshView mv = M->mesh->dm.view();
     ensure_tiles(M);
-    *in_use = M->lc.variant == 2                          ? 3
+    *in_use = M->linear                                   ? 6
+              : M->lc.variant == 2                        ? 3
               : hextile_usable(mv, M->mesh->tiles, M->lc) ? 5
               : hexu_usable(mv, M->lc)                    ? 4
               : hex_usable(mv, M->lc)  ? 2
@@ -679,7 +703,8 @@ int diag_integral(omega_model_t M, omega_state_t S, double *out, omega_stream_t 
     hipStream_t s = as_stream(stream);
     launch_reduce(which, h.ncells, h.nvertices, h.max_edges, h.nlayers, dm.noc.p, dm.eoc.p,
                   dm.eov.p, dm.ae_c.p, dm.area_cell.p, dm.dcs_v.p, dm.area_tri.p, M->f_v.p,
-                  M->params.grav, S->ptr(0), S->ptr(1), M->partial.p, M->result.p, s);
+                  M->params.grav, S->ptr(0), S->ptr(1), M->partial.p, M->result.p, s,
+                  M->linear ? M->h0 : -1.0);
     OMEGA_HIP_CHECK(hipMemcpyAsync(out, M->result.p, sizeof(double), hipMemcpyDeviceToHost, s));
     OMEGA_HIP_CHECK(hipStreamSynchronize(s));
   });

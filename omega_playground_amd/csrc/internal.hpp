@@ -232,6 +232,17 @@ void launch_diag(const MeshView &m, const Launch &lc, const ModelCoef &mc, int t
 void launch_vdel4(const MeshView &m, const Launch &lc, const DiagOut &d, hipStream_t s);
 void launch_tend(const MeshView &m, const Launch &lc, const ModelCoef &mc, const StageArgs &a,
                  const DiagOut &d, hipStream_t s);
+// Linear model (LinearShallowWaterModel): h_t = -h0 div(vn),
+// vn_t = f_e * (TRiSK tangential velocity) - g grad(h), with the stage update
+struct LinCoef {
+  const Real *cdv;  // dv*sign of each cell edge [nc][me]
+  const Real *area; // area_cell [nc]
+  const Real *dc;   // dc_edge [ne]
+  const Real *f_e;  // f_edge [ne]
+  Real h0, grav;
+};
+void launch_tend_linear(const MeshView &m, const Launch &lc, const LinCoef &lin,
+                        const StageArgs &a, hipStream_t s);
 // Pattern-specialised kernels for the regular hexagonal mesh (kernels_hex.hip)
 bool hex_usable(const MeshView &m, const Launch &lc);
 void launch_diag_hex(const MeshView &m, const Launch &lc, int terms, int ntr, const Real *h,
@@ -257,7 +268,8 @@ int reduce_blocks(int n);
 void launch_reduce(int which, int nc, int nv, int me, int L, const int *noc, const int *eoc,
                    const int *eov, const double *ae_c, const double *area_c, const double *dcs_v,
                    const double *area_t, const double *f_v, double grav, const double *h,
-                   const double *vn, double *partial, double *out, hipStream_t s);
+                   const double *vn, double *partial, double *out, hipStream_t s,
+                   double linear_h0 = -1.0);
 
 // Timing hooks
 struct TimerScope {

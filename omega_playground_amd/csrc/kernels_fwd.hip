@@ -95,6 +95,84 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------------------
+// Linear model (shallow_water_hierarchical.cpp:680-741) + stage update
+// ---------------------------------------------------------------------------
+
+// In replace mode the reference assigns the term itself (a -0 stays -0), in
+// increment mode it adds it to the (scaled) tendency.
+template <int N>
+__device__ __forceinline__ Vd<N> lin_total(const StageArgs &a, const FieldIO &f, size_t idx,
+                                           const Vd<N> &x) {
+  if (!a.base_from_tend) return x;
+  Vd<N> T = tend_base<N>(a, f, idx);
+#pragma unroll
+  for (int i = 0; i < N; ++i) T[i] = T[i] + x[i];
+  return T;
+}
+
+template <int N>
+__global__ void __launch_bounds__(256)
+    k_lin(MeshView m, LinCoef lc, StageArgs a, int swz) {
+  const int s = logical_block(blockIdx.x, gridDim.x, swz);
+  const int R = blockDim.y;
+  const int c0 = s * R;
+  if (c0 >= m.nc) return;
+  const int c1 = min(c0 + R, m.nc);
+  const int k = (blockIdx.y * blockDim.x + threadIdx.x) * N;
+  if (k >= m.L) return;
+  const int L = m.L;
+  const double *__restrict__ h = a.h.in;
+  const double *__restrict__ vn = a.vn.in;
+  const int eb = m.eofs[c0], ee = m.eofs[c1];
+  for (int e = eb + (int)threadIdx.y; e < ee; e += R) {
+    const size_t idx = (size_t)e * L + k;
+    if (!a.do_vn) {
+      if (a.epi != EPI_STORE) epilogue<N>(a, a.vn, idx, tend_base<N>(a, a.vn, idx));
+      continue;
+    }
+    // compute_vn_tendency (:709-741)
+    Vd<N> vt = zero<N>();
+    const int n = m.noe[e];
+    for (int j = 0; j < n; ++j) {
+      const int ej = m.eoe[e * m.me2 + j];
+      const double wj = m.w[e * m.me2 + j];
+      const Vd<N> v = ld<N>(vn + (size_t)ej * L + k);
+#pragma unroll
+      for (int i = 0; i < N; ++i) vt[i] = vt[i] + wj * v[i];
+    }
+    const int a0 = m.coe[2 * e], a1 = m.coe[2 * e + 1];
+    const Vd<N> h0 = ld<N>(h + (size_t)a0 * L + k), h1 = ld<N>(h + (size_t)a1 * L + k);
+    const double fe = lc.f_e[e], dc = lc.dc[e];
+    Vd<N> x;
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = fe * vt[i] - lc.grav * ((h1[i] - h0[i]) / dc);
+    epilogue<N>(a, a.vn, idx, lin_total<N>(a, a.vn, idx, x));
+  }
+  for (int c = c0 + (int)threadIdx.y; c < c1; c += R) {
+    const size_t idx = (size_t)c * L + k;
+    if (!a.do_h) {
+      if (a.epi != EPI_STORE) epilogue<N>(a, a.h, idx, tend_base<N>(a, a.h, idx));
+      continue;
+    }
+    // compute_h_tendency (:680-707)
+    Vd<N> acc = zero<N>();
+    const int n = m.noc[c];
+    for (int j = 0; j < n; ++j) {
+      const int e = m.eoc[c * m.me + j];
+      const double cd = lc.cdv[c * m.me + j];
+      const Vd<N> v = ld<N>(vn + (size_t)e * L + k);
+#pragma unroll
+      for (int i = 0; i < N; ++i) acc[i] = acc[i] + cd * v[i];
+    }
+    const double area = lc.area[c];
+    Vd<N> x;
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = -lc.h0 * acc[i] / area;
+    epilogue<N>(a, a.h, idx, lin_total<N>(a, a.h, idx, x));
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Element-wise helpers
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_axpy(double *__restrict__ y, double alpha,
@@ -198,6 +276,18 @@ void launch_tend(const MeshView &m, const Launch &lc, const ModelCoef &mc, const
     hipLaunchKernelGGL(k_tend<2>, g.grid, g.block, 0, s, m, mc, a, d, lc.swizzle);
   else
     hipLaunchKernelGGL(k_tend<1>, g.grid, g.block, 0, s, m, mc, a, d, lc.swizzle);
+  OMEGA_HIP_CHECK(hipGetLastError());
+}
+
+void launch_tend_linear(const MeshView &m, const Launch &lc, const LinCoef &lin,
+                        const StageArgs &a, hipStream_t s) {
+  TimerScope ts("tend", s);
+  const int N = (lc.lpt == 2 && m.L % 2 == 0) ? 2 : 1;
+  const Geo g = geometry(m, lc, N);
+  if (N == 2)
+    hipLaunchKernelGGL(k_lin<2>, g.grid, g.block, 0, s, m, lin, a, lc.swizzle);
+  else
+    hipLaunchKernelGGL(k_lin<1>, g.grid, g.block, 0, s, m, lin, a, lc.swizzle);
   OMEGA_HIP_CHECK(hipGetLastError());
 }
 

@@ -33,6 +33,7 @@ struct DiagGeom {
   const double *area_t; // [nv]
   const double *f_v;    // [nv]
   double grav;
+  double h0; // >= 0: the linear model's mean thickness weighs K; < 0: h does
 };
 
 // which: 0 energy, 1 mass, 2 circulation
@@ -55,7 +56,8 @@ __global__ void __launch_bounds__(kRB)
       }
       K /= A;
       const double hc = h[(size_t)i * L + k];
-      acc += A * (g.grav * hc * hc / 2 + hc * K);
+      // the linear model weighs K with h0 (shallow_water_hierarchical.cpp:767-769)
+      acc += A * (g.grav * hc * hc / 2 + (g.h0 >= 0 ? g.h0 : hc) * K);
     }
   } else if (which == 1 && i < g.nc) {
     // shallow_water_diagnostics.cpp:51-55
@@ -92,9 +94,10 @@ int reduce_blocks(int n) { return (n + kRB - 1) / kRB; }
 void launch_reduce(int which, int nc, int nv, int me, int L, const int *noc, const int *eoc,
                    const int *eov, const double *ae_c, const double *area_c, const double *dcs_v,
                    const double *area_t, const double *f_v, double grav, const double *h,
-                   const double *vn, double *partial, double *out, hipStream_t s) {
+                   const double *vn, double *partial, double *out, hipStream_t s,
+                   double linear_h0) {
   TimerScope ts("reduce", s);
-  DiagGeom g{nc, nv, me, L, noc, eoc, eov, ae_c, area_c, dcs_v, area_t, f_v, grav};
+  DiagGeom g{nc, nv, me, L, noc, eoc, eov, ae_c, area_c, dcs_v, area_t, f_v, grav, linear_h0};
   const int n = which == 2 ? nv : nc;
   const int nb = reduce_blocks(n);
   hipLaunchKernelGGL(k_reduce_partial, dim3(nb), dim3(kRB), 0, s, g, which, h, vn, partial);

@@ -51,6 +51,10 @@ struct omega_model_s {
   DBuf<Real> partial, result;
   omega_additional_tendency_fn add_fn = nullptr;
   void *add_user = nullptr;
+  // LinearShallowWaterModel (shallow_water_hierarchical.cpp:676-773)
+  bool linear = false;
+  Real h0 = 0;
+  DBuf<Real> lin_cdv, lin_dc, lin_fe;
   // host copies of the tracer mixing coefficients (for the adjoint tables)
   std::vector<Real> ct2_h, ct4_h;
   // adjoint: transposed tracer coefficients and cotangent scratch, built on
@@ -61,6 +65,9 @@ struct omega_model_s {
       a_partial, a_pgrad, a_mpart;
   int a_nparts = 0;
 
+  omega_hip::LinCoef lin_coef() const {
+    return omega_hip::LinCoef{lin_cdv.p, mesh->dm.area_cell.p, lin_dc.p, lin_fe.p, h0, params.grav};
+  }
   ModelCoef coef() const {
     return ModelCoef{params.grav, f_v.p, cv2.p, cv4.p, ct2.p, ct4.p};
   }
@@ -133,6 +140,7 @@ bool stream_capturing(hipStream_t s);
 
 void model_build_coefficients(omega_model_s *M);
 void ensure_vdel4(omega_model_s *M);
+void model_build_linear(omega_model_s *M);
 // Cells per tile of the tiled kernels for this model's launch shape, and
 // the tables for it (built before any capture).
 int tile_cells(const omega_model_s *M);

@@ -52,6 +52,26 @@ void model_build_coefficients(omega_model_s *M) {
   M->result.alloc(4);
 }
 
+// Tables of the linear model: dv*sign per cell edge (the leading factors of
+// shallow_water_hierarchical.cpp:696-697), dc and f per edge (:718-735).
+void model_build_linear(omega_model_s *M) {
+  const DeviceMesh &dm = M->mesh->dm;
+  const HostMesh &hm = dm.host;
+  const int NC = hm.ncells, NE = hm.nedges, ME = hm.max_edges;
+  std::vector<Real> cdv((size_t)NC * ME, 0.0), dc(NE), fe(NE, M->params.f0);
+  for (int i = 0; i < NC; ++i) {
+    const int c = dm.cell_n2o[i];
+    for (int j = 0; j < dm.noc_h[i]; ++j) {
+      const int eo = hm.edges_on_cell[(size_t)c * ME + j];
+      cdv[(size_t)i * ME + j] = hm.dv_edge[eo] * hm.edge_sign_on_cell[(size_t)c * ME + j];
+    }
+  }
+  for (int i = 0; i < NE; ++i) dc[i] = hm.dc_edge[dm.edge_n2o[i]];
+  M->lin_cdv.upload(cdv);
+  M->lin_dc.upload(dc);
+  M->lin_fe.upload(fe);
+}
+
 void ensure_vdel4(omega_model_s *M) {
   const HostMesh &hm = M->mesh->dm.host;
   const size_t L = hm.nlayers;
@@ -96,6 +116,10 @@ void run_stage(omega_model_s *M, StageArgs a, hipStream_t s, int fields) {
   a.do_h = !M->params.disable_h_tendency && (fields & 1);
   a.do_vn = !M->params.disable_vn_tendency && (fields & 2);
   a.ntr = (fields & 4) ? M->params.ntracers : 0;
+  if (M->linear) {
+    launch_tend_linear(mv, M->lc, M->lin_coef(), a, s);
+    return;
+  }
   run_diag(M, a.h.in, a.vn.in, a.tr.in, s);
   const DiagOut d = M->diag();
   if (M->lc.variant == 2 || M->lc.variant == 4) ensure_tiles(M);

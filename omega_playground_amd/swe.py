@@ -91,6 +91,7 @@ def lib():
         "omega_state_device_ptrs": ([vp, pp, pp, pp], C.c_int),
         "omega_state_copy": ([vp, vp, vp], C.c_int),
         "omega_model_create": ([vp, C.POINTER(Params), pp], C.c_int),
+        "omega_model_create_linear": ([vp, C.POINTER(Params), C.c_double, pp], C.c_int),
         "omega_model_destroy": ([vp], C.c_int),
         "omega_model_set_f_vertex": ([vp, dp], C.c_int),
         "omega_model_set_additional_tendency": ([vp, ADDFN, vp], C.c_int),
@@ -431,7 +432,7 @@ class ShallowWaterModel:
         self.params = params
         h = C.c_void_p()
         self._cparams = params.to_c()
-        check(lib().omega_model_create(mesh._h, C.byref(self._cparams), C.byref(h)))
+        self._create(h)
         self._h = h
         self._cb = None
         if type(self).additional_tendency is not ShallowWaterModel.additional_tendency:
@@ -444,6 +445,9 @@ class ShallowWaterModel:
                     return 1
             self._cb = ADDFN(tramp)
             check(lib().omega_model_set_additional_tendency(self._h, self._cb, None))
+
+    def _create(self, h):
+        check(lib().omega_model_create(self.mesh._h, C.byref(self._cparams), C.byref(h)))
 
     def additional_tendency(self, h_tend, vn_tend, h, vn, t, stream):
         pass
@@ -501,7 +505,7 @@ class ShallowWaterModel:
         check(lib().omega_model_set_tiling(self._h, int(cells_per_tile), int(levels_per_block)))
 
     def kernel_variant(self) -> int:
-        """5 hexagonal tiled, 4 union-stencil, 3 tiled, 2 pattern-specialised,
+        """6 linear model, 5 hexagonal tiled, 4 union-stencil, 3 tiled, 2 pattern-specialised,
         1 general kernels."""
         v = C.c_int()
         check(lib().omega_model_kernel_variant(self._h, C.byref(v)))
@@ -515,6 +519,24 @@ class ShallowWaterModel:
         if getattr(self, "_h", None) is not None and _lib is not None:
             _lib.omega_model_destroy(self._h)
             self._h = None
+
+
+class LinearShallowWaterModel(ShallowWaterModel):
+    """LinearShallowWaterModel(mesh, params, h0)
+    (src/shallow_water_hierarchical.cpp:676-773): the equations linearised
+    about a resting layer of thickness h0, h_t = -h0 div(vn),
+    vn_t = f0 * (TRiSK tangential velocity) - g grad(h); no tracers
+    (params.ntracers must be 0).  Steppers, hooks, increment mode and the
+    integrals work as for ShallowWaterModel; the adjoint and ensembles
+    reject it."""
+
+    def __init__(self, mesh: MPASMesh, params: ShallowWaterParams, h0: float):
+        self.h0 = float(h0)
+        super().__init__(mesh, params)
+
+    def _create(self, h):
+        check(lib().omega_model_create_linear(self.mesh._h, C.byref(self._cparams), self.h0,
+                                              C.byref(h)))
 
 
 class _Stepper:

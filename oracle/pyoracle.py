@@ -71,6 +71,7 @@ def lib():
         L.orc_model_create.restype = vp
         L.orc_model_create.argtypes = [vp, C.POINTER(Params)]
         L.orc_model_free.argtypes = [vp]
+        L.orc_model_set_linear.argtypes = [vp, C.c_double]
         L.orc_model_array.restype = vp
         L.orc_model_array.argtypes = [vp, C.c_char_p, ip, lp, lp]
         L.orc_model_set_additional.argtypes = [vp, ADDFN, vp]
@@ -197,13 +198,18 @@ class Model:
 
     def __init__(self, mesh: Mesh, f0=0.0, grav=9.81, drag_coeff=0.0, visc_del2=0.0,
                  visc_del4=0.0, eddy_diff2=0.0, eddy_diff4=0.0, ntracers=0,
-                 disable_h_tendency=False, disable_vn_tendency=False):
+                 disable_h_tendency=False, disable_vn_tendency=False, linear_h0=None):
         self.mesh = mesh
         self.params = Params(f0, grav, drag_coeff, visc_del2, visc_del4, eddy_diff2, eddy_diff4,
                              int(ntracers), int(disable_h_tendency), int(disable_vn_tendency))
         self._h = lib().orc_model_create(mesh._h, C.byref(self.params))
         self._cb = None
         self.ntracers = int(ntracers)
+        if linear_h0 is not None:
+            # LinearShallowWaterModel (src/shallow_water_hierarchical.cpp:676-773)
+            if self.ntracers:
+                raise ValueError("the linear model carries no tracers")
+            lib().orc_model_set_linear(self._h, float(linear_h0))
 
     def __getattr__(self, name):
         if name.startswith("_"):

@@ -29,6 +29,8 @@
 //  3 del2 curl half divided by dc instead of dv       4 tracer edge value without 1/2
 //  5 RK4 weights b0 and b1 swapped                    6 LSRK5 a2 sign flipped
 //  7 kinetic energy without 1/4                       8 vertex circulation sign flipped
+//  9 linear model: SSH gradient sign                 10 linear model: h0 dropped from div
+// 11 linear model: energy weighs K with h, not h0
 // 0 (the default) is the restatement itself.
 #ifndef ORC_MUTANT
 #define ORC_MUTANT 0
@@ -651,10 +653,65 @@ void Model::compute_tr_tendency(Real *tr_tend, const Real *tr, const Real *vn,
   }
 }
 
-// src/model/shallow_water_model.cpp:313-347
+// LinearShallowWaterModel::compute_h_tendency
+// (src/shallow_water_hierarchical.cpp:680-707): -h0 div(vn)
+void Model::linear_h_tendency(Real *h_tend, const Real *vn, AddMode mode) const {
+  const Mesh &M = *mesh;
+  const Int L = M.nlayers, ME = Mesh::maxedges;
+  ORC_PARALLEL_FOR
+  for (Int ic = 0; ic < M.ncells; ++ic) {
+    for (Int k = 0; k < L; ++k) {
+      Real accum = 0;
+      for (Int j = 0; j < M.nedges_on_cell[ic]; ++j) {
+        const Int jedge = M.edges_on_cell[ic * ME + j];
+        accum += M.dv_edge[jedge] * M.edge_sign_on_cell[ic * ME + j] * vn[(size_t)jedge * L + k];
+      }
+      const Real x = (mutant == 10 ? -1.0 : -h0) * accum / M.area_cell[ic];
+      Real &t = h_tend[(size_t)ic * L + k];
+      if (mode == increment) t += x;
+      else t = x;
+    }
+  }
+}
+
+// LinearShallowWaterModel::compute_vn_tendency
+// (src/shallow_water_hierarchical.cpp:709-741): f_edge * (TRiSK tangential
+// velocity) - g grad(h)
+void Model::linear_vn_tendency(Real *vn_tend, const Real *h, const Real *vn,
+                               AddMode mode) const {
+  const Mesh &M = *mesh;
+  const Int L = M.nlayers, ME2 = 2 * Mesh::maxedges;
+  const Real grav = params.grav;
+  ORC_PARALLEL_FOR
+  for (Int ie = 0; ie < M.nedges; ++ie) {
+    for (Int k = 0; k < L; ++k) {
+      Real vt = 0;
+      for (Int j = 0; j < M.nedges_on_edge[ie]; ++j) {
+        const Int jedge = M.edges_on_edge[ie * ME2 + j];
+        vt += M.weights_on_edge[ie * ME2 + j] * vn[(size_t)jedge * L + k];
+      }
+      const Int ic0 = M.cells_on_edge[2 * ie], ic1 = M.cells_on_edge[2 * ie + 1];
+      const Real grad_h = (h[(size_t)ic1 * L + k] - h[(size_t)ic0 * L + k]) / M.dc_edge[ie];
+      const Real x = mutant == 9 ? f_edge[ie] * vt + grav * grad_h : f_edge[ie] * vt - grav * grad_h;
+      Real &t = vn_tend[(size_t)ie * L + k];
+      if (mode == increment) t += x;
+      else t = x;
+    }
+  }
+}
+
+// src/model/shallow_water_model.cpp:313-347; the linear model keeps the
+// base class's sequence with no auxiliary variables and no tracers
+// (src/shallow_water_hierarchical.cpp:20-56)
 void Model::compute_tendency(Real *h_tend, Real *vn_tend, Real *tr_tend,
                              const Real *h, const Real *vn, const Real *tr,
                              Real t, AddMode mode) {
+  if (linear) {
+    if (!params.disable_h_tendency) linear_h_tendency(h_tend, vn, mode);
+    if (!params.disable_vn_tendency) linear_vn_tendency(vn_tend, h, vn, mode);
+    if (additional) additional(additional_user, h_tend, vn_tend, h, vn, t);
+    return;
+  }
   aux_compute(h, vn, tr);
   if (!params.disable_h_tendency) compute_h_tendency(h_tend, h, vn, mode);
   if (!params.disable_vn_tendency) compute_vn_tendency(vn_tend, h, vn, mode);
@@ -787,7 +844,8 @@ Real energy_integral(const Model &m, const Real *h, const Real *vn) {
       }
       K /= M.area_cell[ic];
       const Real hc = h[(size_t)ic * L + k];
-      total += M.area_cell[ic] * (grav * hc * hc / 2 + hc * K);
+      // the linear model weighs K with h0 (shallow_water_hierarchical.cpp:767-769)
+      total += M.area_cell[ic] * (grav * hc * hc / 2 + (m.linear && mutant != 11 ? m.h0 : hc) * K);
     }
   }
   return total;
@@ -912,6 +970,13 @@ void *orc_mesh_alloc(int ncells, int nedges, int nvertices, int nlayers) {
 
 void *orc_model_create(void *mesh, const Params *p) {
   return new Model(static_cast<Mesh *>(mesh), *p);
+}
+// Turns a model into LinearShallowWaterModel with mean thickness h0
+// (shallow_water_hierarchical.cpp:676-678).
+void orc_model_set_linear(void *mp, double h0) {
+  Model *m = static_cast<Model *>(mp);
+  m->linear = true;
+  m->h0 = h0;
 }
 void orc_model_free(void *m) { delete static_cast<Model *>(m); }
 void *orc_model_array(void *mp, const char *name, int *dtype, long *n0, long *n1) {

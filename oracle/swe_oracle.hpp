@@ -99,6 +99,11 @@ struct Model {
   AdditionalFn additional = nullptr;
   void *additional_user = nullptr;
 
+  // LinearShallowWaterModel (src/shallow_water_hierarchical.cpp:676-773):
+  // linearised about a resting layer of thickness h0, no tracers
+  bool linear = false;
+  Real h0 = 0;
+
   Model(const Mesh *m, const Params &p);
   void aux_compute(const Real *h, const Real *vn, const Real *tr);
   void compute_h_tendency(Real *h_tend, const Real *h, const Real *vn,
@@ -110,6 +115,9 @@ struct Model {
   void compute_tendency(Real *h_tend, Real *vn_tend, Real *tr_tend,
                         const Real *h, const Real *vn, const Real *tr, Real t,
                         AddMode mode);
+  void linear_h_tendency(Real *h_tend, const Real *vn, AddMode mode) const;
+  void linear_vn_tendency(Real *vn_tend, const Real *h, const Real *vn,
+                          AddMode mode) const;
 };
 
 struct Stepper {

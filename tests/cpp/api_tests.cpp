@@ -4,6 +4,8 @@
 //   api_tests adjoint            adjoint_run == Tape forward + misfit + reverse
 //   api_tests ensemble           Ensemble steps == each member stepped alone
 //   api_tests ensemble_adjoint   Ensemble::adjoint_run == per-member adjoint_run
+//   api_tests linear             LinearShallowWaterModel: RK4 keeps mass and
+//                                (to its own O(dt^5) dissipation) energy
 //
 // Results are compared bit for bit (a gradient of the shared coefficients
 // to 1e-12, since it sums members in another order).  Exit status 0 on
@@ -161,6 +163,30 @@ void ensemble_adjoint() {
             "shared coefficient gradient");
 }
 
+void linear() {
+  auto mesh = std::make_unique<PlanarHexagonalMesh>(16, 16, 10e3, 4);
+  LinearShallowWaterParams p;
+  p.m_f0 = 1e-4;
+  p.m_grav = 9.81;
+  p.m_h0 = 1000;
+  LinearShallowWaterModel model(mesh.get(), p);
+  ShallowWaterState s(mesh.get(), p), rest(mesh.get(), p);
+  fill(*mesh, s, 0.5);
+  for (Int c = 0; c < mesh->m_ncells; ++c)
+    for (Int k = 0; k < mesh->m_nlayers; ++k) rest.m_h_cell(c, k) = p.m_h0;
+  rest.copy_to_device();
+  const Real e0 = energy_integral(s, model), m0 = mass_integral(s, model);
+  const Real er = energy_integral(rest, model);
+  RK4Stepper st(model);
+  st.enable_graph();
+  for (Int n = 0; n < 100; ++n) st.do_step(n * 2.0, 2.0, s);
+  const Real e1 = energy_integral(s, model), m1 = mass_integral(s, model);
+  std::cout << "linear energy change " << (e1 - e0) / (e0 - er) << " mass change "
+            << (m1 - m0) / m0 << std::endl;
+  require(std::abs(e1 - e0) <= 1e-8 * std::abs(e0 - er), "linear energy conserved");
+  require(std::abs(m1 - m0) <= 1e-13 * std::abs(m0), "linear mass conserved");
+}
+
 } // namespace
 
 int main(int argc, char **argv) {
@@ -169,8 +195,9 @@ int main(int argc, char **argv) {
     if (what == "adjoint") adjoint();
     else if (what == "ensemble") ensemble();
     else if (what == "ensemble_adjoint") ensemble_adjoint();
+    else if (what == "linear") linear();
     else {
-      std::cerr << "usage: api_tests adjoint|ensemble|ensemble_adjoint" << std::endl;
+      std::cerr << "usage: api_tests adjoint|ensemble|ensemble_adjoint|linear" << std::endl;
       return 2;
     }
   } catch (const std::exception &e) {

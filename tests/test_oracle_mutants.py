@@ -1,8 +1,8 @@
 """Mutation check of the oracle's pinning: each build of the oracle with a
-planted plausible mistake (swe_oracle.cpp, ORC_MUTANT = 1..8: a dropped
+planted plausible mistake (swe_oracle.cpp, ORC_MUTANT = 1..11: a dropped
 term, a flipped sign, a swapped operand or coefficient) must fail at least
-one of the pinning / reference tests.  The unmodified oracle passes them all
-(test_oracle_pinning.py, test_oracle_reference.py)."""
+one of the pinning tests.  The unmodified oracle passes them all
+(test_oracle_pinning.py, test_oracle_reference.py, test_linear_model.py)."""
 import os
 import subprocess
 import sys
@@ -22,6 +22,9 @@ EXPECTED = {
     6: "test_lsrk5_fourth_order",                   # LSRK5 coefficient sign
     7: "test_uniform_flow_closed_forms",            # KE without 1/4
     8: "test_momentum_del2_summation_by_parts",     # vertex circulation sign
+    9: ("test_linear_model.py", "test_linear_energy_is_conserved"),     # linear SSH sign
+    10: ("test_linear_model.py", "test_linear_equals_nonlinear_at_rest"),  # h0 dropped
+    11: ("test_linear_model.py", "test_linear_energy_uses_h0"),         # energy with h
 }
 
 
@@ -34,8 +37,10 @@ def mutants():
 @pytest.mark.parametrize("n", sorted(EXPECTED))
 def test_planted_mistake_is_caught(mutants, n):
     env = dict(os.environ, ORACLE_LIB=mutants[n])
+    fname, k = EXPECTED[n] if isinstance(EXPECTED[n], tuple) else ("test_oracle_pinning.py",
+                                                                  EXPECTED[n])
     r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-p", "no:cacheprovider",
-                        os.path.join(ROOT, "tests", "test_oracle_pinning.py"),
-                        "-k", EXPECTED[n]], cwd=ROOT, env=env, capture_output=True, text=True)
+                        os.path.join(ROOT, "tests", fname), "-k", k],
+                       cwd=ROOT, env=env, capture_output=True, text=True)
     assert r.returncode == 1, f"mutant {n} not caught by {EXPECTED[n]}:\n{r.stdout[-2000:]}"
     assert "failed" in r.stdout
