@@ -258,6 +258,14 @@ int omega_model_get_launch(omega_model_t model, int *rows_per_block, int *levels
  * counterpart (performance control only; all give the same bits). */
 int omega_model_set_variant(omega_model_t model, int variant);
 int omega_model_kernel_variant(omega_model_t model, int *in_use);
+/* Forms of the two-cells-per-wavefront kernels this model runs: bit 0 set
+ * when every coefficient table (mesh and model) is bit-identical at each
+ * stencil slot, so a wavefront's two cells share one scalar read of each;
+ * bit 1 set when all [rows][levels] arrays have < 2^24 rows and < 4 GiB,
+ * so row addresses are 32-bit offsets from the array base.  No reference
+ * counterpart (introspection for tests and tuning; results are the same
+ * bits either way). */
+int omega_model_kernel_flags(omega_model_t model, int *flags);
 /* Tiled kernels: cells per tile (1..256, default 32) and levels per block
  * (8, 16 or 32, default 8; the general tiled kernels use 8 or 16 and take
  * 32 as 16, the hexagonal tiled kernel uses 16 or 32 and takes 8 as 16);

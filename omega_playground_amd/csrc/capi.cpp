@@ -493,6 +493,7 @@ int omega_model_set_f_vertex(omega_model_t M, const double *f) {
     std::vector<Real> tmp(dm.host.nvertices);
     for (int i = 0; i < dm.host.nvertices; ++i) tmp[i] = f[dm.vert_n2o[i]];
     M->f_v.upload(tmp);
+    M->f_uniform = rows_equal(tmp, 1);
   });
 }
 
@@ -610,6 +611,15 @@ int omega_model_kernel_variant(omega_model_t M, int *in_use) {
               : hexu_usable(mv, M->lc)                    ? 4
               : hex_usable(mv, M->lc)  ? 2
                                        : 1;
+  });
+}
+
+int omega_model_kernel_flags(omega_model_t M, int *flags) {
+  return guarded([&] {
+    OMEGA_REQUIRE(M && flags, "null argument");
+    const MeshView mv = M->mesh->dm.view();
+    const ModelCoef mc = M->coef();
+    *flags = (mv.coef_uni && mc.uniform ? 1 : 0) | (mv.off32 ? 2 : 0);
   });
 }
 

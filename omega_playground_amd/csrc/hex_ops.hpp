@@ -19,9 +19,16 @@ template <int N> __device__ __forceinline__ int ent(int x) {
   else return x;
 }
 
-template <int N>
+// Row e of a [rows][L] array at levels k..k+N-1.  W = false when every
+// array the kernel reads has fewer than 2^24 rows and is below 4 GiB
+// (MeshView::off32, checked on the host): the address is then the array's
+// uniform base plus one 32-bit lane offset (one 24-bit multiply-add instead
+// of 64-bit index arithmetic).
+template <int N, bool W = true>
 __device__ __forceinline__ Vd<N> rowv(const double *__restrict__ p, int e, int L, int k) {
-  return ld<N>(p + (size_t)e * L + k);
+  if (W) return ld<N>(p + (size_t)e * L + k);
+  const unsigned off = __umul24((unsigned)e, (unsigned)L * 8u) + (unsigned)k * 8u;
+  return ld<N>(reinterpret_cast<const double *>(reinterpret_cast<const char *>(p) + off));
 }
 
 // ---------------------------------------------------------------------------
@@ -378,7 +385,16 @@ __device__ __forceinline__ T pick2(const T *p, long iA, long iB, bool hi) {
   return hi ? b : a;
 }
 
-template <int K>
+// A coefficient of the pair's two entities.  U: the coefficient tables are
+// the same for every cell of the mesh at each stencil slot (checked bit for
+// bit on the host, ModelCoef::uniform), so A's entry is B's as well.
+template <bool U, class T>
+__device__ __forceinline__ T cpick(const T *p, long iA, long iB, bool hi) {
+  if (U) return cld(p, iA);
+  return pick2(p, iA, iB, hi);
+}
+
+template <int K, bool U, bool W>
 __device__ __forceinline__ void edge_tend_hex2(const MeshView &m, const ModelCoef &mc,
                                                const StageArgs &a, const DiagOut &d, int eA,
                                                int eB, bool hi, int k) {
@@ -410,29 +426,29 @@ __device__ __forceinline__ void edge_tend_hex2(const MeshView &m, const ModelCoe
   const bool pv = a.terms & T_PV;
   Vd<N> hU[10], rvU[10], itU[10], vJ[10];
 #pragma unroll
-  for (int u = 0; u < 10; ++u) hU[u] = rowv<N>(h, uc[u], L, k);
+  for (int u = 0; u < 10; ++u) hU[u] = rowv<N, W>(h, uc[u], L, k);
   if (pv) {
 #pragma unroll
-    for (int j = 0; j < 10; ++j) vJ[j] = rowv<N>(vn, ej[j], L, k);
+    for (int j = 0; j < 10; ++j) vJ[j] = rowv<N, W>(vn, ej[j], L, k);
 #pragma unroll
     for (int u = 0; u < 10; ++u) {
-      rvU[u] = rowv<N>(d.rvort_v, uv[u], L, k);
-      itU[u] = rowv<N>(d.inv_thick_v, uv[u], L, k);
+      rvU[u] = rowv<N, W>(d.rvort_v, uv[u], L, k);
+      itU[u] = rowv<N, W>(d.inv_thick_v, uv[u], L, k);
     }
   } else if (a.terms & T_VDEL2) {
-    rvU[o0] = rowv<N>(d.rvort_v, uv[o0], L, k);
-    rvU[o1] = rowv<N>(d.rvort_v, uv[o1], L, k);
+    rvU[o0] = rowv<N, W>(d.rvort_v, uv[o0], L, k);
+    rvU[o1] = rowv<N, W>(d.rvort_v, uv[o1], L, k);
   }
   const int c0 = uc[q0], c1 = uc[q1];
   Vd<N> T = tend_base<N>(a, a.vn, idx);
-  const double idc = pick2(m.inv_dc, eA, eB, hi);
+  const double idc = cpick<U>(m.inv_dc, eA, eB, hi);
   if (pv) {
     Vd<N> nrvU[10], npvU[10];
 #pragma unroll
     for (int u = 0; u < 10; ++u) {
       const double fa = cld(mc.f_v, uvA[u]);
       const double fb = cld(mc.f_v, uvB[u]);
-      const double f = hi ? fb : fa;
+      const double f = U ? fa : (hi ? fb : fa);
 #pragma unroll
       for (int i = 0; i < N; ++i) {
         nrvU[u][i] = itU[u][i] * rvU[u][i];
@@ -441,7 +457,7 @@ __device__ __forceinline__ void edge_tend_hex2(const MeshView &m, const ModelCoe
     }
     double w[10];
 #pragma unroll
-    for (int j = 0; j < 10; ++j) w[j] = pick2(m.w, (long)eA * 10 + j, (long)eB * 10 + j, hi);
+    for (int j = 0; j < 10; ++j) w[j] = cpick<U>(m.w, (long)eA * 10 + j, (long)eB * 10 + j, hi);
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       const double qs = 0.5 * (nrvU[o0][i] + nrvU[o1][i]) + 0.5 * (npvU[o0][i] + npvU[o1][i]);
@@ -458,7 +474,7 @@ __device__ __forceinline__ void edge_tend_hex2(const MeshView &m, const ModelCoe
     }
   }
   if (a.terms & T_KE) {
-    const Vd<N> k0 = rowv<N>(d.ke_c, c0, L, k), k1 = rowv<N>(d.ke_c, c1, L, k);
+    const Vd<N> k0 = rowv<N, W>(d.ke_c, c0, L, k), k1 = rowv<N, W>(d.ke_c, c1, L, k);
 #pragma unroll
     for (int i = 0; i < N; ++i) T[i] = T[i] - (k1[i] - k0[i]) * idc;
   }
@@ -467,11 +483,11 @@ __device__ __forceinline__ void edge_tend_hex2(const MeshView &m, const ModelCoe
     for (int i = 0; i < N; ++i) T[i] = T[i] - mc.grav * (hU[q1][i] - hU[q0][i]) * idc;
   }
   if (a.terms & (T_VDEL2 | T_VDEL4)) {
-    const int ml = pick2(m.maxlev_e, eA, eB, hi);
-    const double idv = pick2(m.inv_dv, eA, eB, hi);
+    const int ml = cpick<U>(m.maxlev_e, eA, eB, hi);
+    const double idv = cpick<U>(m.inv_dv, eA, eB, hi);
     if (a.terms & T_VDEL2) {
-      const Vd<N> d0 = rowv<N>(d.div_c, c0, L, k), d1 = rowv<N>(d.div_c, c1, L, k);
-      const double cv = pick2(mc.cv2, eA, eB, hi);
+      const Vd<N> d0 = rowv<N, W>(d.div_c, c0, L, k), d1 = rowv<N, W>(d.div_c, c1, L, k);
+      const double cv = cpick<U>(mc.cv2, eA, eB, hi);
 #pragma unroll
       for (int i = 0; i < N; ++i) {
         const double del2u = (d1[i] - d0[i]) * idc - (rvU[o1][i] - rvU[o0][i]) * idv;
@@ -480,9 +496,9 @@ __device__ __forceinline__ void edge_tend_hex2(const MeshView &m, const ModelCoe
     }
     if (a.terms & T_VDEL4) {
       const int b0 = uv[o0], b1 = uv[o1];
-      const Vd<N> d0 = rowv<N>(d.vdel2_div_c, c0, L, k), d1 = rowv<N>(d.vdel2_div_c, c1, L, k);
-      const Vd<N> r0 = rowv<N>(d.vdel2_rv_v, b0, L, k), r1 = rowv<N>(d.vdel2_rv_v, b1, L, k);
-      const double cv = pick2(mc.cv4, eA, eB, hi);
+      const Vd<N> d0 = rowv<N, W>(d.vdel2_div_c, c0, L, k), d1 = rowv<N, W>(d.vdel2_div_c, c1, L, k);
+      const Vd<N> r0 = rowv<N, W>(d.vdel2_rv_v, b0, L, k), r1 = rowv<N, W>(d.vdel2_rv_v, b1, L, k);
+      const double cv = cpick<U>(mc.cv4, eA, eB, hi);
 #pragma unroll
       for (int i = 0; i < N; ++i) {
         const double del2u = (d1[i] - d0[i]) * idc - (r1[i] - r0[i]) * idv;
@@ -495,6 +511,7 @@ __device__ __forceinline__ void edge_tend_hex2(const MeshView &m, const ModelCoe
 
 // Pair forms of vertex_diag_u / cell_diag_u (N = 2, two cells per
 // wavefront, tables through the scalar cache); same expressions.
+template <bool U, bool W>
 __device__ __forceinline__ void vertex_diag_p(const MeshView &m, int vA, int vB, bool hi, int k,
                                               const double *__restrict__ h,
                                               const double *__restrict__ vn, const DiagOut &d) {
@@ -507,14 +524,14 @@ __device__ __forceinline__ void vertex_diag_p(const MeshView &m, int vA, int vB,
   for (int j = 0; j < 3; ++j) {
     e[j] = pick2(m.eov, 3L * vA + j, 3L * vB + j, hi);
     c[j] = pick2(m.cov, 3L * vA + j, 3L * vB + j, hi);
-    crv[j] = pick2(m.crv, 3L * vA + j, 3L * vB + j, hi);
-    cth[j] = pick2(m.cth, 3L * vA + j, 3L * vB + j, hi);
+    crv[j] = cpick<U>(m.crv, 3L * vA + j, 3L * vB + j, hi);
+    cth[j] = cpick<U>(m.cth, 3L * vA + j, 3L * vB + j, hi);
   }
   Vd<N> ve[3], hc[3];
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
-    ve[j] = rowv<N>(vn, e[j], L, k);
-    hc[j] = rowv<N>(h, c[j], L, k);
+    ve[j] = rowv<N, W>(vn, e[j], L, k);
+    hc[j] = rowv<N, W>(h, c[j], L, k);
   }
   Vd<N> rv = zero<N>(), th = zero<N>();
 #pragma unroll
@@ -531,6 +548,7 @@ __device__ __forceinline__ void vertex_diag_p(const MeshView &m, int vA, int vB,
   st<N>(d.inv_thick_v + (size_t)v * L + k, inv);
 }
 
+template <bool U, bool W>
 __device__ __forceinline__ void cell_diag_p(const MeshView &m, int terms, int ntr, int cA, int cB,
                                             bool hi, int k, const double *__restrict__ h,
                                             const double *__restrict__ vn,
@@ -548,12 +566,12 @@ __device__ __forceinline__ void cell_diag_p(const MeshView &m, int terms, int nt
   if (terms & (T_KE | T_VDEL2 | T_VDEL4)) {
     Vd<N> ve[6];
 #pragma unroll
-    for (int j = 0; j < 6; ++j) ve[j] = rowv<N>(vn, E[j], L, k);
+    for (int j = 0; j < 6; ++j) ve[j] = rowv<N, W>(vn, E[j], L, k);
     Vd<N> ke = zero<N>(), div = zero<N>();
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
-      const double cke = pick2(m.cke, (long)cA * me + j, (long)cB * me + j, hi);
-      const double cd = pick2(m.cdiv, (long)cA * me + j, (long)cB * me + j, hi);
+      const double cke = cpick<U>(m.cke, (long)cA * me + j, (long)cB * me + j, hi);
+      const double cd = cpick<U>(m.cdiv, (long)cA * me + j, (long)cB * me + j, hi);
 #pragma unroll
       for (int i = 0; i < N; ++i) {
         ke[i] = ke[i] + cke * ve[j][i] * ve[j][i] * 0.25;
@@ -567,17 +585,17 @@ __device__ __forceinline__ void cell_diag_p(const MeshView &m, int terms, int nt
     int NB[6];
 #pragma unroll
     for (int j = 0; j < 6; ++j) NB[j] = pick2(m.coc, (long)cA * 6 + j, (long)cB * 6 + j, hi);
-    const Vd<N> hC = rowv<N>(h, c, L, k);
+    const Vd<N> hC = rowv<N, W>(h, c, L, k);
     Vd<N> hN[6], invN[6], cm[6], invC;
     double idc[6];
 #pragma unroll
-    for (int j = 0; j < 6; ++j) hN[j] = rowv<N>(h, NB[j], L, k);
+    for (int j = 0; j < 6; ++j) hN[j] = rowv<N, W>(h, NB[j], L, k);
 #pragma unroll
     for (int i = 0; i < N; ++i) invC[i] = 1.0 / hC[i];
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
-      const double cd = pick2(m.cdiv, (long)cA * me + j, (long)cB * me + j, hi);
-      idc[j] = pick2(m.inv_dc, EA[j], EB[j], hi);
+      const double cd = cpick<U>(m.cdiv, (long)cA * me + j, (long)cB * me + j, hi);
+      idc[j] = cpick<U>(m.inv_dc, EA[j], EB[j], hi);
 #pragma unroll
       for (int i = 0; i < N; ++i) {
         invN[j][i] = 1.0 / hN[j][i];
@@ -586,10 +604,10 @@ __device__ __forceinline__ void cell_diag_p(const MeshView &m, int terms, int nt
     }
     for (int l = 0; l < ntr; ++l) {
       const double *trl = tr + (size_t)l * m.nc * L;
-      const Vd<N> tC = rowv<N>(trl, c, L, k);
+      const Vd<N> tC = rowv<N, W>(trl, c, L, k);
       Vd<N> tN[6];
 #pragma unroll
-      for (int j = 0; j < 6; ++j) tN[j] = rowv<N>(trl, NB[j], L, k);
+      for (int j = 0; j < 6; ++j) tN[j] = rowv<N, W>(trl, NB[j], L, k);
       Vd<N> d2 = zero<N>();
 #pragma unroll
       for (int j = 0; j < 6; ++j)
@@ -610,6 +628,7 @@ __device__ __forceinline__ void cell_diag_p(const MeshView &m, int terms, int nt
 // edges of its owned vertices are loaded once and shared by the vertex,
 // kinetic-energy/divergence and tracer-Laplacian parts (hexd maps).  Same
 // expressions, same bits.
+template <bool U, bool W>
 __device__ __forceinline__ void diag_union_p(const MeshView &m, int terms, int ntr, int cA,
                                              int cB, bool hi, int k, const double *__restrict__ h,
                                              const double *__restrict__ vn,
@@ -634,16 +653,16 @@ __device__ __forceinline__ void diag_union_p(const MeshView &m, int terms, int n
   Vd<N> ve[8], hR[7];
   if (vert || kediv) {
 #pragma unroll
-    for (int j = 0; j < 6; ++j) ve[j] = rowv<N>(vn, E[j], L, k);
+    for (int j = 0; j < 6; ++j) ve[j] = rowv<N, W>(vn, E[j], L, k);
   }
   if (vert) {
-    ve[6] = rowv<N>(vn, E[6], L, k);
-    ve[7] = rowv<N>(vn, E[7], L, k);
+    ve[6] = rowv<N, W>(vn, E[6], L, k);
+    ve[7] = rowv<N, W>(vn, E[7], L, k);
   }
   if (vert || del2) {
-    hR[0] = rowv<N>(h, c, L, k);
+    hR[0] = rowv<N, W>(h, c, L, k);
 #pragma unroll
-    for (int j = 0; j < 6; ++j) hR[1 + j] = rowv<N>(h, NB[j], L, k);
+    for (int j = 0; j < 6; ++j) hR[1 + j] = rowv<N, W>(h, NB[j], L, k);
   }
   if (vert) {
     // VorticityAuxVars::compute_vort_vertex (vorticity_aux.hpp:85-116)
@@ -654,8 +673,8 @@ __device__ __forceinline__ void diag_union_p(const MeshView &m, int terms, int n
       double crv[3], cth[3];
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
-        crv[j] = pick2(m.crv, 3L * vA + j, 3L * vB + j, hi);
-        cth[j] = pick2(m.cth, 3L * vA + j, 3L * vB + j, hi);
+        crv[j] = cpick<U>(m.crv, 3L * vA + j, 3L * vB + j, hi);
+        cth[j] = cpick<U>(m.cth, 3L * vA + j, 3L * vB + j, hi);
       }
       Vd<N> rv = zero<N>(), th = zero<N>();
 #pragma unroll
@@ -677,8 +696,8 @@ __device__ __forceinline__ void diag_union_p(const MeshView &m, int terms, int n
     Vd<N> ke = zero<N>(), div = zero<N>();
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
-      const double cke = pick2(m.cke, (long)cA * me + j, (long)cB * me + j, hi);
-      const double cd = pick2(m.cdiv, (long)cA * me + j, (long)cB * me + j, hi);
+      const double cke = cpick<U>(m.cke, (long)cA * me + j, (long)cB * me + j, hi);
+      const double cd = cpick<U>(m.cdiv, (long)cA * me + j, (long)cB * me + j, hi);
 #pragma unroll
       for (int q = 0; q < N; ++q) {
         ke[q] = ke[q] + cke * ve[j][q] * ve[j][q] * 0.25;
@@ -697,8 +716,8 @@ __device__ __forceinline__ void diag_union_p(const MeshView &m, int terms, int n
     for (int q = 0; q < N; ++q) invC[q] = 1.0 / hR[0][q];
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
-      const double cd = pick2(m.cdiv, (long)cA * me + j, (long)cB * me + j, hi);
-      idc[j] = pick2(m.inv_dc, EA[j], EB[j], hi);
+      const double cd = cpick<U>(m.cdiv, (long)cA * me + j, (long)cB * me + j, hi);
+      idc[j] = cpick<U>(m.inv_dc, EA[j], EB[j], hi);
 #pragma unroll
       for (int q = 0; q < N; ++q) {
         invN[j][q] = 1.0 / hR[1 + j][q];
@@ -707,10 +726,10 @@ __device__ __forceinline__ void diag_union_p(const MeshView &m, int terms, int n
     }
     for (int l = 0; l < ntr; ++l) {
       const double *trl = tr + (size_t)l * m.nc * L;
-      const Vd<N> tC = rowv<N>(trl, c, L, k);
+      const Vd<N> tC = rowv<N, W>(trl, c, L, k);
       Vd<N> tN[6];
 #pragma unroll
-      for (int j = 0; j < 6; ++j) tN[j] = rowv<N>(trl, NB[j], L, k);
+      for (int j = 0; j < 6; ++j) tN[j] = rowv<N, W>(trl, NB[j], L, k);
       Vd<N> d2 = zero<N>();
 #pragma unroll
       for (int j = 0; j < 6; ++j)
@@ -730,7 +749,7 @@ __device__ __forceinline__ void diag_union_p(const MeshView &m, int terms, int n
 // union vertices (hex_pattern.hpp, namespace hexu) are formed once per cell
 // by the caller and shared by its three edges; the rows per edge are then
 // 10 thicknesses and 10 velocities.  Same values, same expressions.
-template <int K>
+template <int K, bool U, bool W>
 __device__ __forceinline__ void edge_tend_hex2v(const MeshView &m, const ModelCoef &mc,
                                                 const StageArgs &a, const DiagOut &d, int eA,
                                                 int eB, bool hi, int k, const Vd<2> (&P)[hexu::kNV],
@@ -761,18 +780,18 @@ __device__ __forceinline__ void edge_tend_hex2v(const MeshView &m, const ModelCo
   const bool pv = a.terms & T_PV;
   Vd<N> hU[10], vJ[10];
 #pragma unroll
-  for (int u = 0; u < 10; ++u) hU[u] = rowv<N>(h, uc[u], L, k);
+  for (int u = 0; u < 10; ++u) hU[u] = rowv<N, W>(h, uc[u], L, k);
   if (pv) {
 #pragma unroll
-    for (int j = 0; j < 10; ++j) vJ[j] = rowv<N>(vn, ej[j], L, k);
+    for (int j = 0; j < 10; ++j) vJ[j] = rowv<N, W>(vn, ej[j], L, k);
   }
   const int c0 = uc[q0], c1 = uc[q1];
   Vd<N> T = tend_base<N>(a, a.vn, idx);
-  const double idc = pick2(m.inv_dc, eA, eB, hi);
+  const double idc = cpick<U>(m.inv_dc, eA, eB, hi);
   if (pv) {
     double w[10];
 #pragma unroll
-    for (int j = 0; j < 10; ++j) w[j] = pick2(m.w, (long)eA * 10 + j, (long)eB * 10 + j, hi);
+    for (int j = 0; j < 10; ++j) w[j] = cpick<U>(m.w, (long)eA * 10 + j, (long)eB * 10 + j, hi);
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       const double qs = 0.5 * (P[v0][i] + P[v1][i]) + 0.5 * (Q[v0][i] + Q[v1][i]);
@@ -789,7 +808,7 @@ __device__ __forceinline__ void edge_tend_hex2v(const MeshView &m, const ModelCo
     }
   }
   if (a.terms & T_KE) {
-    const Vd<N> k0 = rowv<N>(d.ke_c, c0, L, k), k1 = rowv<N>(d.ke_c, c1, L, k);
+    const Vd<N> k0 = rowv<N, W>(d.ke_c, c0, L, k), k1 = rowv<N, W>(d.ke_c, c1, L, k);
 #pragma unroll
     for (int i = 0; i < N; ++i) T[i] = T[i] - (k1[i] - k0[i]) * idc;
   }
@@ -798,14 +817,14 @@ __device__ __forceinline__ void edge_tend_hex2v(const MeshView &m, const ModelCo
     for (int i = 0; i < N; ++i) T[i] = T[i] - mc.grav * (hU[q1][i] - hU[q0][i]) * idc;
   }
   if (a.terms & (T_VDEL2 | T_VDEL4)) {
-    const int ml = pick2(m.maxlev_e, eA, eB, hi);
-    const double idv = pick2(m.inv_dv, eA, eB, hi);
+    const int ml = cpick<U>(m.maxlev_e, eA, eB, hi);
+    const double idv = cpick<U>(m.inv_dv, eA, eB, hi);
     const int b0 = pick2(rA, 10 + o0, (long)(rB - rA) + 10 + o0, hi);
     const int b1 = pick2(rA, 10 + o1, (long)(rB - rA) + 10 + o1, hi);
     if (a.terms & T_VDEL2) {
-      const Vd<N> d0 = rowv<N>(d.div_c, c0, L, k), d1 = rowv<N>(d.div_c, c1, L, k);
-      const Vd<N> r0 = rowv<N>(d.rvort_v, b0, L, k), r1 = rowv<N>(d.rvort_v, b1, L, k);
-      const double cv = pick2(mc.cv2, eA, eB, hi);
+      const Vd<N> d0 = rowv<N, W>(d.div_c, c0, L, k), d1 = rowv<N, W>(d.div_c, c1, L, k);
+      const Vd<N> r0 = rowv<N, W>(d.rvort_v, b0, L, k), r1 = rowv<N, W>(d.rvort_v, b1, L, k);
+      const double cv = cpick<U>(mc.cv2, eA, eB, hi);
 #pragma unroll
       for (int i = 0; i < N; ++i) {
         const double del2u = (d1[i] - d0[i]) * idc - (r1[i] - r0[i]) * idv;
@@ -813,9 +832,9 @@ __device__ __forceinline__ void edge_tend_hex2v(const MeshView &m, const ModelCo
       }
     }
     if (a.terms & T_VDEL4) {
-      const Vd<N> d0 = rowv<N>(d.vdel2_div_c, c0, L, k), d1 = rowv<N>(d.vdel2_div_c, c1, L, k);
-      const Vd<N> r0 = rowv<N>(d.vdel2_rv_v, b0, L, k), r1 = rowv<N>(d.vdel2_rv_v, b1, L, k);
-      const double cv = pick2(mc.cv4, eA, eB, hi);
+      const Vd<N> d0 = rowv<N, W>(d.vdel2_div_c, c0, L, k), d1 = rowv<N, W>(d.vdel2_div_c, c1, L, k);
+      const Vd<N> r0 = rowv<N, W>(d.vdel2_rv_v, b0, L, k), r1 = rowv<N, W>(d.vdel2_rv_v, b1, L, k);
+      const double cv = cpick<U>(mc.cv4, eA, eB, hi);
 #pragma unroll
       for (int i = 0; i < N; ++i) {
         const double del2u = (d1[i] - d0[i]) * idc - (r1[i] - r0[i]) * idv;
@@ -828,6 +847,7 @@ __device__ __forceinline__ void edge_tend_hex2v(const MeshView &m, const ModelCo
 
 // PV products inv_thick*rel_vort and inv_thick*f of the 16 union vertices
 // of the wavefront's two cells (lane's cell picked per half-wavefront).
+template <bool U, bool W>
 __device__ __forceinline__ void union_pv2(const MeshView &m, const ModelCoef &mc,
                                           const DiagOut &d, int cA, int cB, bool hi, int k,
                                           Vd<2> (&P)[hexu::kNV], Vd<2> (&Q)[hexu::kNV]) {
@@ -839,7 +859,7 @@ __device__ __forceinline__ void union_pv2(const MeshView &m, const ModelCoef &mc
     const int va = cld(uA, x), vb = cld(uB, x);
     const int v = hi ? vb : va;
     const double fa = cld(mc.f_v, va), fb = cld(mc.f_v, vb);
-    const double f = hi ? fb : fa;
+    const double f = U ? fa : (hi ? fb : fa);
     const Vd<2> rv = rowv<2>(d.rvort_v, v, L, k), it = rowv<2>(d.inv_thick_v, v, L, k);
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -849,6 +869,7 @@ __device__ __forceinline__ void union_pv2(const MeshView &m, const ModelCoef &mc
   }
 }
 
+template <bool U, bool W>
 __device__ __forceinline__ void cell_tend_hex2(const MeshView &m, const ModelCoef &mc,
                                                const StageArgs &a, const DiagOut &d, int cA,
                                                int cB, bool hi, int k) {
@@ -865,17 +886,17 @@ __device__ __forceinline__ void cell_tend_hex2(const MeshView &m, const ModelCoe
     E[j] = hi ? EB[j] : EA[j];
     NB[j] = pick2(m.coc, (long)cA * 6 + j, (long)cB * 6 + j, hi);
   }
-  const Vd<N> hC = rowv<N>(h, c, L, k);
+  const Vd<N> hC = rowv<N, W>(h, c, L, k);
   Vd<N> hN[6], vE[6], fh[6];
   double cd[6];
 #pragma unroll
   for (int j = 0; j < 6; ++j) {
-    hN[j] = rowv<N>(h, NB[j], L, k);
-    vE[j] = rowv<N>(vn, E[j], L, k);
+    hN[j] = rowv<N, W>(h, NB[j], L, k);
+    vE[j] = rowv<N, W>(vn, E[j], L, k);
   }
 #pragma unroll
   for (int j = 0; j < 6; ++j) {
-    cd[j] = pick2(m.cdiv, (long)cA * me + j, (long)cB * me + j, hi);
+    cd[j] = cpick<U>(m.cdiv, (long)cA * me + j, (long)cB * me + j, hi);
 #pragma unroll
     for (int i = 0; i < N; ++i) fh[j][i] = 0.5 * (hN[j][i] + hC[i]);
   }
@@ -904,22 +925,22 @@ __device__ __forceinline__ void cell_tend_hex2(const MeshView &m, const ModelCoe
   for (int j = 0; j < 6; ++j) {
 #pragma unroll
     for (int i = 0; i < N; ++i) invN[j][i] = 1.0 / hN[j][i];
-    idc[j] = pick2(m.inv_dc, EA[j], EB[j], hi);
-    c2[j] = td2 ? pick2(mc.ct2, (long)cA * me + j, (long)cB * me + j, hi) : 0.0;
-    s2[j] = td2 ? pick2(m.s2, EA[j], EB[j], hi) : 0.0;
-    c4[j] = td4 ? pick2(mc.ct4, (long)cA * me + j, (long)cB * me + j, hi) : 0.0;
+    idc[j] = cpick<U>(m.inv_dc, EA[j], EB[j], hi);
+    c2[j] = td2 ? cpick<U>(mc.ct2, (long)cA * me + j, (long)cB * me + j, hi) : 0.0;
+    s2[j] = td2 ? cpick<U>(m.s2, EA[j], EB[j], hi) : 0.0;
+    c4[j] = td4 ? cpick<U>(mc.ct4, (long)cA * me + j, (long)cB * me + j, hi) : 0.0;
   }
   for (int l = 0; l < ntr; ++l) {
     const double *trl = a.tr.in + (size_t)l * NC * L;
-    const Vd<N> tC = rowv<N>(trl, c, L, k);
+    const Vd<N> tC = rowv<N, W>(trl, c, L, k);
     Vd<N> tN[6], dN[6], dC = zero<N>();
 #pragma unroll
-    for (int j = 0; j < 6; ++j) tN[j] = rowv<N>(trl, NB[j], L, k);
+    for (int j = 0; j < 6; ++j) tN[j] = rowv<N, W>(trl, NB[j], L, k);
     if (td4) {
       const double *dl = d.trdel2_c + (size_t)l * NC * L;
-      dC = rowv<N>(dl, c, L, k);
+      dC = rowv<N, W>(dl, c, L, k);
 #pragma unroll
-      for (int j = 0; j < 6; ++j) dN[j] = rowv<N>(dl, NB[j], L, k);
+      for (int j = 0; j < 6; ++j) dN[j] = rowv<N, W>(dl, NB[j], L, k);
     }
     const size_t tidx = ((size_t)l * NC + c) * L + k;
     Vd<N> T = tend_base<N>(a, a.tr, tidx);

@@ -120,6 +120,10 @@ struct MeshView {
   const Real *cth;              // kite*inv_area_triangle     [nv][3]
   // regular-hexagon records (valid when hex_ok, see hex_pattern.hpp)
   int hex_ok, hexu_ok, hexd_ok;
+  // coef_uni: the mesh's coefficient tables (inv_dc, inv_dv, maxlev_e, s2,
+  // s4, w, cdiv, cke, crv, cth) are equal at every stencil slot, bit for
+  // bit; off32: every [rows][L] array has < 2^24 rows and < 4 GiB
+  int coef_uni, off32;
   const int *hexrec;            // [ne][30]
   const int *hexu;              // [nc][49] per-cell union stencil (hex_pattern.hpp)
   const int *coc;               // cells_on_cell [nc][6]
@@ -143,6 +147,8 @@ struct DeviceMesh {
   // regular-hexagon records for the pattern-specialised kernels
   bool hex_ok = false;
   bool hexu_ok = false; // per-cell union records valid (hexu)
+  bool coef_uni = false; // coefficient tables equal at every stencil slot
+  bool off32 = false;    // 32-bit byte offsets suffice for every row array
   bool hexd_ok = false; // owned vertices follow the diagnostics maps (hexd)
   std::vector<int> hexrec_h, coc_h, hexu_h;
   DBuf<int> hexrec, coc, hexu;
@@ -212,6 +218,9 @@ struct ModelCoef {
   const Real *cv4;   // visc_del4*mesh_scaling_del4 [ne]
   const Real *ct2;   // ((eddy_diff2*inv_area)*dv)*sign [nc][me]
   const Real *ct4;   // (((eddy_diff4*inv_area)*dv)*sign)*scaling4 [nc][me]
+  // f_v, cv2, cv4, ct2, ct4 equal at every stencil slot (bit for bit) on a
+  // regular hexagonal mesh, so a pair wavefront's two cells share them
+  int uniform;
 };
 
 struct StageArgs {
@@ -224,6 +233,9 @@ struct StageArgs {
   Real db, da;       // dt*b_s, dt*a_s
   FieldIO h, vn, tr;
 };
+
+// x[i*stride + j] == x[j] bit for bit for every row i (mesh_host.cpp)
+template <class T> bool rows_equal(const std::vector<T> &x, size_t stride);
 
 // Kernel launchers (kernels_fwd.hip)
 void launch_diag(const MeshView &m, const Launch &lc, const ModelCoef &mc, int terms, int ntr,

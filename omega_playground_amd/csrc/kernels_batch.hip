@@ -175,14 +175,14 @@ __global__ void __launch_bounds__(256)
   const DiagOut &d = B.d;
   const double *h = B.a.h.in, *vn = B.a.vn.in, *tr = B.a.tr.in;
   if (m.hexd_ok) {
-    hexops::diag_union_p(m, B.dterms, B.a.ntr, q.cA, q.cB, q.hi, q.k, h, vn, tr, d);
+    hexops::diag_union_p<false, true>(m, B.dterms, B.a.ntr, q.cA, q.cB, q.hi, q.k, h, vn, tr, d);
     return;
   }
   if (B.dterms & (T_PV | T_VDEL2 | T_VDEL4)) {
-    hexops::vertex_diag_p(m, 2 * q.cA, 2 * q.cB, q.hi, q.k, h, vn, d);
-    hexops::vertex_diag_p(m, 2 * q.cA + 1, 2 * q.cB + 1, q.hi, q.k, h, vn, d);
+    hexops::vertex_diag_p<false, true>(m, 2 * q.cA, 2 * q.cB, q.hi, q.k, h, vn, d);
+    hexops::vertex_diag_p<false, true>(m, 2 * q.cA + 1, 2 * q.cB + 1, q.hi, q.k, h, vn, d);
   }
-  hexops::cell_diag_p(m, B.dterms, B.a.ntr, q.cA, q.cB, q.hi, q.k, h, vn, tr, d);
+  hexops::cell_diag_p<false, true>(m, B.dterms, B.a.ntr, q.cA, q.cB, q.hi, q.k, h, vn, tr, d);
 }
 
 __global__ void __launch_bounds__(256)
@@ -198,16 +198,16 @@ __global__ void __launch_bounds__(256)
   const int eA = 3 * q.cA, eB = 3 * q.cB;
   if (m.hexu_ok) {
     Vd<2> P[hexu::kNV], Q[hexu::kNV];
-    if (a.do_vn && (a.terms & T_PV)) hexops::union_pv2(m, mc, d, q.cA, q.cB, q.hi, q.k, P, Q);
-    hexops::edge_tend_hex2v<0>(m, mc, a, d, eA, eB, q.hi, q.k, P, Q);
-    hexops::edge_tend_hex2v<1>(m, mc, a, d, eA + 1, eB + 1, q.hi, q.k, P, Q);
-    hexops::edge_tend_hex2v<2>(m, mc, a, d, eA + 2, eB + 2, q.hi, q.k, P, Q);
+    if (a.do_vn && (a.terms & T_PV)) hexops::union_pv2<false, true>(m, mc, d, q.cA, q.cB, q.hi, q.k, P, Q);
+    hexops::edge_tend_hex2v<0, false, true>(m, mc, a, d, eA, eB, q.hi, q.k, P, Q);
+    hexops::edge_tend_hex2v<1, false, true>(m, mc, a, d, eA + 1, eB + 1, q.hi, q.k, P, Q);
+    hexops::edge_tend_hex2v<2, false, true>(m, mc, a, d, eA + 2, eB + 2, q.hi, q.k, P, Q);
   } else {
-    hexops::edge_tend_hex2<0>(m, mc, a, d, eA, eB, q.hi, q.k);
-    hexops::edge_tend_hex2<1>(m, mc, a, d, eA + 1, eB + 1, q.hi, q.k);
-    hexops::edge_tend_hex2<2>(m, mc, a, d, eA + 2, eB + 2, q.hi, q.k);
+    hexops::edge_tend_hex2<0, false, true>(m, mc, a, d, eA, eB, q.hi, q.k);
+    hexops::edge_tend_hex2<1, false, true>(m, mc, a, d, eA + 1, eB + 1, q.hi, q.k);
+    hexops::edge_tend_hex2<2, false, true>(m, mc, a, d, eA + 2, eB + 2, q.hi, q.k);
   }
-  hexops::cell_tend_hex2(m, mc, a, d, q.cA, q.cB, q.hi, q.k);
+  hexops::cell_tend_hex2<false, true>(m, mc, a, d, q.cA, q.cB, q.hi, q.k);
 }
 
 } // namespace

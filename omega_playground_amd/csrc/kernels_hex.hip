@@ -44,7 +44,9 @@ __global__ void __launch_bounds__(256)
   cell_diag_u<N>(m, terms, ntr, c, k, h, vn, tr, d);
 }
 
-// N = 2, two cells per wavefront, tables through the scalar cache
+// N = 2, two cells per wavefront, tables through the scalar cache; U: the
+// mesh's coefficients are the same at each stencil slot, W: 64-bit offsets
+template <bool U, bool W>
 __global__ void __launch_bounds__(256)
     k_diag_hex2(MeshView m, int terms, int ntr, const double *__restrict__ h,
                 const double *__restrict__ vn, const double *__restrict__ tr, DiagOut d, int swz) {
@@ -57,14 +59,14 @@ __global__ void __launch_bounds__(256)
   if (s * R + (int)threadIdx.y >= m.nc) return;
   const int k = (blockIdx.y * blockDim.x + threadIdx.x) * 2;
   if (m.hexd_ok) {
-    diag_union_p(m, terms, ntr, cA, cB, hi, k, h, vn, tr, d);
+    diag_union_p<U, W>(m, terms, ntr, cA, cB, hi, k, h, vn, tr, d);
     return;
   }
   if (terms & (T_PV | T_VDEL2 | T_VDEL4)) {
-    vertex_diag_p(m, 2 * cA, 2 * cB, hi, k, h, vn, d);
-    vertex_diag_p(m, 2 * cA + 1, 2 * cB + 1, hi, k, h, vn, d);
+    vertex_diag_p<U, W>(m, 2 * cA, 2 * cB, hi, k, h, vn, d);
+    vertex_diag_p<U, W>(m, 2 * cA + 1, 2 * cB + 1, hi, k, h, vn, d);
   }
-  cell_diag_p(m, terms, ntr, cA, cB, hi, k, h, vn, tr, d);
+  cell_diag_p<U, W>(m, terms, ntr, cA, cB, hi, k, h, vn, tr, d);
 }
 
 template <int N>
@@ -85,6 +87,7 @@ __global__ void __launch_bounds__(256)
 // N = 2 with two cells per wavefront and their tables read through the
 // scalar path (hex_ops.hpp: edge_tend_hex2 / cell_tend_hex2); the block's
 // row count is even, so the cells of one wavefront are cA and cA + 1
+template <bool U, bool W>
 __global__ void __launch_bounds__(256)
     k_tend_hex2(MeshView m, ModelCoef mc, StageArgs a, DiagOut d, int swz) {
   const int s = logical_block(blockIdx.x, gridDim.x, swz);
@@ -95,15 +98,16 @@ __global__ void __launch_bounds__(256)
   const bool hi = (threadIdx.y & 1) != 0;
   if (s * R + (int)threadIdx.y >= m.nc) return;
   const int k = (blockIdx.y * blockDim.x + threadIdx.x) * 2;
-  edge_tend_hex2<0>(m, mc, a, d, 3 * cA, 3 * cB, hi, k);
-  edge_tend_hex2<1>(m, mc, a, d, 3 * cA + 1, 3 * cB + 1, hi, k);
-  edge_tend_hex2<2>(m, mc, a, d, 3 * cA + 2, 3 * cB + 2, hi, k);
-  cell_tend_hex2(m, mc, a, d, cA, cB, hi, k);
+  edge_tend_hex2<0, U, W>(m, mc, a, d, 3 * cA, 3 * cB, hi, k);
+  edge_tend_hex2<1, U, W>(m, mc, a, d, 3 * cA + 1, 3 * cB + 1, hi, k);
+  edge_tend_hex2<2, U, W>(m, mc, a, d, 3 * cA + 2, 3 * cB + 2, hi, k);
+  cell_tend_hex2<U, W>(m, mc, a, d, cA, cB, hi, k);
 }
 
 // vertex-union form: the PV products of the cell's 16 union vertices are
 // formed once and shared by its three edges (needs the per-cell union
 // records, hexu)
+template <bool U, bool W>
 __global__ void __launch_bounds__(256)
     k_tend_hex2v(MeshView m, ModelCoef mc, StageArgs a, DiagOut d, int swz) {
   const int s = logical_block(blockIdx.x, gridDim.x, swz);
@@ -115,11 +119,11 @@ __global__ void __launch_bounds__(256)
   if (s * R + (int)threadIdx.y >= m.nc) return;
   const int k = (blockIdx.y * blockDim.x + threadIdx.x) * 2;
   Vd<2> P[hexu::kNV], Q[hexu::kNV];
-  if (a.do_vn && (a.terms & T_PV)) union_pv2(m, mc, d, cA, cB, hi, k, P, Q);
-  edge_tend_hex2v<0>(m, mc, a, d, 3 * cA, 3 * cB, hi, k, P, Q);
-  edge_tend_hex2v<1>(m, mc, a, d, 3 * cA + 1, 3 * cB + 1, hi, k, P, Q);
-  edge_tend_hex2v<2>(m, mc, a, d, 3 * cA + 2, 3 * cB + 2, hi, k, P, Q);
-  cell_tend_hex2(m, mc, a, d, cA, cB, hi, k);
+  if (a.do_vn && (a.terms & T_PV)) union_pv2<U, W>(m, mc, d, cA, cB, hi, k, P, Q);
+  edge_tend_hex2v<0, U, W>(m, mc, a, d, 3 * cA, 3 * cB, hi, k, P, Q);
+  edge_tend_hex2v<1, U, W>(m, mc, a, d, 3 * cA + 1, 3 * cB + 1, hi, k, P, Q);
+  edge_tend_hex2v<2, U, W>(m, mc, a, d, 3 * cA + 2, 3 * cB + 2, hi, k, P, Q);
+  cell_tend_hex2<U, W>(m, mc, a, d, cA, cB, hi, k);
 }
 
 // the cell part alone (thickness and tracer tendencies), for launchers that
@@ -381,9 +385,12 @@ void launch_diag_hex(const MeshView &m, const Launch &lc, int terms, int ntr, co
   dim3 block;
   const int N = hex_n(m, lc);
   const dim3 grid = hex_grid(m, lc, N, block);
-  if (N == 2)
-    hipLaunchKernelGGL(k_diag_hex2, grid, block, 0, s, m, terms, ntr, h, vn, tr, d, lc.swizzle);
-  else
+  if (N == 2) {
+    // the uniform-coefficient and 32-bit-offset forms give the same bits
+    auto kern = m.coef_uni ? (m.off32 ? k_diag_hex2<true, false> : k_diag_hex2<true, true>)
+                           : (m.off32 ? k_diag_hex2<false, false> : k_diag_hex2<false, true>);
+    hipLaunchKernelGGL(kern, grid, block, 0, s, m, terms, ntr, h, vn, tr, d, lc.swizzle);
+  } else
     hipLaunchKernelGGL(k_diag_hex<1>, grid, block, 0, s, m, terms, ntr, h, vn, tr, d, lc.swizzle);
   OMEGA_HIP_CHECK(hipGetLastError());
 }
@@ -406,11 +413,16 @@ void launch_tend_hex(const MeshView &m, const Launch &lc, const ModelCoef &mc, c
   dim3 block;
   const int N = hex_n(m, lc);
   const dim3 grid = hex_grid(m, lc, N, block);
-  if (N == 2 && m.hexu_ok)
-    hipLaunchKernelGGL(k_tend_hex2v, grid, block, 0, s, m, mc, a, d, lc.swizzle);
-  else if (N == 2)
-    hipLaunchKernelGGL(k_tend_hex2, grid, block, 0, s, m, mc, a, d, lc.swizzle);
-  else
+  const bool U = m.coef_uni && mc.uniform;
+  if (N == 2 && m.hexu_ok) {
+    auto kern = U ? (m.off32 ? k_tend_hex2v<true, false> : k_tend_hex2v<true, true>)
+                  : (m.off32 ? k_tend_hex2v<false, false> : k_tend_hex2v<false, true>);
+    hipLaunchKernelGGL(kern, grid, block, 0, s, m, mc, a, d, lc.swizzle);
+  } else if (N == 2) {
+    auto kern = U ? (m.off32 ? k_tend_hex2<true, false> : k_tend_hex2<true, true>)
+                  : (m.off32 ? k_tend_hex2<false, false> : k_tend_hex2<false, true>);
+    hipLaunchKernelGGL(kern, grid, block, 0, s, m, mc, a, d, lc.swizzle);
+  } else
     hipLaunchKernelGGL(k_tend_hex<1>, grid, block, 0, s, m, mc, a, d, lc.swizzle);
   OMEGA_HIP_CHECK(hipGetLastError());
 }

@@ -1,6 +1,7 @@
 // Host-side mesh construction, space-filling-curve ordering and upload.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 
 #include "internal.hpp"
@@ -296,6 +297,17 @@ std::vector<int> order_cells(const HostMesh &m, int ordering) {
 // mesh.  Neighbour lists keep their local order, so every sum runs over the
 // same terms in the same order as in the reference.
 // ---------------------------------------------------------------------------
+
+// x[i*stride + j] == x[j] bit for bit for every row i
+template <class T> bool rows_equal(const std::vector<T> &x, size_t stride) {
+  if (stride == 0 || x.size() < stride || x.size() % stride) return false;
+  for (size_t i = stride; i < x.size(); i += stride)
+    if (std::memcmp(&x[i], &x[0], stride * sizeof(T)) != 0) return false;
+  return true;
+}
+template bool rows_equal(const std::vector<double> &, size_t);
+template bool rows_equal(const std::vector<int> &, size_t);
+
 void DeviceMesh::build_device(int ord) {
   HostMesh &m = host;
   ordering = ord;
@@ -410,6 +422,14 @@ void DeviceMesh::build_device(int ord) {
   }
 
   build_hex_records(*this);
+  // tables the pair kernels may read from one cell for both (bit-identical
+  // at every stencil slot: e = 3c+K, v = 2c+x on the hexagonal numbering)
+  coef_uni = hex_ok && rows_equal(inv_dc_h, 1) && rows_equal(inv_dv_h, 1) &&
+             rows_equal(maxlev_h, 1) && rows_equal(s2_h, 1) && rows_equal(s4_h, 1) &&
+             rows_equal(w_h, 3 * (size_t)me2) && rows_equal(cdiv_h, ME) &&
+             rows_equal(cke_h, ME) && rows_equal(crv_h, 6) && rows_equal(cth_h, 6);
+  const size_t rows = std::max({(size_t)NC, (size_t)NE, (size_t)NV});
+  off32 = rows < (size_t(1) << 24) && rows * (size_t)L * sizeof(double) < (size_t(1) << 32);
   uploaded = false;
 }
 
@@ -482,6 +502,8 @@ MeshView DeviceMesh::view() const {
   v.hex_ok = hex_ok ? 1 : 0;
   v.hexu_ok = hexu_ok ? 1 : 0;
   v.hexd_ok = hex_ok && hexd_ok ? 1 : 0;
+  v.coef_uni = hex_ok && coef_uni ? 1 : 0;
+  v.off32 = off32 ? 1 : 0;
   v.hexrec = hexrec.p;
   v.hexu = hexu.p;
   v.coc = coc.p;

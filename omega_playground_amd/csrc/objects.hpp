@@ -68,8 +68,12 @@ struct omega_model_s {
   omega_hip::LinCoef lin_coef() const {
     return omega_hip::LinCoef{lin_cdv.p, mesh->dm.area_cell.p, lin_dc.p, lin_fe.p, h0, params.grav};
   }
+  // f_v, and cv2/cv4/ct2/ct4 at every stencil slot, bit-identical across
+  // the mesh (ModelCoef::uniform)
+  bool f_uniform = false, mix_uniform = false;
   ModelCoef coef() const {
-    return ModelCoef{params.grav, f_v.p, cv2.p, cv4.p, ct2.p, ct4.p};
+    return ModelCoef{params.grav, f_v.p, cv2.p, cv4.p, ct2.p, ct4.p,
+                     f_uniform && mix_uniform ? 1 : 0};
   }
   DiagOut diag() {
     return DiagOut{rvort_v.p, inv_thick_v.p, ke_c.p, div_c.p, trdel2_c.p,

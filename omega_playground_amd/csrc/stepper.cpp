@@ -43,6 +43,9 @@ void model_build_coefficients(omega_model_s *M) {
   M->ct2_h = ct2;
   M->ct4_h = ct4;
   M->ct4.upload(ct4);
+  M->f_uniform = rows_equal(f, 1);
+  M->mix_uniform = rows_equal(cv2, 1) && rows_equal(cv4, 1) && rows_equal(ct2, ME) &&
+                   rows_equal(ct4, ME);
   M->rvort_v.alloc((size_t)NV * L);
   M->inv_thick_v.alloc((size_t)NV * L);
   M->ke_c.alloc((size_t)NC * L);

@@ -104,6 +104,7 @@ def lib():
         "omega_model_get_launch": ([vp, ip, ip, ip, ip], C.c_int),
         "omega_model_set_variant": ([vp, C.c_int], C.c_int),
         "omega_model_kernel_variant": ([vp, ip], C.c_int),
+        "omega_model_kernel_flags": ([vp, ip], C.c_int),
         "omega_model_set_tiling": ([vp, C.c_int, C.c_int], C.c_int),
         "omega_mesh_tile_stats": ([vp, C.c_int, ip], C.c_int),
         "omega_stepper_create": ([vp, C.c_int, C.c_int, pp], C.c_int),
@@ -503,6 +504,13 @@ class ShallowWaterModel:
         """Tile shape of the LDS-tiled kernels (variants 2 and 4): cells per
         tile and levels per block (8/16 for variant 2, 16/32 for variant 4)."""
         check(lib().omega_model_set_tiling(self._h, int(cells_per_tile), int(levels_per_block)))
+
+    def kernel_flags(self) -> int:
+        """bit 0: coefficient tables uniform per stencil slot (shared by a
+        wavefront's two cells); bit 1: 32-bit row offsets."""
+        v = C.c_int()
+        check(lib().omega_model_kernel_flags(self._h, C.byref(v)))
+        return v.value
 
     def kernel_variant(self) -> int:
         """6 linear model, 5 hexagonal tiled, 4 union-stencil, 3 tiled, 2 pattern-specialised,

@@ -209,6 +209,47 @@ def test_specialised_kernels_bitwise_L64(variant, lpt, ordering):
             assert same_bits(a, b), (kind, name)
 
 
+@pytest.mark.parametrize("perturb", ["none", "scalings", "f_vertex", "weights"])
+def test_specialised_kernel_forms_bitwise(perturb):
+    """The pair kernels read a coefficient once for both cells of a
+    wavefront when every table is the same at each stencil slot (the planar
+    mesh), and per cell otherwise: per-edge mesh scalings, a varying
+    Coriolis parameter or perturbed TRiSK weights switch to the per-cell
+    form.  Both give the oracle's bits."""
+    nx, ny, L, ntr, dc = 12, 10, 64, 2, 10e3
+    pd = params_dict(dc, ntr)
+    om = po.Mesh(nx, ny, dc, L)
+    rng = np.random.default_rng(21)
+    if perturb == "scalings":
+        om.mesh_scaling_del2[:] = rng.uniform(0.5, 1.5, om.nedges)
+        om.mesh_scaling_del4[:] = rng.uniform(0.5, 1.5, om.nedges)
+    if perturb == "weights":
+        om.weights_on_edge[:] *= 1 + 1e-3 * rng.standard_normal(om.weights_on_edge.shape)
+    omodel = po.Model(om, **pd)
+    gm = swe.MPASMesh.from_arrays({f: np.array(getattr(om, f)) for f in MPAS_FIELDS}, L)
+    gmodel = swe.ShallowWaterModel(gm, swe.ShallowWaterParams(**pd))
+    if perturb == "f_vertex":
+        f = pd["f0"] * (1 + 0.1 * rng.standard_normal(om.nvertices))
+        omodel.f_vertex[:] = f
+        gmodel.set_f_vertex(f)
+    assert gm.regular_hex and gmodel.kernel_variant() == 2
+    assert gmodel.kernel_flags() == (3 if perturb == "none" else 2)
+    print("case", perturb, gmodel.kernel_flags(), flush=True)
+    state = rich_state(om, L, ntr, 5)
+    tend_o = omodel.new_state()
+    omodel.compute_tendency(tend_o, state, 0.0)
+    gt = swe.ShallowWaterState(gm, ntr)
+    gmodel.compute_tendency(gt, upload(gm, ntr, state), 0.0)
+    for name, a, b in zip(("h", "vn", "tr"), gt.download(), tend_o):
+        assert same_bits(a, b), name
+    ost = tuple(a.copy() for a in state)
+    po.Stepper(omodel, "rk4").run(0.0, 30.0, 2, ost)
+    gs = upload(gm, ntr, state)
+    swe.RK4Stepper(gmodel).run(0.0, 30.0, 2, gs)
+    for name, a, b in zip(("h", "vn", "tr"), gs.download(), ost):
+        assert same_bits(a, b), name
+
+
 @pytest.mark.parametrize("terms", [1, 2, 4, 8, 16, 32, 64])
 def test_specialised_each_term_bitwise(terms):
     om, omodel, gm, gmodel, state = make_pair(10, 10, 64, 2)
