@@ -164,6 +164,23 @@ __device__ __forceinline__ Pair pair_of(const BatchMember &B, int lb) {
   return q;
 }
 
+// the forms of the pair kernels a member's flags allow (same bits)
+template <bool U, bool W>
+__device__ __forceinline__ void batch_diag_pair(const BatchMember &B, const Pair &q) {
+  const MeshView &m = B.m;
+  const DiagOut &d = B.d;
+  const double *h = B.a.h.in, *vn = B.a.vn.in, *tr = B.a.tr.in;
+  if (m.hexd_ok) {
+    hexops::diag_union_p<U, W>(m, B.dterms, B.a.ntr, q.cA, q.cB, q.hi, q.k, h, vn, tr, d);
+    return;
+  }
+  if (B.dterms & (T_PV | T_VDEL2 | T_VDEL4)) {
+    hexops::vertex_diag_p<U, W>(m, 2 * q.cA, 2 * q.cB, q.hi, q.k, h, vn, d);
+    hexops::vertex_diag_p<U, W>(m, 2 * q.cA + 1, 2 * q.cB + 1, q.hi, q.k, h, vn, d);
+  }
+  hexops::cell_diag_p<U, W>(m, B.dterms, B.a.ntr, q.cA, q.cB, q.hi, q.k, h, vn, tr, d);
+}
+
 __global__ void __launch_bounds__(256)
     k_batch_diag_hex2(const BatchMember *__restrict__ desc, const int *__restrict__ blk) {
   const int id = uniform(blk[blockIdx.x]);
@@ -171,18 +188,29 @@ __global__ void __launch_bounds__(256)
   if (!B.dterms) return;
   const Pair q = pair_of(B, (int)blockIdx.x - B.block0);
   if (!q.ok) return;
+  if (B.m.coef_uni && B.m.off32) batch_diag_pair<true, false>(B, q);
+  else batch_diag_pair<false, true>(B, q);
+}
+
+template <bool U, bool W>
+__device__ __forceinline__ void batch_tend_pair(const BatchMember &B, const Pair &q) {
   const MeshView &m = B.m;
+  const ModelCoef &mc = B.mc;
+  const StageArgs &a = B.a;
   const DiagOut &d = B.d;
-  const double *h = B.a.h.in, *vn = B.a.vn.in, *tr = B.a.tr.in;
-  if (m.hexd_ok) {
-    hexops::diag_union_p<false, true>(m, B.dterms, B.a.ntr, q.cA, q.cB, q.hi, q.k, h, vn, tr, d);
-    return;
+  const int eA = 3 * q.cA, eB = 3 * q.cB;
+  if (m.hexu_ok) {
+    Vd<2> P[hexu::kNV], Q[hexu::kNV];
+    if (a.do_vn && (a.terms & T_PV)) hexops::union_pv2<U, W>(m, mc, d, q.cA, q.cB, q.hi, q.k, P, Q);
+    hexops::edge_tend_hex2v<0, U, W>(m, mc, a, d, eA, eB, q.hi, q.k, P, Q);
+    hexops::edge_tend_hex2v<1, U, W>(m, mc, a, d, eA + 1, eB + 1, q.hi, q.k, P, Q);
+    hexops::edge_tend_hex2v<2, U, W>(m, mc, a, d, eA + 2, eB + 2, q.hi, q.k, P, Q);
+  } else {
+    hexops::edge_tend_hex2<0, U, W>(m, mc, a, d, eA, eB, q.hi, q.k);
+    hexops::edge_tend_hex2<1, U, W>(m, mc, a, d, eA + 1, eB + 1, q.hi, q.k);
+    hexops::edge_tend_hex2<2, U, W>(m, mc, a, d, eA + 2, eB + 2, q.hi, q.k);
   }
-  if (B.dterms & (T_PV | T_VDEL2 | T_VDEL4)) {
-    hexops::vertex_diag_p<false, true>(m, 2 * q.cA, 2 * q.cB, q.hi, q.k, h, vn, d);
-    hexops::vertex_diag_p<false, true>(m, 2 * q.cA + 1, 2 * q.cB + 1, q.hi, q.k, h, vn, d);
-  }
-  hexops::cell_diag_p<false, true>(m, B.dterms, B.a.ntr, q.cA, q.cB, q.hi, q.k, h, vn, tr, d);
+  hexops::cell_tend_hex2<U, W>(m, mc, a, d, q.cA, q.cB, q.hi, q.k);
 }
 
 __global__ void __launch_bounds__(256)
@@ -191,23 +219,8 @@ __global__ void __launch_bounds__(256)
   const BatchMember &B = desc[id];
   const Pair q = pair_of(B, (int)blockIdx.x - B.block0);
   if (!q.ok) return;
-  const MeshView &m = B.m;
-  const ModelCoef &mc = B.mc;
-  const StageArgs &a = B.a;
-  const DiagOut &d = B.d;
-  const int eA = 3 * q.cA, eB = 3 * q.cB;
-  if (m.hexu_ok) {
-    Vd<2> P[hexu::kNV], Q[hexu::kNV];
-    if (a.do_vn && (a.terms & T_PV)) hexops::union_pv2<false, true>(m, mc, d, q.cA, q.cB, q.hi, q.k, P, Q);
-    hexops::edge_tend_hex2v<0, false, true>(m, mc, a, d, eA, eB, q.hi, q.k, P, Q);
-    hexops::edge_tend_hex2v<1, false, true>(m, mc, a, d, eA + 1, eB + 1, q.hi, q.k, P, Q);
-    hexops::edge_tend_hex2v<2, false, true>(m, mc, a, d, eA + 2, eB + 2, q.hi, q.k, P, Q);
-  } else {
-    hexops::edge_tend_hex2<0, false, true>(m, mc, a, d, eA, eB, q.hi, q.k);
-    hexops::edge_tend_hex2<1, false, true>(m, mc, a, d, eA + 1, eB + 1, q.hi, q.k);
-    hexops::edge_tend_hex2<2, false, true>(m, mc, a, d, eA + 2, eB + 2, q.hi, q.k);
-  }
-  hexops::cell_tend_hex2<false, true>(m, mc, a, d, q.cA, q.cB, q.hi, q.k);
+  if (B.m.coef_uni && B.mc.uniform && B.m.off32) batch_tend_pair<true, false>(B, q);
+  else batch_tend_pair<false, true>(B, q);
 }
 
 } // namespace
