@@ -266,6 +266,9 @@ int omega_model_kernel_variant(omega_model_t model, int *in_use);
  * counterpart (introspection for tests and tuning; results are the same
  * bits either way). */
 int omega_model_kernel_flags(omega_model_t model, int *flags);
+/* Which of those forms the model may use (bits as above, default 3);
+ * clearing a bit forces the per-cell or 64-bit form (tests, tuning). */
+int omega_model_set_kernel_forms(omega_model_t model, int forms);
 /* Tiled kernels: cells per tile (1..256, default 32) and levels per block
  * (8, 16 or 32, default 8; the general tiled kernels use 8 or 16 and take
  * 32 as 16, the hexagonal tiled kernel uses 16 or 32 and takes 8 as 16);

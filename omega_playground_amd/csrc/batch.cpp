@@ -122,7 +122,7 @@ void build_tables(omega_batch_s *B) {
         omega_model_s *M = mb.M;
         omega_state_s *S = mb.S;
         BatchMember g = geo[i];
-        g.m = M->mesh->dm.view();
+        g.m = M->view();
         g.mc = M->coef();
         g.d = M->diag();
         StageArgs a{};

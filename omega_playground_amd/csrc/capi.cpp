@@ -603,7 +603,7 @@ int omega_model_set_tiling(omega_model_t M, int cells_per_tile, int levels_per_b
 int omega_model_kernel_variant(omega_model_t M, int *in_use) {
   return guarded([&] {
     OMEGA_REQUIRE(M && in_use, "null argument");
-    const MeshView mv = M->mesh->dm.view();
+    const MeshView mv = M->view();
     ensure_tiles(M);
     *in_use = M->linear                                   ? 6
               : M->lc.variant == 2                        ? 3
@@ -614,10 +614,18 @@ int omega_model_kernel_variant(omega_model_t M, int *in_use) {
   });
 }
 
+int omega_model_set_kernel_forms(omega_model_t M, int forms) {
+  return guarded([&] {
+    OMEGA_REQUIRE(M, "null model");
+    OMEGA_REQUIRE(forms >= 0 && forms <= 3, "forms must be 0..3");
+    M->forms = forms;
+  });
+}
+
 int omega_model_kernel_flags(omega_model_t M, int *flags) {
   return guarded([&] {
     OMEGA_REQUIRE(M && flags, "null argument");
-    const MeshView mv = M->mesh->dm.view();
+    const MeshView mv = M->view();
     const ModelCoef mc = M->coef();
     *flags = (mv.coef_uni && mc.uniform ? 1 : 0) | (mv.off32 ? 2 : 0);
   });

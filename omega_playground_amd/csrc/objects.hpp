@@ -71,6 +71,16 @@ struct omega_model_s {
   // f_v, and cv2/cv4/ct2/ct4 at every stencil slot, bit-identical across
   // the mesh (ModelCoef::uniform)
   bool f_uniform = false, mix_uniform = false;
+  // pair-kernel forms the model may use (bit 0 shared coefficients, bit 1
+  // 32-bit offsets); clearing bits forces the general forms (tests, tuning)
+  int forms = 3;
+  // the mesh view this model's kernels get
+  omega_hip::MeshView view() const {
+    omega_hip::MeshView v = mesh->dm.view();
+    if (!(forms & 1)) v.coef_uni = 0;
+    if (!(forms & 2)) v.off32 = 0;
+    return v;
+  }
   ModelCoef coef() const {
     return ModelCoef{params.grav, f_v.p, cv2.p, cv4.p, ct2.p, ct4.p,
                      f_uniform && mix_uniform ? 1 : 0};

@@ -98,7 +98,7 @@ void ensure_tiles(omega_model_s *M) {
 // energy and divergence, tracer Laplacian, momentum del4 pre-pass) into the
 // model's buffers; only what the enabled terms read.
 void run_diag(omega_model_s *M, const Real *h, const Real *vn, const Real *tr, hipStream_t s) {
-  const MeshView mv = M->mesh->dm.view();
+  const MeshView mv = M->view();
   const int ntr = M->params.ntracers;
   const bool do_vn = !M->params.disable_vn_tendency;
   // diagnostics are needed by the vn tendency and by tracer del4 only
@@ -113,7 +113,7 @@ void run_diag(omega_model_s *M, const Real *h, const Real *vn, const Real *tr, h
 
 // One tendency evaluation (+ optional fused stage update).
 void run_stage(omega_model_s *M, StageArgs a, hipStream_t s, int fields) {
-  const MeshView mv = M->mesh->dm.view();
+  const MeshView mv = M->view();
   const ModelCoef mc = M->coef();
   a.terms = M->terms;
   a.do_h = !M->params.disable_h_tendency && (fields & 1);

@@ -105,6 +105,7 @@ def lib():
         "omega_model_set_variant": ([vp, C.c_int], C.c_int),
         "omega_model_kernel_variant": ([vp, ip], C.c_int),
         "omega_model_kernel_flags": ([vp, ip], C.c_int),
+        "omega_model_set_kernel_forms": ([vp, C.c_int], C.c_int),
         "omega_model_set_tiling": ([vp, C.c_int, C.c_int], C.c_int),
         "omega_mesh_tile_stats": ([vp, C.c_int, ip], C.c_int),
         "omega_stepper_create": ([vp, C.c_int, C.c_int, pp], C.c_int),
@@ -511,6 +512,11 @@ class ShallowWaterModel:
         v = C.c_int()
         check(lib().omega_model_kernel_flags(self._h, C.byref(v)))
         return v.value
+
+    def set_kernel_forms(self, forms: int = 3):
+        """Allow (bits set) or forbid the shared-coefficient (bit 0) and
+        32-bit-offset (bit 1) forms of the pair kernels."""
+        check(lib().omega_model_set_kernel_forms(self._h, int(forms)))
 
     def kernel_variant(self) -> int:
         """6 linear model, 5 hexagonal tiled, 4 union-stencil, 3 tiled, 2 pattern-specialised,

@@ -75,6 +75,31 @@ def test_batch_matches_members_stepped_alone(kind, nstages, graph, odd):
             assert same_bits(a, c), (i, name)
 
 
+def test_batch_kernel_forms_per_member():
+    """Two 64-layer members, one allowed the shared-coefficient and 32-bit
+    forms of the pair kernels and one forced to the general forms, in one
+    batch: each ends with the bits of its solo steps and of the oracle."""
+    members = make_members([(8, 8, 64, 2, 10e3, {}, 12.0), (10, 8, 64, 1, 10e3, {}, 15.0)])
+    members[1][1].set_kernel_forms(0)
+    assert members[0][1].kernel_flags() == 3 and members[1][1].kernel_flags() == 0
+    ref = solo(members, "rk4", 4, 3)
+    b = swe.Batch("rk4", 4)
+    states = []
+    for mesh, model, ntr, y0, dt, _ in members:
+        s = state_of(mesh, ntr, y0)
+        b.add(model, s, dt)
+        states.append(s)
+    b.run(3)
+    for i, (s, r) in enumerate(zip(states, ref)):
+        for name, a, c in zip(("h", "vn", "tr"), s.download(), r):
+            assert same_bits(a, c), (i, name)
+    mesh, model, ntr, y0, dt, pd = members[1]
+    oref = tuple(a.copy() for a in y0)
+    po.Stepper(po.Model(po.Mesh(10, 8, 10e3, 64), **pd), "rk4").run(0.0, dt, 3, oref)
+    for name, a, c in zip(("h", "vn", "tr"), states[1].download(), oref):
+        assert same_bits(a, c), name
+
+
 def test_batch_member_matches_oracle():
     members = make_members(MEMBERS[1:2])
     mesh, model, ntr, y0, dt, pd = members[0]

@@ -209,8 +209,9 @@ def test_specialised_kernels_bitwise_L64(variant, lpt, ordering):
             assert same_bits(a, b), (kind, name)
 
 
-@pytest.mark.parametrize("perturb", ["none", "scalings", "f_vertex", "weights"])
-def test_specialised_kernel_forms_bitwise(perturb):
+@pytest.mark.parametrize("perturb,forms", [("none", 3), ("none", 2), ("none", 1), ("none", 0),
+                                           ("scalings", 3), ("f_vertex", 3), ("weights", 3)])
+def test_specialised_kernel_forms_bitwise(perturb, forms):
     """The pair kernels read a coefficient once for both cells of a
     wavefront when every table is the same at each stencil slot (the planar
     mesh), and per cell otherwise: per-edge mesh scalings, a varying
@@ -232,8 +233,9 @@ def test_specialised_kernel_forms_bitwise(perturb):
         f = pd["f0"] * (1 + 0.1 * rng.standard_normal(om.nvertices))
         omodel.f_vertex[:] = f
         gmodel.set_f_vertex(f)
+    gmodel.set_kernel_forms(forms)
     assert gm.regular_hex and gmodel.kernel_variant() == 2
-    assert gmodel.kernel_flags() == (3 if perturb == "none" else 2)
+    assert gmodel.kernel_flags() == (3 if perturb == "none" else 2) & forms
     print("case", perturb, gmodel.kernel_flags(), flush=True)
     state = rich_state(om, L, ntr, 5)
     tend_o = omodel.new_state()
