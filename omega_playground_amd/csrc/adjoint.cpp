@@ -86,6 +86,10 @@ void build_adj_layout(const DeviceMesh &dm, AdjLayout &a) {
       ++fill[x];
     }
   (void)NV;
+  a.fixed = true;
+  for (int c = 0; c < NC && a.fixed; ++c) a.fixed = dm.noc_h[c] == 6;
+  for (int e = 0; e < NE && a.fixed; ++e)
+    a.fixed = dm.noe_h[e] == 10 && tofs[e + 1] - tofs[e] == 10;
   a.ecd.upload(ecd);
   a.ecke.upload(ecke);
   a.ecrv.upload(ecrv);
@@ -127,6 +131,9 @@ void ensure_adjoint(omega_model_s *M) {
   }
   if (!M->a_sbar.p) {
     M->a_se.alloc(NE * L);
+    M->a_fu.alloc(NE * L);
+    M->a_pa.alloc(NE * L);
+    M->a_pb.alloc(NE * L);
     M->a_sbar.alloc(NE * L);
     M->a_fbar.alloc(NE * L);
     M->a_e2bar.alloc(NE * L);
@@ -148,7 +155,7 @@ AdjScratch scratch(omega_model_s *M) {
   return AdjScratch{M->a_sbar.p, M->a_zbar.p,  M->a_tbar.p,  M->a_kbar.p,
                     M->a_dbar.p, M->a_gbar.p,  M->a_fbar.p,  M->a_d2bar.p,
                     M->a_z2bar.p, M->a_e2bar.p, M->a_partial.p, M->a_pgrad.p,
-                    M->a_se.p};
+                    M->a_se.p, M->a_fu.p, M->a_pa.p, M->a_pb.p};
 }
 
 // out += J(y)^T k for the tendency at state y (pointers in device numbering);

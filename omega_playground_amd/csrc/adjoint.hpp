@@ -16,6 +16,9 @@ struct AdjMeshView {
   const int *tofs;    // [ne+1] edges that list e among their edges_on_edge
   const int *tedge;   // [nnz] source edge
   const Real *tw;     // [nnz] its weight for e
+  // every cell has 6 edges, every edge 10 edges-on-edge and appears in 10
+  // such lists (the regular hexagonal mesh): loops get constant trip counts
+  int fixed;
 };
 
 struct AdjLayout {
@@ -24,8 +27,10 @@ struct AdjLayout {
   DBuf<Real> area_e;          // [ne] dc*dv/2, the area an edge accounts for
   std::vector<int> cslot_h;   // [ne][2] slot of the edge in cells_on_edge[0..1]
   bool built = false;
+  bool fixed = false;
   AdjMeshView view() const {
-    return AdjMeshView{ecd.p, ecke.p, ecrv.p, voc.p, ccth.p, tofs.p, tedge.p, tw.p};
+    return AdjMeshView{ecd.p, ecke.p, ecrv.p, voc.p, ccth.p, tofs.p, tedge.p, tw.p,
+                       fixed ? 1 : 0};
   }
 };
 
@@ -58,6 +63,9 @@ struct AdjScratch {
   Real *partial;  // [4][nparts] parameter-gradient block partials
   Real *pgrad;    // [4] accumulated parameter gradient
   Real *se_e;     // [ne][L] edge PV sum Q_e + P_e at the linearisation state
+  Real *fu_e;     // [ne][L] edge thickness flux F_e u_e at the linearisation state
+  Real *pa_e;     // [ne][L] sum over t listing e of w_te kbar_t S_t
+  Real *pb_e;     // [ne][L] sum over t listing e of w_te kbar_t
 };
 
 struct AdjArgs {

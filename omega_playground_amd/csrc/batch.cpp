@@ -86,12 +86,15 @@ void build_tables(omega_batch_s *B) {
       ensure3(mb.tend, S);
     }
     BatchMember &g = geo[i];
-    g.hex = hex_usable(M->mesh->dm.view(), M->lc) ? 1 : 0;
+    // 16 or 32 layers run the pattern kernels with several cells per
+    // wavefront, which needs two levels per lane
+    g.hex = hex_usable(M->mesh->dm.view(), M->lc) && (hm.nlayers % 64 == 0 || N == 2) ? 1 : 0;
     if (g.hex) {
-      // one row of 64/N lanes covers 64 levels
-      g.lx = 64 / N;
+      // one row of lanes covers 64 levels (or all 16 / 32)
+      const int span = hm.nlayers % 64 == 0 ? 64 : hm.nlayers;
+      g.lx = span / N;
       g.rows = 256 / g.lx;
-      g.chunks = hm.nlayers / 64;
+      g.chunks = hm.nlayers / span;
     } else {
       int lanes = (hm.nlayers + N - 1) / N;
       if (lanes > 64) lanes = 64;

@@ -88,7 +88,7 @@ __device__ __forceinline__ double mask(int k, int i, int ml, double c) {
 // Momentum del4 chain: cotangents of the del2 divergence / vorticity, then
 // of the del2 edge field (vel_hyperdiff_edge.hpp:140-237 transposed)
 // ---------------------------------------------------------------------------
-template <int N>
+template <int N, int H>
 __global__ void __launch_bounds__(kAB)
     k_adj_v4a(MeshView m, ModelCoef mc, AdjArgs a, AdjScratch s, int swz) {
   const Seg g = segment<N>(m, swz);
@@ -109,7 +109,9 @@ __global__ void __launch_bounds__(kAB)
   }
   for (int c = g.c0 + (int)threadIdx.y; c < g.c1; c += blockDim.y) {
     Vd<N> d2 = zero<N>();
-    for (int j = 0; j < m.noc[c]; ++j) {
+    const int n = H ? 6 : m.noc[c];
+#pragma unroll
+    for (int j = 0; j < n; ++j) {
       const int e = m.eoc[c * m.me + j];
       const double sg = (m.coe[2 * e + 1] == c) ? -1.0 : 1.0;
       const Vd<N> ut = rd<N>(a.c.vn, e, L, k);
@@ -121,7 +123,7 @@ __global__ void __launch_bounds__(kAB)
   }
 }
 
-template <int N>
+template <int N, int H>
 __global__ void __launch_bounds__(kAB)
     k_adj_v4b(MeshView m, AdjMeshView am, AdjScratch s, int swz) {
   const Seg g = segment<N>(m, swz);
@@ -144,7 +146,37 @@ __global__ void __launch_bounds__(kAB)
 // PV flux: cotangent of S_e = Q_e + P_e, from the edge's own sum and from the
 // sums of the edges that list it (transposed edges_on_edge)
 // ---------------------------------------------------------------------------
-template <int N>
+// Per edge at the linearisation state: the PV sum S_e = Q_e + P_e and the
+// thickness flux F_e u_e, read by the PV pass for every edge whose
+// edges-on-edge list holds e and by the edge pass.
+template <int N, int H>
+__global__ void __launch_bounds__(kAB)
+    k_adj_pre(MeshView m, ModelCoef mc, AdjArgs a, DiagOut d, AdjScratch s, int swz) {
+  const Seg g = segment<N>(m, swz);
+  if (!g.active) return;
+  const int L = m.L, k = g.k;
+  for (int e = m.eofs[g.c0] + (int)threadIdx.y; e < m.eofs[g.c1]; e += blockDim.y) {
+    const int v0 = m.voe[2 * e], v1 = m.voe[2 * e + 1];
+    const double fv0 = mc.f_v[v0], fv1 = mc.f_v[v1];
+    const Vd<N> i0 = rd<N>(d.inv_thick_v, v0, L, k), i1 = rd<N>(d.inv_thick_v, v1, L, k);
+    const Vd<N> z0 = rd<N>(d.rvort_v, v0, L, k), z1 = rd<N>(d.rvort_v, v1, L, k);
+    const Vd<N> h0 = rd<N>(a.h, m.coe[2 * e], L, k), h1 = rd<N>(a.h, m.coe[2 * e + 1], L, k);
+    const Vd<N> ue = rd<N>(a.vn, e, L, k);
+    Vd<N> Se, FU;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      Se[i] = 0.5 * (i0[i] * z0[i] + i1[i] * z1[i]) + 0.5 * (i0[i] * fv0 + i1[i] * fv1);
+      FU[i] = 0.5 * (h0[i] + h1[i]) * ue[i];
+    }
+    st<N>(s.se_e + (size_t)e * L + k, Se);
+    st<N>(s.fu_e + (size_t)e * L + k, FU);
+  }
+}
+
+// PV-sum cotangent of each edge, and the two transposed sums the edge pass
+// needs: A_e = sum_t w_te kbar_t S_t, B_e = sum_t w_te kbar_t over the edges
+// t whose edges-on-edge list holds e (kbar = velocity-tendency cotangent).
+template <int N, int H>
 __global__ void __launch_bounds__(kAB)
     k_adj_pv(MeshView m, AdjMeshView am, ModelCoef mc, AdjArgs a, DiagOut d, AdjScratch s,
              int swz) {
@@ -152,52 +184,45 @@ __global__ void __launch_bounds__(kAB)
   if (!g.active) return;
   const int L = m.L, k = g.k;
   for (int e = m.eofs[g.c0] + (int)threadIdx.y; e < m.eofs[g.c1]; e += blockDim.y) {
-    {
-      // edge PV sum S_e = Q_e + P_e, read by the edge pass for e and for
-      // every edge whose edges-on-edge list holds e
-      const int v0 = m.voe[2 * e], v1 = m.voe[2 * e + 1];
-      const double fv0 = mc.f_v[v0], fv1 = mc.f_v[v1];
-      const Vd<N> i0 = rd<N>(d.inv_thick_v, v0, L, k), i1 = rd<N>(d.inv_thick_v, v1, L, k);
-      const Vd<N> z0 = rd<N>(d.rvort_v, v0, L, k), z1 = rd<N>(d.rvort_v, v1, L, k);
-      Vd<N> Se;
-#pragma unroll
-      for (int i = 0; i < N; ++i)
-        Se[i] = 0.5 * (i0[i] * z0[i] + i1[i] * z1[i]) + 0.5 * (i0[i] * fv0 + i1[i] * fv1);
-      st<N>(s.se_e + (size_t)e * L + k, Se);
-    }
-    const Vd<N> ue = rd<N>(a.vn, e, L, k);
+    const Vd<N> fue = rd<N>(s.fu_e, e, L, k);
     const Vd<N> ute = rd<N>(a.c.vn, e, L, k);
-    const Vd<N> he0 = rd<N>(a.h, m.coe[2 * e], L, k), he1 = rd<N>(a.h, m.coe[2 * e + 1], L, k);
-    Vd<N> own = zero<N>(), oth = zero<N>();
-    for (int j = 0; j < m.noe[e]; ++j) {
+    Vd<N> own = zero<N>(), pa = zero<N>(), pb = zero<N>();
+    const int ne2 = H ? 10 : m.noe[e];
+#pragma unroll
+    for (int j = 0; j < ne2; ++j) {
       const int ej = m.eoe[e * m.me2 + j];
       const double w = m.w[e * m.me2 + j];
-      const Vd<N> h0 = rd<N>(a.h, m.coe[2 * ej], L, k), h1 = rd<N>(a.h, m.coe[2 * ej + 1], L, k);
-      const Vd<N> uj = rd<N>(a.vn, ej, L, k);
+      const Vd<N> fu = rd<N>(s.fu_e, ej, L, k);
 #pragma unroll
-      for (int i = 0; i < N; ++i) own[i] += w * (0.5 * (h0[i] + h1[i])) * uj[i];
+      for (int i = 0; i < N; ++i) own[i] += w * fu[i];
     }
-    for (int q = am.tofs[e]; q < am.tofs[e + 1]; ++q) {
+    const int q0 = am.tofs[e], nq = H ? 10 : am.tofs[e + 1] - q0;
+#pragma unroll
+    for (int qq = 0; qq < nq; ++qq) {
+      const int q = q0 + qq;
       const int t = am.tedge[q];
       const double w = am.tw[q];
-      const Vd<N> ut = rd<N>(a.c.vn, t, L, k);
+      const Vd<N> ut = rd<N>(a.c.vn, t, L, k), St = rd<N>(s.se_e, t, L, k);
 #pragma unroll
-      for (int i = 0; i < N; ++i) oth[i] += ut[i] * w;
+      for (int i = 0; i < N; ++i) {
+        const double wu = ut[i] * w;
+        pb[i] += wu;
+        pa[i] += wu * St[i];
+      }
     }
     Vd<N> out;
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-      const double Fe = 0.5 * (he0[i] + he1[i]);
-      out[i] = 0.5 * (ute[i] * own[i] + oth[i] * Fe * ue[i]);
-    }
+    for (int i = 0; i < N; ++i) out[i] = 0.5 * (ute[i] * own[i] + pb[i] * fue[i]);
     st<N>(s.sbar_e + (size_t)e * L + k, out);
+    st<N>(s.pa_e + (size_t)e * L + k, pa);
+    st<N>(s.pb_e + (size_t)e * L + k, pb);
   }
 }
 
 // ---------------------------------------------------------------------------
 // Vertex and cell diagnostics cotangents
 // ---------------------------------------------------------------------------
-template <int N>
+template <int N, int H>
 __global__ void __launch_bounds__(kAB)
     k_adj_vc(MeshView m, ModelCoef mc, AdjCoef ac, AdjArgs a, DiagOut d, AdjScratch s, int swz) {
   const Seg g = segment<N>(m, swz);
@@ -246,7 +271,8 @@ __global__ void __launch_bounds__(kAB)
   const bool td4 = (a.terms & T_TDEL4) && a.ntr > 0;
   for (int c = g.c0 + (int)threadIdx.y; c < g.c1; c += blockDim.y) {
     Vd<N> kb = zero<N>(), db = zero<N>();
-    const int n = m.noc[c];
+    const int n = H ? 6 : m.noc[c];
+#pragma unroll
     for (int j = 0; j < n; ++j) {
       const int e = m.eoc[c * m.me + j];
       const bool first = m.coe[2 * e] == c;
@@ -271,6 +297,7 @@ __global__ void __launch_bounds__(kAB)
       for (int l = 0; l < a.ntr; ++l) {
         const double *tl = a.c.tr + (size_t)l * NC * L;
         Vd<N> gb = zero<N>();
+#pragma unroll
         for (int j = 0; j < n; ++j) {
           const int e = m.eoc[c * m.me + j];
           const int a0 = m.coe[2 * e], a1 = m.coe[2 * e + 1];
@@ -289,7 +316,7 @@ __global__ void __launch_bounds__(kAB)
 // ---------------------------------------------------------------------------
 // Edge pass: velocity cotangent (accumulated) and edge-thickness cotangent
 // ---------------------------------------------------------------------------
-template <int N>
+template <int N, int H>
 __global__ void __launch_bounds__(kAB)
     k_adj_edge(MeshView m, AdjMeshView am, ModelCoef mc, AdjCoef ac, AdjArgs a, DiagOut d,
                AdjScratch s, int swz) {
@@ -339,20 +366,14 @@ __global__ void __launch_bounds__(kAB)
       }
       if (a.do_vn) {
         if (pv) {
-          // S_e = Q_e + P_e of this edge and S_t of each t (k_adj_pv)
+          // sum_t kbar_t w_te (S_t + S_e)/2 = (A_e + S_e B_e)/2 (k_adj_pre, k_adj_pv)
           const Vd<N> Se = rd<N>(s.se_e, e, L, k);
-          for (int q = am.tofs[e]; q < am.tofs[e + 1]; ++q) {
-            const int t = am.tedge[q];
-            const double w = am.tw[q];
-            const Vd<N> Stv = rd<N>(s.se_e, t, L, k);
-            const Vd<N> ut = rd<N>(a.c.vn, t, L, k);
+          const Vd<N> pa = rd<N>(s.pa_e, e, L, k), pb = rd<N>(s.pb_e, e, L, k);
 #pragma unroll
-            for (int i = 0; i < N; ++i) {
-              const double St = Stv[i];
-              const double nv = (St + Se[i]) * 0.5;
-              ub[i] += ut[i] * w * F[i] * nv;
-              fb[i] += ut[i] * w * ue[i] * nv;
-            }
+          for (int i = 0; i < N; ++i) {
+            const double x = 0.5 * (pa[i] + Se[i] * pb[i]);
+            ub[i] += x * F[i];
+            fb[i] += x * ue[i];
           }
         }
         if (pv || vd2 || vd4) {
@@ -404,7 +425,7 @@ __global__ void __launch_bounds__(kAB)
 // ---------------------------------------------------------------------------
 // Cell pass: thickness and tracer cotangents (accumulated)
 // ---------------------------------------------------------------------------
-template <int N>
+template <int N, int H>
 __global__ void __launch_bounds__(kAB)
     k_adj_cell(MeshView m, AdjMeshView am, ModelCoef mc, AdjCoef ac, AdjArgs a, DiagOut d,
                AdjScratch s, int swz) {
@@ -415,9 +436,10 @@ __global__ void __launch_bounds__(kAB)
   double p_t2 = 0.0, p_t4 = 0.0;
   if (g.active) {
     for (int c = g.c0 + (int)threadIdx.y; c < g.c1; c += blockDim.y) {
-      const int n = m.noc[c];
+      const int n = H ? 6 : m.noc[c];
       const Vd<N> hc = rd<N>(a.h, c, L, k);
       Vd<N> hb = zero<N>();
+#pragma unroll
       for (int j = 0; j < n; ++j) {
         const int e = m.eoc[c * me + j];
         const Vd<N> fb = rd<N>(s.fbar_e, e, L, k);
@@ -436,26 +458,57 @@ __global__ void __launch_bounds__(kAB)
           for (int i = 0; i < N; ++i) hb[i] += am.ccth[c * me + j] * tb[i];
         }
       }
+      // the cell's own rows are loaded once; each edge then loads only the
+      // rows of the cell across it (same values, same expressions)
+      Vd<N> ihc;
+#pragma unroll
+      for (int i = 0; i < N; ++i) ihc[i] = 1.0 / hc[i];
       for (int l = 0; l < a.ntr; ++l) {
         const size_t off = (size_t)l * NC * L;
         const Vd<N> tcl = rd<N>(a.tr + off, c, L, k);
         const Vd<N> bcl = rd<N>(a.c.tr + off, c, L, k);
+        Vd<N> gcl = zero<N>(), Gcl = zero<N>();
+        if (td4) {
+          gcl = rd<N>(s.gbar_c + off, c, L, k);
+          Gcl = rd<N>(d.trdel2_c + off, c, L, k);
+        }
         Vd<N> nb = zero<N>(), t2 = zero<N>(), t4 = zero<N>();
+#pragma unroll
         for (int j = 0; j < n; ++j) {
           const int e = m.eoc[c * me + j];
           const int a0 = m.coe[2 * e], a1 = m.coe[2 * e + 1];
-          const double sg = (a1 == c) ? 1.0 : -1.0;
+          const bool second = a1 == c;
+          const int o = second ? a0 : a1;
+          const double sg = second ? 1.0 : -1.0;
           const double cd0 = am.ecd[2 * e], cd1 = am.ecd[2 * e + 1], idc = m.inv_dc[e];
-          const Vd<N> h0 = rd<N>(a.h, a0, L, k), h1 = rd<N>(a.h, a1, L, k);
+          const Vd<N> ho = rd<N>(a.h, o, L, k);
           const Vd<N> ue = rd<N>(a.vn, e, L, k);
-          const Vd<N> b0 = rd<N>(a.c.tr + off, a0, L, k), b1 = rd<N>(a.c.tr + off, a1, L, k);
-          const Vd<N> tt0 = rd<N>(a.tr + off, a0, L, k), tt1 = rd<N>(a.tr + off, a1, L, k);
-          Vd<N> g0 = zero<N>(), g1 = zero<N>(), G0 = zero<N>(), G1 = zero<N>();
+          const Vd<N> bo = rd<N>(a.c.tr + off, o, L, k);
+          const Vd<N> to = rd<N>(a.tr + off, o, L, k);
+          Vd<N> h0, h1, b0, b1, tt0, tt1, g0 = zero<N>(), g1 = zero<N>(), G0 = zero<N>(),
+                                         G1 = zero<N>(), ih0, ih1;
+          Vd<N> iho;
+#pragma unroll
+          for (int i = 0; i < N; ++i) {
+            iho[i] = 1.0 / ho[i];
+            h0[i] = second ? ho[i] : hc[i];
+            h1[i] = second ? hc[i] : ho[i];
+            ih0[i] = second ? iho[i] : ihc[i];
+            ih1[i] = second ? ihc[i] : iho[i];
+            b0[i] = second ? bo[i] : bcl[i];
+            b1[i] = second ? bcl[i] : bo[i];
+            tt0[i] = second ? to[i] : tcl[i];
+            tt1[i] = second ? tcl[i] : to[i];
+          }
           if (td4) {
-            g0 = rd<N>(s.gbar_c + off, a0, L, k);
-            g1 = rd<N>(s.gbar_c + off, a1, L, k);
-            G0 = rd<N>(d.trdel2_c + off, a0, L, k);
-            G1 = rd<N>(d.trdel2_c + off, a1, L, k);
+            const Vd<N> go = rd<N>(s.gbar_c + off, o, L, k), Go = rd<N>(d.trdel2_c + off, o, L, k);
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+              g0[i] = second ? go[i] : gcl[i];
+              g1[i] = second ? gcl[i] : go[i];
+              G0[i] = second ? Go[i] : Gcl[i];
+              G1[i] = second ? Gcl[i] : Go[i];
+            }
           }
           const double cdc = m.cdiv[c * me + j];
 #pragma unroll
@@ -463,7 +516,7 @@ __global__ void __launch_bounds__(kAB)
             const double F = 0.5 * (h0[i] + h1[i]);
             const double w = cd0 * b0[i] + cd1 * b1[i];
             nb[i] += 0.5 * (-w * F * ue[i]);
-            const double grad = (tt1[i] * (1.0 / h1[i]) - tt0[i] * (1.0 / h0[i])) * idc;
+            const double grad = (tt1[i] * ih1[i] - tt0[i] * ih0[i]) * idc;
             if (td2) {
               const double w2 = ac.ect2[2 * e] * b0[i] + ac.ect2[2 * e + 1] * b1[i];
               nb[i] += sg * w2 * F * m.s2[e] * idc;
@@ -479,7 +532,7 @@ __global__ void __launch_bounds__(kAB)
         Vd<N> acc = rd<N>(a.c.otr + off, c, L, k);
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-          const double ih = 1.0 / hc[i];
+          const double ih = ihc[i];
           acc[i] += nb[i] * ih;
           hb[i] -= nb[i] * tcl[i] * ih * ih;
           p_t2 += bcl[i] * t2[i];
@@ -644,16 +697,26 @@ void launch_tendency_vjp(const MeshView &m, const AdjMeshView &am, const Launch 
   const int swz = lc.swizzle;
   const bool vd4 = a.do_vn && (a.terms & T_VDEL4);
   const bool pv = a.do_vn && (a.terms & T_PV);
+  // regular hexagonal meshes: constant neighbour counts, unrolled loops
 #define OMEGA_ADJ_LAUNCH(K, ...)                                                                \
   do {                                                                                         \
-    if (g.N == 2) hipLaunchKernelGGL(K<2>, g.grid, g.block, 0, st, __VA_ARGS__);               \
-    else hipLaunchKernelGGL(K<1>, g.grid, g.block, 0, st, __VA_ARGS__);                        \
+    if (g.N == 2 && am.fixed)                                                                  \
+      hipLaunchKernelGGL((K<2, 1>), g.grid, g.block, 0, st, __VA_ARGS__);                       \
+    else if (g.N == 2)                                                                         \
+      hipLaunchKernelGGL((K<2, 0>), g.grid, g.block, 0, st, __VA_ARGS__);                       \
+    else if (am.fixed)                                                                         \
+      hipLaunchKernelGGL((K<1, 1>), g.grid, g.block, 0, st, __VA_ARGS__);                       \
+    else                                                                                       \
+      hipLaunchKernelGGL((K<1, 0>), g.grid, g.block, 0, st, __VA_ARGS__);                       \
   } while (0)
   if (vd4) {
     OMEGA_ADJ_LAUNCH(k_adj_v4a, m, mc, a, s, swz);
     OMEGA_ADJ_LAUNCH(k_adj_v4b, m, am, s, swz);
   }
-  if (pv) OMEGA_ADJ_LAUNCH(k_adj_pv, m, am, mc, a, d, s, swz);
+  if (pv) {
+    OMEGA_ADJ_LAUNCH(k_adj_pre, m, mc, a, d, s, swz);
+    OMEGA_ADJ_LAUNCH(k_adj_pv, m, am, mc, a, d, s, swz);
+  }
   OMEGA_ADJ_LAUNCH(k_adj_vc, m, mc, ac, a, d, s, swz);
   OMEGA_ADJ_LAUNCH(k_adj_edge, m, am, mc, ac, a, d, s, swz);
   OMEGA_ADJ_LAUNCH(k_adj_cell, m, am, mc, ac, a, d, s, swz);

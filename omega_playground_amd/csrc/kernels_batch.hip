@@ -181,11 +181,32 @@ __device__ __forceinline__ void batch_diag_pair(const BatchMember &B, const Pair
   hexops::cell_diag_p<U, W>(m, B.dterms, B.a.ntr, q.cA, q.cB, q.hi, q.k, h, vn, tr, d);
 }
 
+// members with 16 or 32 layers in the N = 2 launches: one cell per row of
+// L/2 lanes (hex_ops.hpp single-cell operators)
+struct Row {
+  int c, k;
+};
+__device__ __forceinline__ Row row_of(const BatchMember &B, int lb) {
+  const int chunk = lb % B.chunks, seg = lb / B.chunks;
+  return Row{seg * B.rows + (int)threadIdx.x / B.lx, (chunk * B.lx + (int)threadIdx.x % B.lx) * 2};
+}
+
 __global__ void __launch_bounds__(256)
     k_batch_diag_hex2(const BatchMember *__restrict__ desc, const int *__restrict__ blk) {
   const int id = uniform(blk[blockIdx.x]);
   const BatchMember &B = desc[id];
   if (!B.dterms) return;
+  if (B.m.L % 64 != 0) {
+    const Row r = row_of(B, (int)blockIdx.x - B.block0);
+    if (r.c >= B.m.nc) return;
+    const double *h = B.a.h.in, *vn = B.a.vn.in, *tr = B.a.tr.in;
+    if (B.dterms & (T_PV | T_VDEL2 | T_VDEL4)) {
+      hexops::vertex_diag_u<2>(B.m, 2 * r.c, r.k, h, vn, B.d);
+      hexops::vertex_diag_u<2>(B.m, 2 * r.c + 1, r.k, h, vn, B.d);
+    }
+    hexops::cell_diag_u<2>(B.m, B.dterms, B.a.ntr, r.c, r.k, h, vn, tr, B.d);
+    return;
+  }
   const Pair q = pair_of(B, (int)blockIdx.x - B.block0);
   if (!q.ok) return;
   if (B.m.coef_uni && B.m.off32) batch_diag_pair<true, false>(B, q);
@@ -217,6 +238,15 @@ __global__ void __launch_bounds__(256)
     k_batch_tend_hex2(const BatchMember *__restrict__ desc, const int *__restrict__ blk) {
   const int id = uniform(blk[blockIdx.x]);
   const BatchMember &B = desc[id];
+  if (B.m.L % 64 != 0) {
+    const Row r = row_of(B, (int)blockIdx.x - B.block0);
+    if (r.c >= B.m.nc) return;
+    hexops::edge_tend_hex<2, 0>(B.m, B.mc, B.a, B.d, 3 * r.c, r.k);
+    hexops::edge_tend_hex<2, 1>(B.m, B.mc, B.a, B.d, 3 * r.c + 1, r.k);
+    hexops::edge_tend_hex<2, 2>(B.m, B.mc, B.a, B.d, 3 * r.c + 2, r.k);
+    hexops::cell_tend_hex<2>(B.m, B.mc, B.a, B.d, r.c, r.k);
+    return;
+  }
   const Pair q = pair_of(B, (int)blockIdx.x - B.block0);
   if (!q.ok) return;
   if (B.m.coef_uni && B.mc.uniform && B.m.off32) batch_tend_pair<true, false>(B, q);

@@ -349,14 +349,20 @@ __global__ void __launch_bounds__(256) k_tend_hexu(MeshView m, ModelCoef mc, Sta
 
 // lanes per row = 64 / N (one row covers 64 levels); blockIdx.y walks the
 // 64-level chunks
+// the two-cells-per-wavefront kernels need rows of 64 levels
+bool hex_pair(const MeshView &m, int N) { return N == 2 && m.L % 64 == 0; }
+
+// One cell per row of lanes covering 64 levels (L a multiple of 64) or all
+// L levels (L = 16 or 32: 4 or 8 cells per wavefront, N = 2).
 dim3 hex_grid(const MeshView &m, const Launch &lc, int N, dim3 &block) {
-  const int lanes = 64 / N;
+  const int span = m.L % 64 == 0 ? 64 : m.L;
+  const int lanes = span / N;
   int rows = lc.rows > 0 ? lc.rows : 256 / lanes;
   if (rows * lanes > 256) rows = 256 / lanes;
-  // N = 2: an even row count keeps the two cells of a wavefront adjacent
-  if (N == 2 && rows % 2) rows = rows > 1 ? rows - 1 : 2;
+  // pair kernels: an even row count keeps the two cells of a wavefront adjacent
+  if (hex_pair(m, N) && rows % 2) rows = rows > 1 ? rows - 1 : 2;
   block = dim3(lanes, rows);
-  return dim3((m.nc + rows - 1) / rows, m.L / 64);
+  return dim3((m.nc + rows - 1) / rows, m.L / span);
 }
 
 int hex_n(const MeshView &m, const Launch &lc) { return lc.lpt == 1 ? 1 : 2; }
@@ -364,7 +370,9 @@ int hex_n(const MeshView &m, const Launch &lc) { return lc.lpt == 1 ? 1 : 2; }
 } // namespace
 
 bool hex_usable(const MeshView &m, const Launch &lc) {
-  return m.hex_ok && (m.L % 64 == 0) && lc.variant != 1;
+  // 16 or 32 layers: several cells per wavefront, two levels per lane
+  const bool rows_ok = m.L % 64 == 0 || ((m.L == 16 || m.L == 32) && lc.lpt != 1);
+  return m.hex_ok && rows_ok && lc.variant != 1;
 }
 
 bool hexu_usable(const MeshView &m, const Launch &lc) {
@@ -385,12 +393,14 @@ void launch_diag_hex(const MeshView &m, const Launch &lc, int terms, int ntr, co
   dim3 block;
   const int N = hex_n(m, lc);
   const dim3 grid = hex_grid(m, lc, N, block);
-  if (N == 2) {
+  if (hex_pair(m, N)) {
     // the uniform-coefficient and 32-bit-offset forms give the same bits
     auto kern = m.coef_uni ? (m.off32 ? k_diag_hex2<true, false> : k_diag_hex2<true, true>)
                            : (m.off32 ? k_diag_hex2<false, false> : k_diag_hex2<false, true>);
     hipLaunchKernelGGL(kern, grid, block, 0, s, m, terms, ntr, h, vn, tr, d, lc.swizzle);
-  } else
+  } else if (N == 2)
+    hipLaunchKernelGGL(k_diag_hex<2>, grid, block, 0, s, m, terms, ntr, h, vn, tr, d, lc.swizzle);
+  else
     hipLaunchKernelGGL(k_diag_hex<1>, grid, block, 0, s, m, terms, ntr, h, vn, tr, d, lc.swizzle);
   OMEGA_HIP_CHECK(hipGetLastError());
 }
@@ -414,15 +424,17 @@ void launch_tend_hex(const MeshView &m, const Launch &lc, const ModelCoef &mc, c
   const int N = hex_n(m, lc);
   const dim3 grid = hex_grid(m, lc, N, block);
   const bool U = m.coef_uni && mc.uniform;
-  if (N == 2 && m.hexu_ok) {
+  if (hex_pair(m, N) && m.hexu_ok) {
     auto kern = U ? (m.off32 ? k_tend_hex2v<true, false> : k_tend_hex2v<true, true>)
                   : (m.off32 ? k_tend_hex2v<false, false> : k_tend_hex2v<false, true>);
     hipLaunchKernelGGL(kern, grid, block, 0, s, m, mc, a, d, lc.swizzle);
-  } else if (N == 2) {
+  } else if (hex_pair(m, N)) {
     auto kern = U ? (m.off32 ? k_tend_hex2<true, false> : k_tend_hex2<true, true>)
                   : (m.off32 ? k_tend_hex2<false, false> : k_tend_hex2<false, true>);
     hipLaunchKernelGGL(kern, grid, block, 0, s, m, mc, a, d, lc.swizzle);
-  } else
+  } else if (N == 2)
+    hipLaunchKernelGGL(k_tend_hex<2>, grid, block, 0, s, m, mc, a, d, lc.swizzle);
+  else
     hipLaunchKernelGGL(k_tend_hex<1>, grid, block, 0, s, m, mc, a, d, lc.swizzle);
   OMEGA_HIP_CHECK(hipGetLastError());
 }

@@ -20,6 +20,8 @@ MEMBERS = [
     (12, 8, 2, 0, 5e3, dict(visc_del4=0.0), 10.0),
     (6, 6, 8, 2, 20e3, dict(disable_vn_tendency=True), 30.0),
     (8, 8, 64, 1, 10e3, dict(visc_del2=0.0), 12.0),
+    (10, 8, 32, 2, 10e3, {}, 14.0),
+    (8, 6, 16, 1, 10e3, dict(visc_del4=0.0), 18.0),
 ]
 
 

@@ -209,6 +209,48 @@ def test_specialised_kernels_bitwise_L64(variant, lpt, ordering):
             assert same_bits(a, b), (kind, name)
 
 
+@pytest.mark.parametrize("L", [16, 32])
+@pytest.mark.parametrize("ordering", ["hilbert", "original"])
+def test_specialised_kernels_16_32_layers(L, ordering):
+    """16 or 32 layers run the pattern-specialised kernels with several
+    cells per wavefront (one cell per row of L/2 lanes, two levels per
+    lane): the oracle's bits for a tendency and for RK4 / LSRK steps."""
+    om, omodel, gm, gmodel, state = make_pair(12, 10, L, 3, ordering=ordering)
+    assert gmodel.kernel_variant() == 2
+    print("case", L, ordering, flush=True)
+    tend_o = omodel.new_state()
+    omodel.compute_tendency(tend_o, state, 0.0)
+    gt = swe.ShallowWaterState(gm, 3)
+    gmodel.compute_tendency(gt, upload(gm, 3, state), 0.0)
+    for name, a, b in zip(("h", "vn", "tr"), gt.download(), tend_o):
+        assert same_bits(a, b), name
+    for kind in ("rk4", "lsrk"):
+        ost = tuple(a.copy() for a in state)
+        po.Stepper(omodel, kind, 5).run(0.0, 45.0, 3, ost)
+        gs = upload(gm, 3, state)
+        gst = swe.RK4Stepper(gmodel) if kind == "rk4" else swe.LSRKStepper(gmodel, 5)
+        gst.set_graph(True)
+        gst.run(0.0, 45.0, 3, gs)
+        for name, a, b in zip(("h", "vn", "tr"), gs.download(), ost):
+            assert same_bits(a, b), (kind, name)
+    # one level per lane keeps the general kernels
+    gmodel.set_launch(0, 1)
+    assert gmodel.kernel_variant() == 1
+
+
+@pytest.mark.parametrize("terms", [1, 2, 4, 8, 16, 32, 64])
+def test_specialised_32_layers_each_term_bitwise(terms):
+    om, omodel, gm, gmodel, state = make_pair(10, 8, 32, 2, seed=terms)
+    omodel.terms = terms
+    gmodel.terms = terms
+    tend_o = omodel.new_state()
+    omodel.compute_tendency(tend_o, state, 0.0)
+    gt = swe.ShallowWaterState(gm, 2)
+    gmodel.compute_tendency(gt, upload(gm, 2, state), 0.0)
+    for name, a, b in zip(("h", "vn", "tr"), gt.download(), tend_o):
+        assert same_bits(a, b), name
+
+
 @pytest.mark.parametrize("perturb,forms", [("none", 3), ("none", 2), ("none", 1), ("none", 0),
                                            ("scalings", 3), ("f_vertex", 3), ("weights", 3)])
 def test_specialised_kernel_forms_bitwise(perturb, forms):
