@@ -161,7 +161,7 @@ AdjScratch scratch(omega_model_s *M) {
 // out += J(y)^T k for the tendency at state y (pointers in device numbering);
 // the mixing-coefficient gradient accumulates in M->a_pgrad.
 void tendency_vjp_ptrs(omega_model_s *M, const Real *const y[3], const Real *const k[3],
-                       Real *const out[3], hipStream_t s) {
+                       Real *const out[3], hipStream_t s, const AdjEpi &epi = AdjEpi{}) {
   run_diag(M, y[0], y[1], y[2], s);
   AdjArgs a{};
   a.terms = M->terms;
@@ -174,6 +174,7 @@ void tendency_vjp_ptrs(omega_model_s *M, const Real *const y[3], const Real *con
   a.tr = y[2];
   a.c = CotIO{k[0], k[1], k[2], out[0], out[1], out[2]};
   a.nparts = M->a_nparts;
+  a.epi = epi;
   const AdjCoef ac{M->ect2.p, M->ect4.p};
   launch_tendency_vjp(M->mesh->dm.view(), M->mesh->adj.view(), M->lc, M->coef(), ac, a,
                       M->diag(), scratch(M), s);
@@ -242,20 +243,28 @@ void step_vjp_ptrs(omega_stepper_s *St, const Real *const y[3], Real *const cot[
       a.tr = fio(in[2], nullptr, nullptr, y[2], out[2]);
       run_stage(M, a, s);
     }
-    const auto gy = P(St->adj_gy), pb = P(St->adj_pb), kb = P(St->adj_kb);
+    // reverse sweep: Cbar += J(P_s)^T kbar_s and, in the same kernels,
+    // kbar_{s-1} = dt*b_{s-1}*gy + dt*a_{s-1}*J(P_s)^T kbar_s (two kbar
+    // buffers alternate, since each VJP reads its kbar at neighbours)
+    const auto gy = P(St->adj_gy);
+    std::array<Real *, 3> kin = P(St->adj_kb), kout = P(St->adj_pb);
     copy_fields(gy.data(), C.data(), n, s);
-    for (int f = 0; f < 3; ++f) launch_lincomb(kb[f], dt * St->rkb[3], gy[f], 0.0, gy[f], n[f], s);
+    for (int f = 0; f < 3; ++f) launch_lincomb(kin[f], dt * St->rkb[3], gy[f], 0.0, gy[f], n[f], s);
     for (int st = 3; st >= 1; --st) {
-      for (int f = 0; f < 3; ++f) launch_fill(pb[f], 0.0, n[f], s);
       const auto ps = K(St->adj_y[st - 1]);
-      const std::array<const Real *, 3> kc{kb[0], kb[1], kb[2]};
-      tendency_vjp_ptrs(M, ps.data(), kc.data(), pb.data(), s);
+      const std::array<const Real *, 3> kc{kin[0], kin[1], kin[2]};
+      AdjEpi epi{};
+      epi.rk4 = 1;
+      epi.db = dt * St->rkb[st - 1];
+      epi.da = dt * St->rka[st - 1];
       for (int f = 0; f < 3; ++f) {
-        launch_axpy(C[f], 1.0, pb[f], n[f], s);
-        launch_lincomb(kb[f], dt * St->rkb[st - 1], gy[f], dt * St->rka[st - 1], pb[f], n[f], s);
+        epi.gy[f] = gy[f];
+        epi.kbo[f] = kout[f];
       }
+      tendency_vjp_ptrs(M, ps.data(), kc.data(), C.data(), s, epi);
+      std::swap(kin, kout);
     }
-    const std::array<const Real *, 3> kc{kb[0], kb[1], kb[2]};
+    const std::array<const Real *, 3> kc{kin[0], kin[1], kin[2]};
     tendency_vjp_ptrs(M, Y.data(), kc.data(), C.data(), s);
   } else {
     // stage inputs Y_1..Y_{S-1} (lsrk.cpp:36-89: tend = a_s*tend + F(Y_s),

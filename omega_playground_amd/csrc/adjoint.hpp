@@ -68,12 +68,23 @@ struct AdjScratch {
   Real *pb_e;     // [ne][L] sum over t listing e of w_te kbar_t
 };
 
+// What the VJP does with its result v for each state element: always
+// out += v; with rk4 set also kbo = db*gy + da*v, the next tendency
+// cotangent of the RK4 reverse sweep (kbo must not alias the input k).
+struct AdjEpi {
+  int rk4;
+  Real db, da;
+  const Real *gy[3];
+  Real *kbo[3];
+};
+
 struct AdjArgs {
   int terms, do_h, do_vn, ntr;
   Real grav;
   const Real *h, *vn, *tr;  // state at which the tendency is linearised
   CotIO c;
   int nparts;               // partial slots available per parameter
+  AdjEpi epi;
 };
 
 void launch_tendency_vjp(const MeshView &m, const AdjMeshView &am, const Launch &lc,

@@ -79,6 +79,23 @@ template <int N> __device__ __forceinline__ Seg segment(const MeshView &m, int s
   return g;
 }
 
+// out[idx] += v; for the RK4 reverse sweep also kbo[idx] = db*gy[idx] + da*v
+template <int N>
+__device__ __forceinline__ void adj_out(const AdjArgs &a, int f, Real *out, size_t idx,
+                                        const Vd<N> &v) {
+  Vd<N> acc = ld<N>(out + idx);
+#pragma unroll
+  for (int i = 0; i < N; ++i) acc[i] += v[i];
+  st<N>(out + idx, acc);
+  if (a.epi.rk4) {
+    const Vd<N> g = ld<N>(a.epi.gy[f] + idx);
+    Vd<N> kn;
+#pragma unroll
+    for (int i = 0; i < N; ++i) kn[i] = a.epi.db * g[i] + a.epi.da * v[i];
+    st<N>(a.epi.kbo[f] + idx, kn);
+  }
+}
+
 template <int N>
 __device__ __forceinline__ double mask(int k, int i, int ml, double c) {
   return (k + i < ml) ? c : 0.0;
@@ -411,10 +428,7 @@ __global__ void __launch_bounds__(kAB)
                     ((D1[i] - D0[i]) * idc - (Z1[i] - Z0[i]) * m.inv_dv[e]);
         }
       }
-      Vd<N> acc = rd<N>(a.c.ovn, e, L, k);
-#pragma unroll
-      for (int i = 0; i < N; ++i) acc[i] += ub[i];
-      st<N>(a.c.ovn + (size_t)e * L + k, acc);
+      adj_out<N>(a, 1, a.c.ovn, (size_t)e * L + k, ub);
       st<N>(s.fbar_e + (size_t)e * L + k, fb);
     }
   }
@@ -529,21 +543,18 @@ __global__ void __launch_bounds__(kAB)
             }
           }
         }
-        Vd<N> acc = rd<N>(a.c.otr + off, c, L, k);
+        Vd<N> vt;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
           const double ih = ihc[i];
-          acc[i] += nb[i] * ih;
+          vt[i] = nb[i] * ih;
           hb[i] -= nb[i] * tcl[i] * ih * ih;
           p_t2 += bcl[i] * t2[i];
           p_t4 += bcl[i] * t4[i];
         }
-        st<N>(a.c.otr + off + (size_t)c * L + k, acc);
+        adj_out<N>(a, 2, a.c.otr, off + (size_t)c * L + k, vt);
       }
-      Vd<N> acc = rd<N>(a.c.oh, c, L, k);
-#pragma unroll
-      for (int i = 0; i < N; ++i) acc[i] += hb[i];
-      st<N>(a.c.oh + (size_t)c * L + k, acc);
+      adj_out<N>(a, 0, a.c.oh, (size_t)c * L + k, hb);
     }
   }
   const int b = blockIdx.y * gridDim.x + blockIdx.x;
