@@ -313,13 +313,21 @@ __global__ void __launch_bounds__(kAB)
     if (td4) {
       for (int l = 0; l < a.ntr; ++l) {
         const double *tl = a.c.tr + (size_t)l * NC * L;
+        const Vd<N> tc = rd<N>(tl, c, L, k);
         Vd<N> gb = zero<N>();
 #pragma unroll
         for (int j = 0; j < n; ++j) {
           const int e = m.eoc[c * m.me + j];
           const int a0 = m.coe[2 * e], a1 = m.coe[2 * e + 1];
-          const double sg = (a1 == c) ? -1.0 : 1.0;
-          const Vd<N> t0 = rd<N>(tl, a0, L, k), t1 = rd<N>(tl, a1, L, k);
+          const bool second = a1 == c;
+          const double sg = second ? -1.0 : 1.0;
+          const Vd<N> to = rd<N>(tl, second ? a0 : a1, L, k);
+          Vd<N> t0, t1;
+#pragma unroll
+          for (int i = 0; i < N; ++i) {
+            t0[i] = second ? to[i] : tc[i];
+            t1[i] = second ? tc[i] : to[i];
+          }
 #pragma unroll
           for (int i = 0; i < N; ++i)
             gb[i] += sg * (ac.ect4[2 * e] * t0[i] + ac.ect4[2 * e + 1] * t1[i]) * m.inv_dc[e];
@@ -349,9 +357,16 @@ __global__ void __launch_bounds__(kAB)
       const double cd0 = am.ecd[2 * e], cd1 = am.ecd[2 * e + 1], idc = m.inv_dc[e];
       const Vd<N> ue = rd<N>(a.vn, e, L, k);
       const Vd<N> h0 = rd<N>(a.h, c0, L, k), h1 = rd<N>(a.h, c1, L, k);
-      Vd<N> F, ub = zero<N>(), fb = zero<N>();
+      Vd<N> F, ih0, ih1, ub = zero<N>(), fb = zero<N>();
 #pragma unroll
       for (int i = 0; i < N; ++i) F[i] = 0.5 * (h0[i] + h1[i]);
+      if (a.ntr > 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          ih0[i] = 1.0 / h0[i];
+          ih1[i] = 1.0 / h1[i];
+        }
+      }
       if (a.do_h) {
         const Vd<N> g0 = rd<N>(a.c.h, c0, L, k), g1 = rd<N>(a.c.h, c1, L, k);
 #pragma unroll
@@ -372,7 +387,7 @@ __global__ void __launch_bounds__(kAB)
         }
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-          const double n0 = t0[i] * (1.0 / h0[i]), n1 = t1[i] * (1.0 / h1[i]);
+          const double n0 = t0[i] * ih0[i], n1 = t1[i] * ih1[i];
           const double ne = (n0 + n1) * 0.5, grad = (n1 - n0) * idc;
           const double w = cd0 * b0[i] + cd1 * b1[i];
           ub[i] -= w * F[i] * ne;
