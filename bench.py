@@ -79,7 +79,7 @@ def cpu_baseline(cfg, sample_steps: int, threads: int):
     }
 
 
-def adjoint_bench(n_snapshots: int = 10, nsteps: int | None = None):
+def adjoint_bench(n_snapshots: int = 10, nsteps: int | None = None, graph: bool = True):
     """The adjoint workload: RK4 on nx=256, 32 layers, 2 tracers, del2
     viscosity on; misfit of h and tracers after the trajectory against
     observations from a run whose initial h was perturbed by a correlated
@@ -94,7 +94,7 @@ def adjoint_bench(n_snapshots: int = 10, nsteps: int | None = None):
     ntr, L, nc = cfg["ntracers"], cfg["nlayers"], mesh.ncells
     dt = wl.dt_rule(cfg["dc"], cfg["grav"])
     st = swe.RK4Stepper(model)
-    st.set_graph(True)
+    st.set_graph(graph)
     y0 = wl.make_state(mesh, cfg, seed=0)
     obs = swe.ShallowWaterState(mesh, ntr)
     obs.upload(*wl.adjoint_observations(mesh, cfg, seed=0))
@@ -118,7 +118,7 @@ def adjoint_bench(n_snapshots: int = 10, nsteps: int | None = None):
         swe.step_vjp(st, 0.0, dt, state, cot)
     swe.synchronize()
     t_adj = (time.perf_counter() - t0) / reps
-    st.set_graph(True)
+    st.set_graph(graph)
     # the whole trajectory: forward sweep + checkpointed reverse sweep
     state.upload(*y0)
     grad = swe.ShallowWaterState(mesh, ntr)
@@ -434,7 +434,7 @@ def main():
     adj = None
     if not args.skip_adjoint:
         log("[bench] adjoint workload")
-        adj = adjoint_bench()
+        adj = adjoint_bench(graph=not args.no_graph)
         log(f"[bench] adjoint: step ratio {adj['ratio_per_step']:.2f}, trajectory ratio "
             f"{adj['ratio_total']:.2f} ({adj['adjoint_run_s']:.3f} s)")
 
