@@ -327,8 +327,8 @@ def test_misfit_matches_numpy():
     assert abs(J - ref_J) <= 1e-13 * abs(ref_J)
 
 
-def _c3_like_case():
-    gm, gmodel, pd, state = setup(12, 12, 4, 2, seed=9)
+def _c3_like_case(L=4):
+    gm, gmodel, pd, state = setup(12, 12, L, 2, seed=9)
     st = swe.RK4Stepper(gmodel)
     dt, n, w = 40.0, 16, (1.0, 0.0, 1.0)
     # observations: the forward run from a perturbed initial thickness
@@ -383,6 +383,30 @@ def test_adjoint_run_finite_differences():
         fd = _richardson(J_param, 1e-3) / pd[name]
         print(name, fd, gp[idx], (fd - gp[idx]) / gp[idx], flush=True)
         assert abs(fd - gp[idx]) <= TOL_FD_RUN * abs(gp[idx]), (name, fd, gp[idx])
+
+
+def test_adjoint_run_finite_differences_32_layers():
+    """The same check with 32 layers, where the forward steps (and the
+    adjoint's stage recomputation) run the pattern kernels with four cells
+    per wavefront."""
+    gm, gmodel, pd, state, st, dt, n, w, obs = _c3_like_case(32)
+    assert gmodel.kernel_variant() == 2
+    grad = swe.ShallowWaterState(gm, 2)
+    J, gp = swe.adjoint_run(st, 0.0, dt, n, upload(gm, 2, state), obs, grad, w, n_snapshots=4)
+    g = grad.download()
+
+    def J_of(y):
+        s = upload(gm, 2, y)
+        swe.RK4Stepper(gmodel).run(0.0, dt, n, s)
+        return swe.misfit(gmodel, s, obs, swe.ShallowWaterState(gm, 2), w)
+
+    assert abs(J_of(state) - J) <= 1e-14 * J
+    for k in range(2):
+        d = random_like(state, 200 + k, (1e-3, 1e-6, 1e-3))
+        fd = _richardson(lambda e: J_of(tuple(a + e * b for a, b in zip(state, d))), 1.0)
+        ad = dot(g, d)
+        print("direction", k, fd, ad, (fd - ad) / ad, flush=True)
+        assert abs(fd - ad) <= TOL_FD_RUN * abs(ad), (k, fd, ad)
 
 
 def test_adjoint_run_matches_autograd():
