@@ -163,29 +163,33 @@ __global__ void __launch_bounds__(kAB)
 // PV flux: cotangent of S_e = Q_e + P_e, from the edge's own sum and from the
 // sums of the edges that list it (transposed edges_on_edge)
 // ---------------------------------------------------------------------------
-// Per edge at the linearisation state: the PV sum S_e = Q_e + P_e and the
-// thickness flux F_e u_e, read by the PV pass for every edge whose
-// edges-on-edge list holds e and by the edge pass.
+// Per edge at the linearisation state: the PV sum S_e = Q_e + P_e (PV term
+// only) and the thickness flux F_e u_e, read by the PV pass for every edge
+// whose edges-on-edge list holds e, by the edge pass and by the cell pass.
 template <int N, int H>
 __global__ void __launch_bounds__(kAB)
     k_adj_pre(MeshView m, ModelCoef mc, AdjArgs a, DiagOut d, AdjScratch s, int swz) {
   const Seg g = segment<N>(m, swz);
   if (!g.active) return;
   const int L = m.L, k = g.k;
+  const bool pv = a.do_vn && (a.terms & T_PV);
   for (int e = m.eofs[g.c0] + (int)threadIdx.y; e < m.eofs[g.c1]; e += blockDim.y) {
-    const int v0 = m.voe[2 * e], v1 = m.voe[2 * e + 1];
-    const double fv0 = mc.f_v[v0], fv1 = mc.f_v[v1];
-    const Vd<N> i0 = rd<N>(d.inv_thick_v, v0, L, k), i1 = rd<N>(d.inv_thick_v, v1, L, k);
-    const Vd<N> z0 = rd<N>(d.rvort_v, v0, L, k), z1 = rd<N>(d.rvort_v, v1, L, k);
+    if (pv) {
+      const int v0 = m.voe[2 * e], v1 = m.voe[2 * e + 1];
+      const double fv0 = mc.f_v[v0], fv1 = mc.f_v[v1];
+      const Vd<N> i0 = rd<N>(d.inv_thick_v, v0, L, k), i1 = rd<N>(d.inv_thick_v, v1, L, k);
+      const Vd<N> z0 = rd<N>(d.rvort_v, v0, L, k), z1 = rd<N>(d.rvort_v, v1, L, k);
+      Vd<N> Se;
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+        Se[i] = 0.5 * (i0[i] * z0[i] + i1[i] * z1[i]) + 0.5 * (i0[i] * fv0 + i1[i] * fv1);
+      st<N>(s.se_e + (size_t)e * L + k, Se);
+    }
     const Vd<N> h0 = rd<N>(a.h, m.coe[2 * e], L, k), h1 = rd<N>(a.h, m.coe[2 * e + 1], L, k);
     const Vd<N> ue = rd<N>(a.vn, e, L, k);
-    Vd<N> Se, FU;
+    Vd<N> FU;
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-      Se[i] = 0.5 * (i0[i] * z0[i] + i1[i] * z1[i]) + 0.5 * (i0[i] * fv0 + i1[i] * fv1);
-      FU[i] = 0.5 * (h0[i] + h1[i]) * ue[i];
-    }
-    st<N>(s.se_e + (size_t)e * L + k, Se);
+    for (int i = 0; i < N; ++i) FU[i] = 0.5 * (h0[i] + h1[i]) * ue[i];
     st<N>(s.fu_e + (size_t)e * L + k, FU);
   }
 }
@@ -488,7 +492,10 @@ __global__ void __launch_bounds__(kAB)
         }
       }
       // the cell's own rows are loaded once; each edge then loads only the
-      // rows of the cell across it (same values, same expressions)
+      // rows of the cell across it.  The advection term reads the edge's
+      // thickness flux F_e u_e (k_adj_pre); thicknesses and tracers across
+      // the edge are needed only by the diffusion terms.
+      const bool mix = td2 || td4;
       Vd<N> ihc;
 #pragma unroll
       for (int i = 0; i < N; ++i) ihc[i] = 1.0 / hc[i];
@@ -510,24 +517,34 @@ __global__ void __launch_bounds__(kAB)
           const int o = second ? a0 : a1;
           const double sg = second ? 1.0 : -1.0;
           const double cd0 = am.ecd[2 * e], cd1 = am.ecd[2 * e + 1], idc = m.inv_dc[e];
-          const Vd<N> ho = rd<N>(a.h, o, L, k);
-          const Vd<N> ue = rd<N>(a.vn, e, L, k);
+          const Vd<N> fu = rd<N>(s.fu_e, e, L, k);
           const Vd<N> bo = rd<N>(a.c.tr + off, o, L, k);
-          const Vd<N> to = rd<N>(a.tr + off, o, L, k);
-          Vd<N> h0, h1, b0, b1, tt0, tt1, g0 = zero<N>(), g1 = zero<N>(), G0 = zero<N>(),
-                                         G1 = zero<N>(), ih0, ih1;
-          Vd<N> iho;
+          Vd<N> b0, b1, h0 = zero<N>(), h1 = zero<N>(), tt0 = zero<N>(), tt1 = zero<N>(),
+                        g0 = zero<N>(), g1 = zero<N>(), G0 = zero<N>(), G1 = zero<N>(),
+                        ih0 = zero<N>(), ih1 = zero<N>();
 #pragma unroll
           for (int i = 0; i < N; ++i) {
-            iho[i] = 1.0 / ho[i];
-            h0[i] = second ? ho[i] : hc[i];
-            h1[i] = second ? hc[i] : ho[i];
-            ih0[i] = second ? iho[i] : ihc[i];
-            ih1[i] = second ? ihc[i] : iho[i];
             b0[i] = second ? bo[i] : bcl[i];
             b1[i] = second ? bcl[i] : bo[i];
-            tt0[i] = second ? to[i] : tcl[i];
-            tt1[i] = second ? tcl[i] : to[i];
+          }
+          if (mix) {
+            const Vd<N> ho = rd<N>(a.h, o, L, k);
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+              h0[i] = second ? ho[i] : hc[i];
+              h1[i] = second ? hc[i] : ho[i];
+            }
+          }
+          if (td2) {
+            const Vd<N> to = rd<N>(a.tr + off, o, L, k);
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+              const double iho = 1.0 / (second ? h0[i] : h1[i]);
+              ih0[i] = second ? iho : ihc[i];
+              ih1[i] = second ? ihc[i] : iho;
+              tt0[i] = second ? to[i] : tcl[i];
+              tt1[i] = second ? tcl[i] : to[i];
+            }
           }
           if (td4) {
             const Vd<N> go = rd<N>(s.gbar_c + off, o, L, k), Go = rd<N>(d.trdel2_c + off, o, L, k);
@@ -544,9 +561,9 @@ __global__ void __launch_bounds__(kAB)
           for (int i = 0; i < N; ++i) {
             const double F = 0.5 * (h0[i] + h1[i]);
             const double w = cd0 * b0[i] + cd1 * b1[i];
-            nb[i] += 0.5 * (-w * F * ue[i]);
-            const double grad = (tt1[i] * ih1[i] - tt0[i] * ih0[i]) * idc;
+            nb[i] += 0.5 * (-w * fu[i]);
             if (td2) {
+              const double grad = (tt1[i] * ih1[i] - tt0[i] * ih0[i]) * idc;
               const double w2 = ac.ect2[2 * e] * b0[i] + ac.ect2[2 * e + 1] * b1[i];
               nb[i] += sg * w2 * F * m.s2[e] * idc;
               t2[i] += cdc * F * m.s2[e] * grad;
@@ -739,10 +756,9 @@ void launch_tendency_vjp(const MeshView &m, const AdjMeshView &am, const Launch 
     OMEGA_ADJ_LAUNCH(k_adj_v4a, m, mc, a, s, swz);
     OMEGA_ADJ_LAUNCH(k_adj_v4b, m, am, s, swz);
   }
-  if (pv) {
-    OMEGA_ADJ_LAUNCH(k_adj_pre, m, mc, a, d, s, swz);
-    OMEGA_ADJ_LAUNCH(k_adj_pv, m, am, mc, a, d, s, swz);
-  }
+  // edge PV sums (PV term) and thickness fluxes (PV term, tracer advection)
+  if (pv || a.ntr > 0) OMEGA_ADJ_LAUNCH(k_adj_pre, m, mc, a, d, s, swz);
+  if (pv) OMEGA_ADJ_LAUNCH(k_adj_pv, m, am, mc, a, d, s, swz);
   OMEGA_ADJ_LAUNCH(k_adj_vc, m, mc, ac, a, d, s, swz);
   OMEGA_ADJ_LAUNCH(k_adj_edge, m, am, mc, ac, a, d, s, swz);
   OMEGA_ADJ_LAUNCH(k_adj_cell, m, am, mc, ac, a, d, s, swz);
