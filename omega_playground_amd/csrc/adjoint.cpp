@@ -134,6 +134,7 @@ void ensure_adjoint(omega_model_s *M) {
     M->a_fu.alloc(NE * L);
     M->a_pa.alloc(NE * L);
     M->a_pb.alloc(NE * L);
+    if (ntr) M->a_xa.alloc(ntr * NE * L);
     M->a_sbar.alloc(NE * L);
     M->a_fbar.alloc(NE * L);
     M->a_e2bar.alloc(NE * L);
@@ -155,14 +156,16 @@ AdjScratch scratch(omega_model_s *M) {
   return AdjScratch{M->a_sbar.p, M->a_zbar.p,  M->a_tbar.p,  M->a_kbar.p,
                     M->a_dbar.p, M->a_gbar.p,  M->a_fbar.p,  M->a_d2bar.p,
                     M->a_z2bar.p, M->a_e2bar.p, M->a_partial.p, M->a_pgrad.p,
-                    M->a_se.p, M->a_fu.p, M->a_pa.p, M->a_pb.p};
+                    M->a_se.p, M->a_fu.p, M->a_pa.p, M->a_pb.p, M->a_xa.p};
 }
 
-// out += J(y)^T k for the tendency at state y (pointers in device numbering);
+// out (+)= J(y)^T k for the tendency at state y (pointers in device numbering;
+// epi says how the result is combined);
 // the mixing-coefficient gradient accumulates in M->a_pgrad.
 void tendency_vjp_ptrs(omega_model_s *M, const Real *const y[3], const Real *const k[3],
-                       Real *const out[3], hipStream_t s, const AdjEpi &epi = AdjEpi{}) {
-  run_diag(M, y[0], y[1], y[2], s);
+                       Real *const out[3], hipStream_t s, const AdjEpi &epi = AdjEpi{},
+                       bool have_diag = false) {
+  if (!have_diag) run_diag(M, y[0], y[1], y[2], s);
   AdjArgs a{};
   a.terms = M->terms;
   a.do_h = !M->params.disable_h_tendency;
@@ -204,6 +207,35 @@ void ensure_sized(DBuf<Real> (&b)[3], const size_t n[3]) {
     if (b[f].n != n[f]) b[f].alloc(n[f]);
 }
 
+// Diagnostic sets 1..nsets for the stage inputs of a step adjoint, sized
+// like the model's own buffers.
+void ensure_diag_sets(omega_model_s *M, int nsets) {
+  if (M->terms & T_VDEL4) ensure_vdel4(M);
+  auto fit = [](DBuf<Real> &b, const DBuf<Real> &like) {
+    if (b.n != like.n) b.alloc(like.n);
+  };
+  for (int i = 0; i < nsets; ++i) {
+    omega_model_s::DiagSet &d = M->diag_sets[i];
+    fit(d.rvort_v, M->rvort_v);
+    fit(d.inv_thick_v, M->inv_thick_v);
+    fit(d.ke_c, M->ke_c);
+    fit(d.div_c, M->div_c);
+    fit(d.trdel2_c, M->trdel2_c);
+    fit(d.vdel2_e, M->vdel2_e);
+    fit(d.vdel2_rv_v, M->vdel2_rv_v);
+    fit(d.vdel2_div_c, M->vdel2_div_c);
+  }
+}
+
+// Selects the diagnostic set for the duration of a step adjoint; every
+// other caller always sees set 0.
+struct DiagSlotScope {
+  omega_model_s *M;
+  explicit DiagSlotScope(omega_model_s *m) : M(m) {}
+  void use(int slot) { M->diag_slot = slot; }
+  ~DiagSlotScope() { M->diag_slot = 0; }
+};
+
 } // namespace
 
 // ---------------------------------------------------------------------------
@@ -230,9 +262,15 @@ void step_vjp_ptrs(omega_stepper_s *St, const Real *const y[3], Real *const cot[
   };
   const std::array<const Real *, 3> Y{y[0], y[1], y[2]};
   const std::array<Real *, 3> C{cot[0], cot[1], cot[2]};
+  // the diagnostics of stage input s (computed by the recomputation) stay in
+  // set s for the VJP at that input; the last stage input's are computed by
+  // its VJP
+  DiagSlotScope slots(M);
   if (St->kind == OMEGA_STEPPER_RK4) {
+    ensure_diag_sets(M, 3);
     // stage inputs P1..P3 (P_{i+1} = y + dt*a_i*k_i), as rk4.cpp:168-189
     for (int st = 0; st < 3; ++st) {
+      slots.use(st);
       StageArgs a{};
       a.epi = EPI_PROVIS;
       a.da = dt * St->rka[st];
@@ -243,36 +281,47 @@ void step_vjp_ptrs(omega_stepper_s *St, const Real *const y[3], Real *const cot[
       a.tr = fio(in[2], nullptr, nullptr, y[2], out[2]);
       run_stage(M, a, s);
     }
-    // reverse sweep: Cbar += J(P_s)^T kbar_s and, in the same kernels,
-    // kbar_{s-1} = dt*b_{s-1}*gy + dt*a_{s-1}*J(P_s)^T kbar_s (two kbar
-    // buffers alternate, since each VJP reads its kbar at neighbours)
-    const auto gy = P(St->adj_gy);
+    // reverse sweep with gy = the incoming cotangent (left in place until the
+    // last VJP): acc = gy + sum_s J(P_s)^T kbar_s and, in the same kernels,
+    // kbar_{s-1} = dt*b_{s-1}*gy + dt*a_{s-1}*J(P_s)^T kbar_s.  kbar_3 =
+    // dt*b_3*gy enters as gy with the result scaled; two kbar buffers
+    // alternate, since each VJP reads its kbar at neighbours.
+    const auto acc = P(St->adj_gy);
     std::array<Real *, 3> kin = P(St->adj_kb), kout = P(St->adj_pb);
-    copy_fields(gy.data(), C.data(), n, s);
-    for (int f = 0; f < 3; ++f) launch_lincomb(kin[f], dt * St->rkb[3], gy[f], 0.0, gy[f], n[f], s);
+    const std::array<const Real *, 3> gy{C[0], C[1], C[2]};
     for (int st = 3; st >= 1; --st) {
       const auto ps = K(St->adj_y[st - 1]);
-      const std::array<const Real *, 3> kc{kin[0], kin[1], kin[2]};
+      const std::array<const Real *, 3> kc =
+          st == 3 ? gy : std::array<const Real *, 3>{kin[0], kin[1], kin[2]};
       AdjEpi epi{};
       epi.rk4 = 1;
       epi.db = dt * St->rkb[st - 1];
       epi.da = dt * St->rka[st - 1];
+      if (st == 3) epi.vs = dt * St->rkb[3];
       for (int f = 0; f < 3; ++f) {
         epi.gy[f] = gy[f];
         epi.kbo[f] = kout[f];
+        if (st == 3) epi.base[f] = gy[f];
       }
-      tendency_vjp_ptrs(M, ps.data(), kc.data(), C.data(), s, epi);
+      slots.use(st);
+      tendency_vjp_ptrs(M, ps.data(), kc.data(), acc.data(), s, epi, st < 3);
       std::swap(kin, kout);
     }
+    // cot = acc + J(y)^T kbar_0
     const std::array<const Real *, 3> kc{kin[0], kin[1], kin[2]};
-    tendency_vjp_ptrs(M, Y.data(), kc.data(), C.data(), s);
+    AdjEpi last{};
+    for (int f = 0; f < 3; ++f) last.base[f] = acc[f];
+    slots.use(0);
+    tendency_vjp_ptrs(M, Y.data(), kc.data(), C.data(), s, last, true);
   } else {
     // stage inputs Y_1..Y_{S-1} (lsrk.cpp:36-89: tend = a_s*tend + F(Y_s),
     // Y_{s+1} = Y_s + dt*b_s*tend)
     const int S = St->nstages;
     OMEGA_REQUIRE(S >= 1 && S <= 5, "step adjoint: unsupported number of stages");
     const auto tb = P(St->adj_t);
+    ensure_diag_sets(M, S - 1);
     for (int st = 0; st + 1 < S; ++st) {
+      slots.use(st);
       StageArgs a{};
       a.epi = EPI_LSRK;
       a.a = St->rka[st];
@@ -293,7 +342,8 @@ void step_vjp_ptrs(omega_stepper_s *St, const Real *const y[3], Real *const cot[
         launch_lincomb(kb[f], anext, kb[f], dt * St->rkb[st], C[f], n[f], s);
       const std::array<const Real *, 3> ys = st == 0 ? Y : K(St->adj_y[st - 1]);
       const std::array<const Real *, 3> kc{kb[0], kb[1], kb[2]};
-      tendency_vjp_ptrs(M, ys.data(), kc.data(), C.data(), s);
+      slots.use(st);
+      tendency_vjp_ptrs(M, ys.data(), kc.data(), C.data(), s, AdjEpi{}, st + 1 < S);
     }
   }
 }

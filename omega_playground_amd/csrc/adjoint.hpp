@@ -66,16 +66,22 @@ struct AdjScratch {
   Real *fu_e;     // [ne][L] edge thickness flux F_e u_e at the linearisation state
   Real *pa_e;     // [ne][L] sum over t listing e of w_te kbar_t S_t
   Real *pb_e;     // [ne][L] sum over t listing e of w_te kbar_t
+  Real *xa_e;     // [ntr][ne][L] tracer advection share of each cell of the edge,
+                  // -(cd0 b0 + cd1 b1) F_e u_e / 2 (no tracer diffusion only)
 };
 
-// What the VJP does with its result v for each state element: always
-// out += v; with rk4 set also kbo = db*gy + da*v, the next tendency
-// cotangent of the RK4 reverse sweep (kbo must not alias the input k).
+// What the VJP does with its result v = vs * J^T k for each state element:
+// out = base + v (base null: out += v); with rk4 set also kbo = db*gy + da*v,
+// the next tendency cotangent of the RK4 reverse sweep (kbo must not alias
+// the input k).  vs scales the result (and the coefficient gradient), so a
+// scaled cotangent need not be formed.
 struct AdjEpi {
-  int rk4;
-  Real db, da;
-  const Real *gy[3];
-  Real *kbo[3];
+  int rk4 = 0;
+  Real db = 0, da = 0;
+  Real vs = 1;
+  const Real *gy[3] = {nullptr, nullptr, nullptr};
+  Real *kbo[3] = {nullptr, nullptr, nullptr};
+  const Real *base[3] = {nullptr, nullptr, nullptr};
 };
 
 struct AdjArgs {

@@ -64,6 +64,20 @@ __device__ void block_partials2(double a, double b, double *out_a, double *out_b
   }
 }
 
+// Block-wide sum of one value into one partial slot (all threads call it).
+__device__ void block_partial(double a, double *out) {
+  __shared__ double sa[kAB];
+  const int t = threadIdx.y * blockDim.x + threadIdx.x;
+  const int n = blockDim.x * blockDim.y;
+  sa[t] = a;
+  __syncthreads();
+  for (int s = kAB / 2; s > 0; s >>= 1) {
+    if (t < s && t + s < n) sa[t] = sa[t] + sa[t + s];
+    __syncthreads();
+  }
+  if (t == 0) *out = sa[0];
+}
+
 struct Seg {
   int c0, c1, k;
   bool active;
@@ -79,11 +93,15 @@ template <int N> __device__ __forceinline__ Seg segment(const MeshView &m, int s
   return g;
 }
 
-// out[idx] += v; for the RK4 reverse sweep also kbo[idx] = db*gy[idx] + da*v
+// out[idx] = base[idx] + vs*v (base defaults to out); for the RK4 reverse
+// sweep also kbo[idx] = db*gy[idx] + da*vs*v
 template <int N>
 __device__ __forceinline__ void adj_out(const AdjArgs &a, int f, Real *out, size_t idx,
-                                        const Vd<N> &v) {
-  Vd<N> acc = ld<N>(out + idx);
+                                        const Vd<N> &v0) {
+  Vd<N> v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = a.epi.vs * v0[i];
+  Vd<N> acc = ld<N>((a.epi.base[f] ? a.epi.base[f] : out) + idx);
 #pragma unroll
   for (int i = 0; i < N; ++i) acc[i] += v[i];
   st<N>(out + idx, acc);
@@ -247,13 +265,16 @@ template <int N, int H>
 __global__ void __launch_bounds__(kAB)
     k_adj_vc(MeshView m, ModelCoef mc, AdjCoef ac, AdjArgs a, DiagOut d, AdjScratch s, int swz) {
   const Seg g = segment<N>(m, swz);
-  if (!g.active) return;
   const int L = m.L, k = g.k, NC = m.nc;
   const bool pv = a.do_vn && (a.terms & T_PV), vd2 = a.do_vn && (a.terms & T_VDEL2),
              vd4 = a.do_vn && (a.terms & T_VDEL4), ke = a.do_vn && (a.terms & T_KE);
-  if (pv || vd2 || vd4) {
+  // gradient w.r.t. visc_del2: sum_e kbar_e s2_e [(D_c1 - D_c0)/dc - (Z_v1 - Z_v0)/dv]
+  // regrouped by cell and vertex, sum_c D_c ds_c + sum_v Z_v zs_v, with ds and
+  // zs formed beside the del2 cotangents (no edge rows re-read)
+  double p_v2 = 0.0;
+  if (g.active && (pv || vd2 || vd4)) {
     for (int v = m.vofs[g.c0] + (int)threadIdx.y; v < m.vofs[g.c1]; v += blockDim.y) {
-      Vd<N> qb = zero<N>(), zb = zero<N>();
+      Vd<N> qb = zero<N>(), zb = zero<N>(), zs = zero<N>();
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         const int e = m.eov[3 * v + j];
@@ -267,7 +288,10 @@ __global__ void __launch_bounds__(kAB)
           const Vd<N> ut = rd<N>(a.c.vn, e, L, k);
           const int ml = m.maxlev_e[e];
 #pragma unroll
-          for (int i = 0; i < N; ++i) zb[i] += sg * mask<N>(k, i, ml, mc.cv2[e]) * ut[i] * m.inv_dv[e];
+          for (int i = 0; i < N; ++i) {
+            zb[i] += sg * mask<N>(k, i, ml, mc.cv2[e]) * ut[i] * m.inv_dv[e];
+            zs[i] += sg * mask<N>(k, i, ml, m.s2[e]) * ut[i] * m.inv_dv[e];
+          }
         }
         if (vd4) {
           const Vd<N> eb = rd<N>(s.e2bar_e, e, L, k);
@@ -275,9 +299,14 @@ __global__ void __launch_bounds__(kAB)
           for (int i = 0; i < N; ++i) zb[i] += sg * eb[i] * m.inv_dv4[e];
         }
       }
-      Vd<N> tb = zero<N>();
+      Vd<N> tb = zero<N>(), z = zero<N>();
+      if (pv || vd2) z = rd<N>(d.rvort_v, v, L, k);
+      if (vd2) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) p_v2 += z[i] * zs[i];
+      }
       if (pv) {
-        const Vd<N> io = rd<N>(d.inv_thick_v, v, L, k), z = rd<N>(d.rvort_v, v, L, k);
+        const Vd<N> io = rd<N>(d.inv_thick_v, v, L, k);
         const double f = mc.f_v[v];
 #pragma unroll
         for (int i = 0; i < N; ++i) {
@@ -290,8 +319,8 @@ __global__ void __launch_bounds__(kAB)
     }
   }
   const bool td4 = (a.terms & T_TDEL4) && a.ntr > 0;
-  for (int c = g.c0 + (int)threadIdx.y; c < g.c1; c += blockDim.y) {
-    Vd<N> kb = zero<N>(), db = zero<N>();
+  for (int c = g.c0 + (int)threadIdx.y; g.active && c < g.c1; c += blockDim.y) {
+    Vd<N> kb = zero<N>(), db = zero<N>(), ds = zero<N>();
     const int n = H ? 6 : m.noc[c];
 #pragma unroll
     for (int j = 0; j < n; ++j) {
@@ -303,7 +332,10 @@ __global__ void __launch_bounds__(kAB)
 #pragma unroll
         for (int i = 0; i < N; ++i) {
           if (ke) kb[i] += (first ? 1.0 : -1.0) * ut[i] * m.inv_dc[e];
-          if (vd2) db[i] += (first ? -1.0 : 1.0) * mask<N>(k, i, ml, mc.cv2[e]) * ut[i] * m.inv_dc[e];
+          if (vd2) {
+            db[i] += (first ? -1.0 : 1.0) * mask<N>(k, i, ml, mc.cv2[e]) * ut[i] * m.inv_dc[e];
+            ds[i] += (first ? -1.0 : 1.0) * mask<N>(k, i, ml, m.s2[e]) * ut[i] * m.inv_dc[e];
+          }
         }
       }
       if (vd4) {
@@ -314,6 +346,11 @@ __global__ void __launch_bounds__(kAB)
     }
     if (ke) st<N>(s.kbar_c + (size_t)c * L + k, kb);
     if (vd2 || vd4) st<N>(s.dbar_c + (size_t)c * L + k, db);
+    if (vd2) {
+      const Vd<N> D = rd<N>(d.div_c, c, L, k);
+#pragma unroll
+      for (int i = 0; i < N; ++i) p_v2 += D[i] * ds[i];
+    }
     if (td4) {
       for (int l = 0; l < a.ntr; ++l) {
         const double *tl = a.c.tr + (size_t)l * NC * L;
@@ -340,21 +377,24 @@ __global__ void __launch_bounds__(kAB)
       }
     }
   }
+  const int b = blockIdx.y * gridDim.x + blockIdx.x;
+  block_partial(p_v2, s.partial + b);
 }
 
 // ---------------------------------------------------------------------------
 // Edge pass: velocity cotangent (accumulated) and edge-thickness cotangent
 // ---------------------------------------------------------------------------
-template <int N, int H>
-__global__ void __launch_bounds__(kAB)
+template <int N, int H, int X>
+// four waves per SIMD (at most 128 VGPRs; fits without spilling)
+__global__ void __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4)))
     k_adj_edge(MeshView m, AdjMeshView am, ModelCoef mc, AdjCoef ac, AdjArgs a, DiagOut d,
                AdjScratch s, int swz) {
   const Seg g = segment<N>(m, swz);
   const int L = m.L, k = g.k, NC = m.nc;
   const bool pv = a.do_vn && (a.terms & T_PV), vd2 = a.do_vn && (a.terms & T_VDEL2),
              vd4 = a.do_vn && (a.terms & T_VDEL4), ke = a.do_vn && (a.terms & T_KE);
-  const bool td2 = a.terms & T_TDEL2, td4 = a.terms & T_TDEL4;
-  double p_v2 = 0.0, p_v4 = 0.0;
+  const bool td2 = X && (a.terms & T_TDEL2), td4 = X && (a.terms & T_TDEL4);
+  double p_v4 = 0.0;
   if (g.active) {
     for (int e = m.eofs[g.c0] + (int)threadIdx.y; e < m.eofs[g.c1]; e += blockDim.y) {
       const int c0 = m.coe[2 * e], c1 = m.coe[2 * e + 1], v0 = m.voe[2 * e], v1 = m.voe[2 * e + 1];
@@ -380,6 +420,7 @@ __global__ void __launch_bounds__(kAB)
           fb[i] -= ue[i] * hc;
         }
       }
+      const bool mix = td2 || td4;
       for (int l = 0; l < a.ntr; ++l) {
         const size_t off = (size_t)l * NC * L;
         const Vd<N> t0 = rd<N>(a.tr + off, c0, L, k), t1 = rd<N>(a.tr + off, c1, L, k);
@@ -398,6 +439,17 @@ __global__ void __launch_bounds__(kAB)
           fb[i] -= w * ue[i] * ne;
           if (td2) fb[i] += (ac.ect2[2 * e] * b0[i] + ac.ect2[2 * e + 1] * b1[i]) * m.s2[e] * grad;
           if (td4) fb[i] += (cd0 * g0[i] + cd1 * g1[i]) * grad;
+        }
+        if (!mix) {
+          // both cells' advection share; the same value the cell pass would
+          // form from the edge's thickness flux and the cotangent rows
+          Vd<N> xa;
+#pragma unroll
+          for (int i = 0; i < N; ++i) {
+            const double w = cd0 * b0[i] + cd1 * b1[i];
+            xa[i] = 0.5 * (-w * (F[i] * ue[i]));
+          }
+          st<N>(s.xa_e + ((size_t)l * m.ne + e) * L + k, xa);
         }
       }
       if (a.do_vn) {
@@ -428,17 +480,9 @@ __global__ void __launch_bounds__(kAB)
 #pragma unroll
           for (int i = 0; i < N; ++i) ub[i] += cd0 * d0[i] + cd1 * d1[i];
         }
-        const Vd<N> ut = rd<N>(a.c.vn, e, L, k);
-        const int ml = m.maxlev_e[e];
-        if (vd2) {
-          const Vd<N> D0 = rd<N>(d.div_c, c0, L, k), D1 = rd<N>(d.div_c, c1, L, k);
-          const Vd<N> Z0 = rd<N>(d.rvort_v, v0, L, k), Z1 = rd<N>(d.rvort_v, v1, L, k);
-#pragma unroll
-          for (int i = 0; i < N; ++i)
-            p_v2 += ut[i] * mask<N>(k, i, ml, m.s2[e]) *
-                    ((D1[i] - D0[i]) * idc - (Z1[i] - Z0[i]) * m.inv_dv[e]);
-        }
         if (vd4) {
+          const Vd<N> ut = rd<N>(a.c.vn, e, L, k);
+          const int ml = m.maxlev_e[e];
           const Vd<N> D0 = rd<N>(d.vdel2_div_c, c0, L, k), D1 = rd<N>(d.vdel2_div_c, c1, L, k);
           const Vd<N> Z0 = rd<N>(d.vdel2_rv_v, v0, L, k), Z1 = rd<N>(d.vdel2_rv_v, v1, L, k);
 #pragma unroll
@@ -452,20 +496,20 @@ __global__ void __launch_bounds__(kAB)
     }
   }
   const int b = blockIdx.y * gridDim.x + blockIdx.x;
-  block_partials2(p_v2, p_v4, s.partial + b, s.partial + a.nparts + b);
+  block_partial(p_v4, s.partial + a.nparts + b);
 }
 
 // ---------------------------------------------------------------------------
 // Cell pass: thickness and tracer cotangents (accumulated)
 // ---------------------------------------------------------------------------
-template <int N, int H>
+template <int N, int H, int X>
 __global__ void __launch_bounds__(kAB)
     k_adj_cell(MeshView m, AdjMeshView am, ModelCoef mc, AdjCoef ac, AdjArgs a, DiagOut d,
                AdjScratch s, int swz) {
   const Seg g = segment<N>(m, swz);
   const int L = m.L, k = g.k, NC = m.nc, me = m.me;
   const bool pv = a.do_vn && (a.terms & T_PV), ssh = a.do_vn && (a.terms & T_SSH);
-  const bool td2 = a.terms & T_TDEL2, td4 = a.terms & T_TDEL4;
+  const bool td2 = X && (a.terms & T_TDEL2), td4 = X && (a.terms & T_TDEL4);
   double p_t2 = 0.0, p_t4 = 0.0;
   if (g.active) {
     for (int c = g.c0 + (int)threadIdx.y; c < g.c1; c += blockDim.y) {
@@ -512,6 +556,13 @@ __global__ void __launch_bounds__(kAB)
 #pragma unroll
         for (int j = 0; j < n; ++j) {
           const int e = m.eoc[c * me + j];
+          if (!mix) {
+            // advection only: the edge pass stored this edge's share
+            const Vd<N> xa = ld<N>(s.xa_e + ((size_t)l * m.ne + e) * L + k);
+#pragma unroll
+            for (int i = 0; i < N; ++i) nb[i] += xa[i];
+            continue;
+          }
           const int a0 = m.coe[2 * e], a1 = m.coe[2 * e + 1];
           const bool second = a1 == c;
           const int o = second ? a0 : a1;
@@ -595,7 +646,7 @@ __global__ void __launch_bounds__(kAB)
 
 // pgrad[p] += sum of the block partials of parameter p, in a fixed order
 __global__ void __launch_bounds__(kAB) k_adj_param_reduce(double *partial, int nparts,
-                                                         int nused, double *pgrad) {
+                                                         int nused, double vs, double *pgrad) {
   __shared__ double sh[kAB];
   const int p = blockIdx.x;
   double acc = 0.0;
@@ -606,7 +657,7 @@ __global__ void __launch_bounds__(kAB) k_adj_param_reduce(double *partial, int n
     if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) pgrad[p] += sh[0];
+  if (threadIdx.x == 0) pgrad[p] += vs * sh[0];
 }
 
 struct AGeo {
@@ -760,11 +811,29 @@ void launch_tendency_vjp(const MeshView &m, const AdjMeshView &am, const Launch 
   if (pv || a.ntr > 0) OMEGA_ADJ_LAUNCH(k_adj_pre, m, mc, a, d, s, swz);
   if (pv) OMEGA_ADJ_LAUNCH(k_adj_pv, m, am, mc, a, d, s, swz);
   OMEGA_ADJ_LAUNCH(k_adj_vc, m, mc, ac, a, d, s, swz);
-  OMEGA_ADJ_LAUNCH(k_adj_edge, m, am, mc, ac, a, d, s, swz);
-  OMEGA_ADJ_LAUNCH(k_adj_cell, m, am, mc, ac, a, d, s, swz);
+#define OMEGA_ADJ_LAUNCH_X(K, X, ...)                                                           \
+  do {                                                                                         \
+    if (g.N == 2 && am.fixed)                                                                  \
+      hipLaunchKernelGGL((K<2, 1, X>), g.grid, g.block, 0, st, __VA_ARGS__);                    \
+    else if (g.N == 2)                                                                         \
+      hipLaunchKernelGGL((K<2, 0, X>), g.grid, g.block, 0, st, __VA_ARGS__);                    \
+    else if (am.fixed)                                                                         \
+      hipLaunchKernelGGL((K<1, 1, X>), g.grid, g.block, 0, st, __VA_ARGS__);                    \
+    else                                                                                       \
+      hipLaunchKernelGGL((K<1, 0, X>), g.grid, g.block, 0, st, __VA_ARGS__);                    \
+  } while (0)
+  // X = 0: no tracer diffusion term (edge and cell passes without its code)
+  if (a.terms & (T_TDEL2 | T_TDEL4)) {
+    OMEGA_ADJ_LAUNCH_X(k_adj_edge, 1, m, am, mc, ac, a, d, s, swz);
+    OMEGA_ADJ_LAUNCH_X(k_adj_cell, 1, m, am, mc, ac, a, d, s, swz);
+  } else {
+    OMEGA_ADJ_LAUNCH_X(k_adj_edge, 0, m, am, mc, ac, a, d, s, swz);
+    OMEGA_ADJ_LAUNCH_X(k_adj_cell, 0, m, am, mc, ac, a, d, s, swz);
+  }
+#undef OMEGA_ADJ_LAUNCH_X
 #undef OMEGA_ADJ_LAUNCH
   hipLaunchKernelGGL(k_adj_param_reduce, dim3(4), dim3(kAB), 0, st, s.partial, a.nparts,
-                     (int)(g.grid.x * g.grid.y), s.pgrad);
+                     (int)(g.grid.x * g.grid.y), a.epi.vs, s.pgrad);
   OMEGA_HIP_CHECK(hipGetLastError());
 }
 

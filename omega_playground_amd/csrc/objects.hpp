@@ -60,7 +60,7 @@ struct omega_model_s {
   // adjoint: transposed tracer coefficients and cotangent scratch, built on
   // first use
   DBuf<Real> ect2, ect4;
-  DBuf<Real> a_se, a_fu, a_pa, a_pb;
+  DBuf<Real> a_se, a_fu, a_pa, a_pb, a_xa;
   DBuf<Real> a_sbar, a_zbar, a_tbar, a_kbar, a_dbar, a_gbar, a_fbar, a_d2bar, a_z2bar, a_e2bar,
       a_partial, a_pgrad, a_mpart;
   int a_nparts = 0;
@@ -85,7 +85,20 @@ struct omega_model_s {
     return ModelCoef{params.grav, f_v.p, cv2.p, cv4.p, ct2.p, ct4.p,
                      f_uniform && mix_uniform ? 1 : 0};
   }
+  // step adjoint: diagnostics of the recomputed stage inputs, one set per
+  // stage input (stage 0 uses the buffers above), kept for the reverse
+  // sweep; diag_slot selects the set run_diag writes and the kernels read
+  struct DiagSet {
+    DBuf<Real> rvort_v, inv_thick_v, ke_c, div_c, trdel2_c, vdel2_e, vdel2_rv_v, vdel2_div_c;
+  };
+  DiagSet diag_sets[4];
+  int diag_slot = 0;
   DiagOut diag() {
+    if (diag_slot > 0) {
+      DiagSet &d = diag_sets[diag_slot - 1];
+      return DiagOut{d.rvort_v.p, d.inv_thick_v.p, d.ke_c.p, d.div_c.p, d.trdel2_c.p,
+                     d.vdel2_e.p, d.vdel2_rv_v.p, d.vdel2_div_c.p};
+    }
     return DiagOut{rvort_v.p, inv_thick_v.p, ke_c.p, div_c.p, trdel2_c.p,
                    vdel2_e.p, vdel2_rv_v.p, vdel2_div_c.p};
   }
