@@ -517,19 +517,20 @@ def test_autograd_function_wrapper_matches_torch_reference():
 ADVECTION_ONLY = ALL_TERMS & ~(swe.TDEL2 | swe.TDEL4)
 
 
+@pytest.mark.parametrize("L", [4, 3])
 @pytest.mark.parametrize("kind,nstages", [("rk4", 4), ("lsrk", 5)])
-def test_step_vjp_tracer_advection_only(kind, nstages):
+def test_step_vjp_tracer_advection_only(kind, nstages, L):
     """With no tracer diffusion term the edge pass hands the cell pass each
     edge's tracer advection share (advection-only kernels); the step adjoint
     still matches autograd, and the parameter gradient (visc_del2 formed in
     the vertex/cell pass) too."""
-    gm, gmodel, pd, state = setup(8, 8, 4, 2, seed=6, terms=ADVECTION_ONLY)
+    gm, gmodel, pd, state = setup(8, 8, L, 2, seed=6, terms=ADVECTION_ONLY)
     st = make_stepper(gmodel, kind, nstages)
     dt = 30.0
     cot = random_like(state, 43)
     gc = upload(gm, 2, cot)
     gp = swe.step_vjp(st, 0.0, dt, upload(gm, 2, state), gc)
-    tm = ts.TorchMesh(gm, 4)
+    tm = ts.TorchMesh(gm, L)
     if kind == "rk4":
         step = lambda p, *y: ts.rk4_step(tm, p, ADVECTION_ONLY, dt, *y)
     else:
