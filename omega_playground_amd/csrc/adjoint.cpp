@@ -334,17 +334,37 @@ void step_vjp_ptrs(omega_stepper_s *St, const Real *const y[3], Real *const cot[
       a.tr = fio(in[2], tb[2], nullptr, nullptr, out[2]);
       run_stage(M, a, s);
     }
-    // Tbar_s = dt*b_s*Ybar_{s+1} + a_{s+1}*Tbar_{s+1};  Ybar_s = Ybar_{s+1} + J(Y_s)^T Tbar_s
-    const auto kb = P(St->adj_kb);
+    // Tbar_s = dt*b_s*Ybar_{s+1} + a_{s+1}*Tbar_{s+1};  Ybar_s = Ybar_{s+1} + J(Y_s)^T Tbar_s.
+    // The VJP at Y_s writes Ybar_s and, in the same kernels, Tbar_{s-1} =
+    // a_s*Tbar_s + dt*b_{s-1}*Ybar_s.  Tbar_{S-1} = dt*b_{S-1}*Ybar_S enters as
+    // the incoming cotangent with the result scaled; Ybar accumulates in acc
+    // (the incoming cotangent is read by that first VJP at neighbours) and
+    // the last VJP writes it to cot; two Tbar buffers alternate.
+    const auto acc = P(St->adj_gy);
+    std::array<Real *, 3> kin = P(St->adj_kb), kout = P(St->adj_pb);
+    const std::array<const Real *, 3> gy{C[0], C[1], C[2]};
     for (int st = S - 1; st >= 0; --st) {
-      const double anext = st + 1 < S ? St->rka[st + 1] : 0.0;
-      for (int f = 0; f < 3; ++f)
-        launch_lincomb(kb[f], anext, kb[f], dt * St->rkb[st], C[f], n[f], s);
+      const bool first = st == S - 1, last = st == 0;
       const std::array<const Real *, 3> ys = st == 0 ? Y : K(St->adj_y[st - 1]);
-      const std::array<const Real *, 3> kc{kb[0], kb[1], kb[2]};
+      const std::array<const Real *, 3> kc =
+          first ? gy : std::array<const Real *, 3>{kin[0], kin[1], kin[2]};
+      AdjEpi epi{};
+      if (first) epi.vs = dt * St->rkb[st];
+      if (!last) {
+        epi.lsrk = 1;
+        epi.db = dt * St->rkb[st - 1];
+        epi.da = first ? St->rka[st] * (dt * St->rkb[st]) : St->rka[st];
+      }
+      for (int f = 0; f < 3; ++f) {
+        epi.base[f] = first ? gy[f] : acc[f];
+        epi.kbo[f] = kout[f];
+      }
+      Real *const *out = (last && !first) ? C.data() : acc.data();
       slots.use(st);
-      tendency_vjp_ptrs(M, ys.data(), kc.data(), C.data(), s, AdjEpi{}, st + 1 < S);
+      tendency_vjp_ptrs(M, ys.data(), kc.data(), out, s, epi, st + 1 < S);
+      std::swap(kin, kout);
     }
+    if (S == 1) copy_fields(C.data(), K(St->adj_gy).data(), n, s);
   }
 }
 

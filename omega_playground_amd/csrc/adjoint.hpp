@@ -72,11 +72,12 @@ struct AdjScratch {
 
 // What the VJP does with its result v = vs * J^T k for each state element:
 // out = base + v (base null: out += v); with rk4 set also kbo = db*gy + da*v,
-// the next tendency cotangent of the RK4 reverse sweep (kbo must not alias
-// the input k).  vs scales the result (and the coefficient gradient), so a
-// scaled cotangent need not be formed.
+// the next tendency cotangent of the RK4 reverse sweep; with lsrk set
+// kbo = da*k + db*out, that of the low-storage RK reverse sweep (kbo must not
+// alias the input k).  vs scales the result (and the coefficient gradient),
+// so a scaled cotangent need not be formed.
 struct AdjEpi {
-  int rk4 = 0;
+  int rk4 = 0, lsrk = 0;
   Real db = 0, da = 0;
   Real vs = 1;
   const Real *gy[3] = {nullptr, nullptr, nullptr};

@@ -94,7 +94,8 @@ template <int N> __device__ __forceinline__ Seg segment(const MeshView &m, int s
 }
 
 // out[idx] = base[idx] + vs*v (base defaults to out); for the RK4 reverse
-// sweep also kbo[idx] = db*gy[idx] + da*vs*v
+// sweep also kbo[idx] = db*gy[idx] + da*vs*v, for the low-storage RK reverse
+// sweep kbo[idx] = da*k[idx] + db*out[idx] (k: this VJP's input cotangent)
 template <int N>
 __device__ __forceinline__ void adj_out(const AdjArgs &a, int f, Real *out, size_t idx,
                                         const Vd<N> &v0) {
@@ -110,6 +111,14 @@ __device__ __forceinline__ void adj_out(const AdjArgs &a, int f, Real *out, size
     Vd<N> kn;
 #pragma unroll
     for (int i = 0; i < N; ++i) kn[i] = a.epi.db * g[i] + a.epi.da * v[i];
+    st<N>(a.epi.kbo[f] + idx, kn);
+  }
+  if (a.epi.lsrk) {
+    const Real *kf = f == 0 ? a.c.h : (f == 1 ? a.c.vn : a.c.tr);
+    const Vd<N> kv = ld<N>(kf + idx);
+    Vd<N> kn;
+#pragma unroll
+    for (int i = 0; i < N; ++i) kn[i] = a.epi.da * kv[i] + a.epi.db * acc[i];
     st<N>(a.epi.kbo[f] + idx, kn);
   }
 }
