@@ -8,10 +8,12 @@ balance, momentum del2 viscosity 1e3 m^2/s, tracer del4 1e9 m^4/s, the step
 captured in a HIP graph.  A step is one full RK4 step (4 tendency
 evaluations + stage updates); the metric counts cell-layer-steps/s.
 
-Prints one JSON line.  `roofline` is measured on the dominant kernel (the
-fused tendency + stage-update kernel) with HIP events on its launch stream;
-`cpu_baseline` times the CPU oracle (OpenMP, all host cores of the job) on a
-bounded sample of the same workload.
+Prints one JSON line.  `roofline` is measured on one RK stage (its
+diagnostics launch and its fused tendency + stage-update launch) with HIP
+events on their launch stream: the stage's algorithmic bytes over the two
+kernels' summed average durations; `step_roofline` is the same bytes over
+the whole graph-replayed step.  `cpu_baseline` times the CPU oracle (OpenMP,
+the job's cores, spread binding) on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -21,6 +23,11 @@ import math
 import os
 import sys
 import time
+
+# the CPU baseline's OpenMP threads are spread over the job's cores; libgomp
+# reads these when the oracle library first loads
+os.environ.setdefault("OMP_PROC_BIND", "spread")
+os.environ.setdefault("OMP_PLACES", "threads")
 
 import numpy as np
 
@@ -54,7 +61,29 @@ def alg_bytes_per_cell_layer_step(ntr: int, stepper: str) -> float:
     return (16.0 if stepper == "rk4" else 18.0) * S
 
 
-def cpu_baseline(cfg, sample_steps: int, threads: int):
+def host_cpu():
+    """CPU model, sockets and physical cores of the host (lscpu)."""
+    info = {}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, ValueError):
+        pass
+    sockets = int(info.get("Socket(s)", "0") or 0)
+    per = int(info.get("Core(s) per socket", "0") or 0)
+    return {"model": info.get("Model name", "unknown"), "sockets": sockets,
+            "physical_cores": sockets * per, "logical_cpus": os.cpu_count()}
+
+
+def cpu_baseline(cfg, sample_steps: int, threads: int, runs: int = 3):
+    """The oracle (C++/OpenMP) on the same workload: the reference's protocol
+    (one warm-up step, then the timed steps), the median of `runs` timings,
+    threads spread over the job's cores (OMP_PROC_BIND=spread,
+    OMP_PLACES=threads, set before the OpenMP runtime starts, as
+    scripts/run_thread_scaling.sh:12-17 runs the reference)."""
     from oracle import pyoracle as po
 
     po.set_num_threads(threads)
@@ -65,17 +94,85 @@ def cpu_baseline(cfg, sample_steps: int, threads: int):
     state = tuple(np.ascontiguousarray(a) for a in wl.make_state(om, cfg, seed=0))
     dt = wl.dt_rule(cfg["dc"], cfg["grav"])
     st.run(0.0, dt, 1, state)  # warm-up step (reference protocol)
-    t0 = time.perf_counter()
-    st.run(dt, dt, sample_steps, state)
-    el = time.perf_counter() - t0
+    times = []
+    for r in range(runs):
+        t0 = time.perf_counter()
+        st.run((1 + r * sample_steps) * dt, dt, sample_steps, state)
+        times.append(time.perf_counter() - t0)
+    el = float(np.median(times))
     work = om.ncells * om.nlayers * sample_steps
+    cpu = host_cpu()
     return {
         "value": work / el,
         "unit": "cell-layer-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{sample_steps} RK4 steps of the same nx={cfg['nx']} workload after one "
-                  f"warm-up step ({el:.2f} s), oracle C++/OpenMP, {threads} threads",
+        "sample": f"median of {runs} runs of {sample_steps} RK4 steps of the same "
+                  f"nx={cfg['nx']} workload after one warm-up step (runs "
+                  f"{', '.join(f'{t:.2f}' for t in times)} s), oracle C++/OpenMP, {threads} "
+                  f"threads with OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')} "
+                  f"OMP_PLACES={os.environ.get('OMP_PLACES')} on the job's CPU share "
+                  f"(host: {cpu['model']}, {cpu['sockets']} socket(s), "
+                  f"{cpu['physical_cores']} physical cores)",
+        "host_cpu": cpu,
+        "runs_s": times,
+    }
+
+
+def reference_benchmark(nx: int = 512, nlayers: int = 64, ntracers: int = 1, nsteps: int = 20,
+                        graph: bool = True):
+    """The reference's own benchmark shape (benchmark/benchmark.cpp:83-164):
+    double vortex on lx = 5000 km, ly = lx*sqrt(3)/2 (dc = lx/nx), tracers
+    1000, all four mixing coefficients 1e-5 (so del2 and del4 momentum and
+    tracer mixing all run), LSRK5, dt = 0.1 dc / sqrt(g h0); one untimed step,
+    then nsteps timed steps between device synchronisations."""
+    dv = wl.DoubleVortexTest(benchmark=True)
+    dc = dv.lx / nx
+    mesh = swe.PlanarHexagonalMesh(nx, nx, dc, nlayers)
+    params = swe.ShallowWaterParams(f0=dv.coriolis, grav=dv.g, visc_del2=1e-5, visc_del4=1e-5,
+                                    eddy_diff2=1e-5, eddy_diff4=1e-5, ntracers=ntracers)
+    model = swe.ShallowWaterModel(mesh, params)
+    xc, yc = np.asarray(mesh.x_cell), np.asarray(mesh.y_cell)
+    xe, ye, ae = np.asarray(mesh.x_edge), np.asarray(mesh.y_edge), np.asarray(mesh.angle_edge)
+    h = np.repeat(dv.h(xc, yc)[:, None], nlayers, 1)
+    vn = np.repeat(wl.normal_velocity(ae, dv.vx(xe, ye), dv.vy(xe, ye))[:, None], nlayers, 1)
+    tr = np.full((ntracers, mesh.ncells, nlayers), 1000.0)
+    state = swe.ShallowWaterState(mesh, ntracers)
+    state.upload(h, vn, tr)
+    dt = 0.1 * dc / math.sqrt(dv.g * dv.h0)
+    st = swe.LSRKStepper(model, 5)
+    st.set_graph(graph)
+    st.run(0.0, dt, 1, state)
+    swe.synchronize()
+    t0 = time.perf_counter()
+    st.run(0.0, dt, nsteps, state, t_offset_steps=1)
+    swe.synchronize()
+    el = time.perf_counter() - t0
+    # per-kernel times on eager steps
+    st.set_graph(False)
+    swe.timing_enable(True)
+    swe.timing_reset()
+    st.run(0.0, dt, 2, state)
+    swe.synchronize()
+    kern = {}
+    for name in ("diag", "vdel4", "tend"):
+        ms, n = swe.timing_read(name)
+        kern[name] = {"avg_ms": ms / max(n, 1), "launches": n}
+    swe.timing_enable(False)
+    work = mesh.ncells * nlayers * nsteps
+    B = alg_bytes_per_cell_layer_step(ntracers, "lsrk")
+    value = work / el
+    return {
+        "workload": f"reference benchmark shape: double vortex, planar hex nx={nx} "
+                    f"({mesh.ncells} cells), {nlayers} layers, {ntracers} tracer(s), LSRK5, "
+                    f"visc_del2 = visc_del4 = eddy_diff2 = eddy_diff4 = 1e-5, {nsteps} steps",
+        "seconds": el,
+        "ms_per_step": 1e3 * el / nsteps,
+        "cell_layer_steps_per_s": value,
+        "B_alg_per_cell_layer_step": B,
+        "roofline_frac": B * value / 1e9 / HBM_PEAK_GBS,
+        "kernels": kern,
+        "kernel_variant": model.kernel_variant(),
     }
 
 
@@ -353,12 +450,14 @@ def main():
     ap.add_argument("--ordering", default="hilbert")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--skip-adjoint", action="store_true")
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--skip-ensemble-adjoint", action="store_true")
-    ap.add_argument("--c5-steps", type=int, default=100,
-                    help="steps of the ensemble adjoint leg (the workload's full length is 500)")
+    ap.add_argument("--c5-steps", type=int, default=500,
+                    help="steps of the ensemble adjoint leg (the workload's length)")
+    ap.add_argument("--skip-reference-shape", action="store_true")
+    ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--launch", default="", help="rows,lpt,swizzle,levels_per_block")
     ap.add_argument("--variant", type=int, default=None,
@@ -422,11 +521,15 @@ def main():
     avg_tend_ms = tend_ms / max(tend_n, 1)
     avg_diag_ms = diag_ms / max(diag_n, 1)
     B = alg_bytes_per_cell_layer_step(ntr, cfg["stepper"])
-    # the fused tendency + update kernel carries the stage's state traffic:
-    # B_alg per step spread over the step's 4 (RK4) launches
+    # one RK stage is the unit of launch: its diagnostics launch and its fused
+    # tendency + update launch carry the stage's share of B_alg (a quarter of
+    # the step's for RK4); the headline roofline charges those bytes to the
+    # two kernels' summed average durations
     nlaunch = 4 if cfg["stepper"] == "rk4" else 5
     alg_bytes_launch = B * nc * L / nlaunch
-    achieved = alg_bytes_launch / (avg_tend_ms * 1e-3) / 1e9
+    stage_ms = avg_tend_ms + avg_diag_ms
+    achieved = alg_bytes_launch / (stage_ms * 1e-3) / 1e9
+    tend_only = alg_bytes_launch / (avg_tend_ms * 1e-3) / 1e9
     step_achieved = B * value / 1e9
     log(f"[bench] tend kernel avg {avg_tend_ms:.4f} ms x{tend_n}, diag avg {avg_diag_ms:.4f} ms "
         f"x{diag_n}; B_alg {B:.0f} B/cell-layer-step")
@@ -452,24 +555,33 @@ def main():
         log(f"[bench] ensemble adjoint: batched {eadj['batched_s']:.3f} s vs looped "
             f"{eadj['looped_s']:.3f} s = {eadj['speedup']:.2f}x, {eadj['snapshots']} snapshots")
 
+    refb = None
+    if not args.skip_reference_shape:
+        log("[bench] reference benchmark shape")
+        refb = reference_benchmark(graph=not args.no_graph)
+        log(f"[bench] reference shape: {refb['ms_per_step']:.3f} ms/step, "
+            f"{refb['cell_layer_steps_per_s']:.4e} cell-layer-steps/s")
+
     cpu = None
     if not args.no_cpu_baseline:
+        # the job's CPU share: OMP_NUM_THREADS when set (the GPU box sets it to
+        # the cores of its share), else every logical CPU
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-        threads = min(threads, 16)
-        log(f"[bench] CPU baseline: {args.cpu_steps} steps, {threads} threads")
-        cpu = cpu_baseline(cfg, args.cpu_steps, threads)
+        log(f"[bench] CPU baseline: {args.cpu_runs} x {args.cpu_steps} steps, {threads} threads")
+        cpu = cpu_baseline(cfg, args.cpu_steps, threads, args.cpu_runs)
         log(f"[bench] CPU {cpu['value']:.4e} cell-layer-steps/s")
 
     workload_key = f"{args.config}:{args.ordering}:" + ",".join(str(x) for x in model.get_launch())
     traffic = load_traffic("tend", workload_key)
     dtraffic = load_traffic("diag", workload_key)
-    # measured HBM rate of the tend kernel (counter bytes per launch over this
-    # run's HIP-event duration) and of the whole step (both kernels per stage)
-    traffic_gbs = traffic["bytes_per_launch"] / (avg_tend_ms * 1e-3) / 1e9 if traffic else None
-    step_traffic_gbs = None
+    # measured HBM bytes per stage (rocprofv3 FETCH_SIZE + WRITE_SIZE of both
+    # kernels, committed in profiles/traffic.json) over this run's durations
+    stage_traffic = None
     if traffic and dtraffic:
-        step_traffic_gbs = nlaunch * (traffic["bytes_per_launch"] + dtraffic["bytes_per_launch"]) \
-            / (ms_per_step * 1e-3) / 1e9
+        stage_traffic = traffic["bytes_per_launch"] + dtraffic["bytes_per_launch"]
+    traffic_gbs = stage_traffic / (stage_ms * 1e-3) / 1e9 if stage_traffic else None
+    step_traffic_gbs = nlaunch * stage_traffic / (ms_per_step * 1e-3) / 1e9 \
+        if stage_traffic else None
 
     out = {
         "metric": "cell-layer-steps/s FP64 fwd+adjoint at 1 GPU; adjoint/fwd ratio; batched speedup",
@@ -492,12 +604,22 @@ def main():
                    "graph": not args.no_graph, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "kernel": "k_tend (fused tendency + RK4 stage update)",
-                     "kernel_avg_ms": avg_tend_ms,
+                     "traffic": stage_traffic,
+                     "traffic_source": "profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE, "
+                                       "WRITE_SIZE; scaled by a plain copy)"
+                     if stage_traffic else None,
+                     "kernel": "one RK stage: k_diag (diagnostics) + k_tend (fused tendency + "
+                               "RK4 stage update)",
+                     "stage_avg_ms": stage_ms,
                      "alg_bytes_per_launch": alg_bytes_launch,
                      "traffic_GBs": traffic_gbs,
-                     "traffic_frac": traffic_gbs / HBM_PEAK_GBS if traffic_gbs else None},
+                     "traffic_frac": traffic_gbs / HBM_PEAK_GBS if traffic_gbs else None,
+                     "kernels": {
+                         "tend": {"avg_ms": avg_tend_ms, "launches": tend_n,
+                                  "alg_frac_if_charged_alone": tend_only / HBM_PEAK_GBS,
+                                  "traffic": traffic["bytes_per_launch"] if traffic else None},
+                         "diag": {"avg_ms": avg_diag_ms, "launches": diag_n,
+                                  "traffic": dtraffic["bytes_per_launch"] if dtraffic else None}}},
         "step_roofline": {"B_alg_per_cell_layer_step": B, "achieved_GBs": step_achieved,
                           "frac": step_achieved / HBM_PEAK_GBS, "diag_kernel_avg_ms": avg_diag_ms,
                           "traffic_GBs": step_traffic_gbs,
@@ -507,6 +629,7 @@ def main():
         "adjoint": adj,
         "batched": bat,
         "ensemble_adjoint": eadj,
+        "reference_shape": refb,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

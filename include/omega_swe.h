@@ -164,7 +164,8 @@ int omega_state_copy(omega_state_t dst, omega_state_t src, omega_stream_t stream
 typedef struct {
   double f0;
   double grav;
-  double drag_coeff; /* carried, unused as in the reference */
+  double drag_coeff; /* carried, unused as in the reference unless the
+                        bottom-drag term (omega_model_set_terms bit 7) is on */
   double visc_del2;
   double visc_del4;
   double eddy_diff2;
@@ -225,10 +226,64 @@ int omega_model_compute_field_tendency(omega_model_t model, int field, omega_sta
                                        omega_state_t state, int add_mode,
                                        omega_stream_t stream);
 
-/* Term switches for tests (bit 0 PV flux, 1 KE gradient, 2 SSH gradient,
- * 3 momentum del2, 4 momentum del4, 5 tracer del2, 6 tracer del4); the
- * default is what the constructor enabled.  No reference counterpart: the
- * reference toggles the same m_enabled flags. */
+/* Replaces ShallowWaterAuxiliaryState::compute(h, vn, tr, f_vertex, mesh)
+ * (src/model/shallow_water_auxstate.cpp:12-50), the aux pass that
+ * compute_tendency runs first (shallow_water_model.cpp:317-319): vertex
+ * vorticity and thickness, cell kinetic energy and divergence, and what the
+ * enabled mixing terms read (tracer Laplacian, momentum del4 pre-pass), into
+ * the model's buffers.  h, vn, tr are device arrays (device numbering; tr may
+ * be NULL without tracers).  The model keeps the three pointers as the
+ * auxiliary state's source: the edge thickness and normalised tracers the
+ * field tendencies below read are formed from them, so they must stay
+ * unchanged until those calls have run.  Every tendency evaluation and step
+ * recomputes the auxiliary state from its own input, as the reference's
+ * compute_tendency does. */
+int omega_model_compute_aux(omega_model_t model, const double *h, const double *vn,
+                            const double *tr, omega_stream_t stream);
+
+/* Replace ShallowWaterModel::compute_h_tendency(Real2d h_tend_cell,
+ * RealConst2d h_cell, RealConst2d vn_edge, AddMode),
+ * compute_vn_tendency(Real2d vn_tend_edge, RealConst2d h_cell,
+ * RealConst2d vn_edge, AddMode) and compute_tr_tendency(Real3d tr_tend_cell,
+ * RealConst3d tr_cell, RealConst2d vn_edge, AddMode)
+ * (src/model/shallow_water_model.hpp:35-40, shallow_water_model.cpp:43-311)
+ * on device arrays in device numbering.  As in the reference they read the
+ * auxiliary state of the last compute_aux / compute_tendency / step:
+ * the thickness flux and the normalised tracers come from its source arrays
+ * (h_cell is not read by compute_h_tendency, tr_cell not by
+ * compute_tr_tendency), and compute_vn_tendency's h must be the auxiliary
+ * state's own thickness array (OMEGA_ERR_ARG otherwise).  Without an
+ * auxiliary state they fail with OMEGA_ERR_ARG.  The linear model reads its
+ * arguments directly. */
+int omega_model_compute_h_tendency(omega_model_t model, double *h_tend, const double *h,
+                                   const double *vn, int add_mode, omega_stream_t stream);
+int omega_model_compute_vn_tendency(omega_model_t model, double *vn_tend, const double *h,
+                                    const double *vn, int add_mode, omega_stream_t stream);
+int omega_model_compute_tr_tendency(omega_model_t model, double *tr_tend, const double *tr,
+                                    const double *vn, int add_mode, omega_stream_t stream);
+
+/* Host copy, in the original numbering, of an array of the auxiliary state
+ * (m_aux_state fields, src/model/shallow_water_auxstate.hpp:10-22 and
+ * the auxiliary_vars headers): "rvort_vertex" [nvertices][L], "inv_thick_vertex"
+ * (1 / the vertex thickness of vorticity_aux.hpp:103-104) [nvertices][L],
+ * "ke_cell", "vel_div_cell" [ncells][L], "tracer_del2_cell"
+ * [ntracers][ncells][L].  Only arrays the enabled terms need are computed;
+ * the others fail with OMEGA_ERR_ARG or hold stale values.  Synchronises. */
+int omega_model_aux_download(omega_model_t model, const char *name, double *out,
+                             omega_stream_t stream);
+
+/* Term switches (bit 0 PV flux, 1 KE gradient, 2 SSH gradient, 3 momentum
+ * del2, 4 momentum del4, 5 tracer del2, 6 tracer del4, 7 bottom drag); the
+ * default is what the constructor enabled.  The reference toggles the same
+ * m_enabled flags.  Bit 7 is opt-in only: the reference's active model
+ * carries m_drag_coeff but applies no drag (shallow_water_params.hpp:10);
+ * its formula survives only as a comment at
+ * src/shallow_water_hierarchical.cpp:479-485, which this term follows:
+ * on an edge's deepest active level k = max_level_edge_top - 1,
+ * vn_tend += -drag_coeff * sqrt(ke_cell0 + ke_cell1) * vn * (1 / h_edge)
+ * with h_edge = 0.5 * (h_cell0 + h_cell1), added after the SSH gradient and
+ * before the mixing terms.  Models with it run the general kernels; the
+ * adjoint rejects them. */
 int omega_model_set_terms(omega_model_t model, int mask);
 int omega_model_get_terms(omega_model_t model, int *mask);
 

@@ -91,8 +91,28 @@ struct DeviceArray2d {
 struct ConstDeviceArray2d {
   const double *ptr = nullptr;
   Int rows = 0, cols = 0;
+  ConstDeviceArray2d() = default;
+  ConstDeviceArray2d(const double *p, Int r, Int c) : ptr(p), rows(r), cols(c) {}
+  ConstDeviceArray2d(const DeviceArray2d &a) : ptr(a.ptr), rows(a.rows), cols(a.cols) {}
   const double *data() const { return ptr; }
   Int extent(int d) const { return d == 0 ? rows : cols; }
+};
+
+// A device tracer array [ntracers][cells][layers] in device numbering.
+struct DeviceArray3d {
+  double *ptr = nullptr;
+  Int n0 = 0, n1 = 0, n2 = 0;
+  double *data() const { return ptr; }
+  Int extent(int d) const { return d == 0 ? n0 : d == 1 ? n1 : n2; }
+};
+struct ConstDeviceArray3d {
+  const double *ptr = nullptr;
+  Int n0 = 0, n1 = 0, n2 = 0;
+  ConstDeviceArray3d() = default;
+  ConstDeviceArray3d(const double *p, Int a, Int b, Int c) : ptr(p), n0(a), n1(b), n2(c) {}
+  ConstDeviceArray3d(const DeviceArray3d &a) : ptr(a.ptr), n0(a.n0), n1(a.n1), n2(a.n2) {}
+  const double *data() const { return ptr; }
+  Int extent(int d) const { return d == 0 ? n0 : d == 1 ? n1 : n2; }
 };
 
 // Waits for the library's work on a stream (nullptr: the device), the

@@ -85,6 +85,42 @@ public:
     field_tendency(2, tend, state, add_mode);
   }
 
+  // m_aux_state.compute(h, vn, tr, f_vertex, mesh)
+  // (shallow_water_auxstate.cpp:12-50) for a state's current buffers.
+  void compute_aux(const ShallowWaterState &state) const {
+    const DeviceArray3d tr = state.tr_cell_device();
+    check(omega_model_compute_aux(m_handle, state.h_cell_device().ptr,
+                                  state.vn_edge_device().ptr, tr.ptr, m_stream));
+  }
+
+  // The reference's signatures (shallow_water_model.hpp:35-40) on device
+  // arrays: they read the auxiliary state of the last compute_aux(),
+  // compute_tendency() or step, as the reference reads m_aux_state.
+  void compute_h_tendency(DeviceArray2d h_tend_cell, ConstDeviceArray2d h_cell,
+                          ConstDeviceArray2d vn_edge, AddMode add_mode) const {
+    check(omega_model_compute_h_tendency(m_handle, h_tend_cell.ptr, h_cell.ptr, vn_edge.ptr,
+                                         add_mode == AddMode::increment, m_stream));
+  }
+  void compute_vn_tendency(DeviceArray2d vn_tend_edge, ConstDeviceArray2d h_cell,
+                           ConstDeviceArray2d vn_edge, AddMode add_mode) const {
+    check(omega_model_compute_vn_tendency(m_handle, vn_tend_edge.ptr, h_cell.ptr, vn_edge.ptr,
+                                          add_mode == AddMode::increment, m_stream));
+  }
+  void compute_tr_tendency(DeviceArray3d tr_tend_cell, ConstDeviceArray3d tr_cell,
+                           ConstDeviceArray2d vn_edge, AddMode add_mode) const {
+    check(omega_model_compute_tr_tendency(m_handle, tr_tend_cell.ptr, tr_cell.ptr, vn_edge.ptr,
+                                          add_mode == AddMode::increment, m_stream));
+  }
+
+  // Opt-in bottom drag with m_params.m_drag_coeff (the reference carries the
+  // coefficient but applies no drag; its formula is a comment at
+  // shallow_water_hierarchical.cpp:479-485).  Off by default.
+  void enable_bottom_drag(bool on = true) const {
+    int mask = 0;
+    check(omega_model_get_terms(m_handle, &mask));
+    check(omega_model_set_terms(m_handle, on ? (mask | 128) : (mask & ~128)));
+  }
+
   // Sends m_f_vertex to the device (the reference's kernels read the view
   // directly).
   void set_f_vertex() const { check(omega_model_set_f_vertex(m_handle, m_f_vertex.data())); }

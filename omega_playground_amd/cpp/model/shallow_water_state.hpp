@@ -46,6 +46,24 @@ struct ShallowWaterState {
     check(omega_state_copy(handle(), other.handle(), stream));
   }
 
+  // The current device buffers of the fields (device numbering); low-storage
+  // RK alternates two buffers per field, so query again after stepping.
+  DeviceArray2d h_cell_device() const {
+    double *h = nullptr;
+    check(omega_state_device_ptrs(handle(), &h, nullptr, nullptr));
+    return DeviceArray2d{h, m_mesh->m_ncells, m_mesh->m_nlayers};
+  }
+  DeviceArray2d vn_edge_device() const {
+    double *vn = nullptr;
+    check(omega_state_device_ptrs(handle(), nullptr, &vn, nullptr));
+    return DeviceArray2d{vn, m_mesh->m_nedges, m_mesh->m_nlayers};
+  }
+  DeviceArray3d tr_cell_device() const {
+    double *tr = nullptr;
+    check(omega_state_device_ptrs(handle(), nullptr, nullptr, &tr));
+    return DeviceArray3d{tr, m_ntracers, m_mesh->m_ncells, m_mesh->m_nlayers};
+  }
+
   omega_state_t handle() const { return m_handle.get(); }
   const MPASMesh *mesh() const { return m_mesh; }
   Int ntracers() const { return m_ntracers; }

@@ -111,6 +111,7 @@ void ensure_adjoint(omega_model_s *M) {
   OMEGA_REQUIRE(!M->linear, "adjoint: not available for the linear model");
   OMEGA_REQUIRE(M->add_fn == nullptr,
                 "adjoint: not available while an additional-tendency hook is set");
+  OMEGA_REQUIRE(!(M->terms & T_DRAG), "adjoint: the bottom drag term has no adjoint");
   DeviceMesh &dm = M->mesh->dm;
   dm.upload();
   if (!M->mesh->adj.built) build_adj_layout(dm, M->mesh->adj);

@@ -25,6 +25,7 @@ struct omega_batch_s {
     int cur0 = 0;                    // state buffer at parity 0
     DBuf<Real> acc[3], p1[3], p2[3]; // RK4
     DBuf<Real> tend[3];              // LSRK
+    uint64_t gen = 0;                // model configuration the tables hold
   };
   std::vector<std::unique_ptr<Member>> mem;
   int N = 2;         // levels per lane
@@ -38,16 +39,9 @@ struct omega_batch_s {
   DBuf<BatchMember> desc; // [parity][stage][member]
   bool use_graph = false;
   std::map<int, hipGraphExec_t> graphs;
-  hipStream_t cap_stream = nullptr;
 
-  ~omega_batch_s() {
-    drop_graphs();
-    if (cap_stream) (void)hipStreamDestroy(cap_stream);
-  }
-  void drop_graphs() {
-    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
-    graphs.clear();
-  }
+  ~omega_batch_s() { drop_graphs(); }
+  void drop_graphs() { destroy_graphs(graphs); }
 };
 
 namespace {
@@ -88,7 +82,10 @@ void build_tables(omega_batch_s *B) {
     BatchMember &g = geo[i];
     // 16 or 32 layers run the pattern kernels with several cells per
     // wavefront, which needs two levels per lane
-    g.hex = hex_usable(M->mesh->dm.view(), M->lc) && (hm.nlayers % 64 == 0 || N == 2) ? 1 : 0;
+    g.hex = hex_usable(M->mesh->dm.view(), M->lc) && !general_only(M) &&
+                    (hm.nlayers % 64 == 0 || N == 2)
+                ? 1
+                : 0;
     if (g.hex) {
       // one row of lanes covers 64 levels (or all 16 / 32)
       const int span = hm.nlayers % 64 == 0 ? 64 : hm.nlayers;
@@ -115,6 +112,7 @@ void build_tables(omega_batch_s *B) {
     g.vdel4 = do_vn && (M->terms & T_VDEL4);
     B->any_vdel4 = B->any_vdel4 || g.vdel4;
     mb.cur0 = S->cur;
+    mb.gen = M->gen;
   }
   for (int q = 0; q < 2; ++q) B->nblocks[q] = (int)blk[q].size();
   std::vector<BatchMember> tab((size_t)2 * ns * nm);
@@ -183,22 +181,33 @@ void build_tables(omega_batch_s *B) {
   B->built = true;
 }
 
-// The tables hold buffer addresses for the states' current buffer parity;
-// a member stepped on its own in between (or re-allocated buffers) forces
-// a rebuild.
+// The tables hold buffer addresses for the states' current buffer parity
+// and each member's model configuration; a member stepped on its own in
+// between, or a model setting changed since (terms, f_vertex, launch shape,
+// kernel forms: the model's configuration generation), forces a rebuild.
 void ensure_tables(omega_batch_s *B) {
   bool ok = B->built;
   if (ok)
     for (auto &mbp : B->mem) {
       const auto &mb = *mbp;
       const int expect = odd_flip(B) ? (mb.cur0 ^ B->parity) : mb.cur0;
-      if (mb.S->cur != expect) ok = false;
+      if (mb.S->cur != expect || mb.gen != mb.M->gen) ok = false;
     }
   if (!ok) build_tables(B);
 }
 
 void step_launches(omega_batch_s *B, int parity, hipStream_t s) {
   const int nm = (int)B->mem.size();
+  if (B->kind == OMEGA_STEPPER_LSRK)
+    for (auto &mb : B->mem) {
+      // the stage-0 input buffer of this parity is the state's current one
+      omega_state_s *S = mb->S;
+      const int cin = (mb->cur0 ^ (odd_flip(B) ? parity : 0)) & 1;
+      const int keep = S->cur;
+      S->cur = cin;
+      carry_extra_tracers(mb->M, S, s);
+      S->cur = keep;
+    }
   for (int st = 0; st < B->nstages; ++st) {
     const BatchMember *d = B->desc.p + ((size_t)parity * B->nstages + st) * nm;
     const int *g0 = B->blk[0].p, *g1 = B->blk[1].p;
@@ -220,18 +229,17 @@ void batch_step(omega_batch_s *B, hipStream_t s) {
   if (B->use_graph && !stream_capturing(s)) {
     auto it = B->graphs.find(p);
     if (it == B->graphs.end()) {
-      if (!B->cap_stream)
-        OMEGA_HIP_CHECK(hipStreamCreateWithFlags(&B->cap_stream, hipStreamNonBlocking));
+      hipStream_t cs = capture_stream();
       hipGraph_t g;
-      OMEGA_HIP_CHECK(hipStreamBeginCapture(B->cap_stream, hipStreamCaptureModeThreadLocal));
+      OMEGA_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
       try {
-        step_launches(B, p, B->cap_stream);
+        step_launches(B, p, cs);
       } catch (...) {
         hipGraph_t dummy;
-        (void)hipStreamEndCapture(B->cap_stream, &dummy);
+        (void)hipStreamEndCapture(cs, &dummy);
         throw;
       }
-      OMEGA_HIP_CHECK(hipStreamEndCapture(B->cap_stream, &g));
+      OMEGA_HIP_CHECK(hipStreamEndCapture(cs, &g));
       hipGraphExec_t ex;
       OMEGA_HIP_CHECK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
       OMEGA_HIP_CHECK(hipGraphDestroy(g));

@@ -494,6 +494,7 @@ int omega_model_set_f_vertex(omega_model_t M, const double *f) {
     for (int i = 0; i < dm.host.nvertices; ++i) tmp[i] = f[dm.vert_n2o[i]];
     M->f_v.upload(tmp);
     M->f_uniform = rows_equal(tmp, 1);
+    M->touch();
   });
 }
 
@@ -503,6 +504,7 @@ int omega_model_set_additional_tendency(omega_model_t M, omega_additional_tenden
     OMEGA_REQUIRE(M, "null model");
     M->add_fn = fn;
     M->add_user = user;
+    M->touch();
   });
 }
 
@@ -535,11 +537,126 @@ int omega_model_compute_field_tendency(omega_model_t M, int field, omega_state_t
   });
 }
 
+int omega_model_compute_aux(omega_model_t M, const double *h, const double *vn,
+                            const double *tr, omega_stream_t stream) {
+  return guarded([&] {
+    OMEGA_REQUIRE(M, "null model");
+    OMEGA_REQUIRE(h && vn, "compute_aux: h and vn are required");
+    OMEGA_REQUIRE(tr || M->params.ntracers == 0, "compute_aux: tr is required with tracers");
+    if (M->linear) {
+      // the linear model has no auxiliary variables; only the source is kept
+      M->aux_h = h;
+      M->aux_vn = vn;
+      M->aux_tr = tr;
+      return;
+    }
+    run_diag(M, h, vn, tr, as_stream(stream));
+  });
+}
+
+namespace {
+// One field of compute_tendency from device pointers; the thickness and
+// tracers come from the auxiliary state's source, as the reference reads
+// flux_h_edge, mean_h_edge and norm_tr_cell from m_aux_state.
+void field_ptrs(omega_model_t M, int field, double *tend, const double *h, const double *vn,
+                const double *tr, int add_mode, omega_stream_t stream) {
+  OMEGA_REQUIRE(M, "null model");
+  OMEGA_REQUIRE(tend && vn, "null argument");
+  OMEGA_REQUIRE(add_mode == OMEGA_ADD_REPLACE || add_mode == OMEGA_ADD_INCREMENT,
+                "bad add_mode");
+  const char *name[3] = {"compute_h_tendency", "compute_vn_tendency", "compute_tr_tendency"};
+  const Real *hh = M->aux_h, *tt = M->aux_tr;
+  if (M->linear) {
+    // the linear tendencies read their arguments directly
+    // (shallow_water_hierarchical.cpp:680-741)
+    OMEGA_REQUIRE(field != 2, "the linear model has no tracers");
+    hh = field == 1 ? h : (M->aux_h ? M->aux_h : h);
+  } else {
+    OMEGA_REQUIRE(M->aux_h != nullptr,
+                  std::string(name[field]) + ": no auxiliary state (call compute_aux or "
+                                             "compute_tendency first)");
+    if (field == 1)
+      OMEGA_REQUIRE(h == M->aux_h,
+                    "compute_vn_tendency: h must be the thickness the auxiliary state was "
+                    "computed from (its SSH gradient and thickness flux read the same array)");
+    if (field == 2 && M->params.ntracers > 0)
+      OMEGA_REQUIRE(tt != nullptr, "compute_tr_tendency: the auxiliary state has no tracers");
+  }
+  Real *ht = field == 0 ? tend : nullptr;
+  Real *vt = field == 1 ? tend : nullptr;
+  Real *trt = field == 2 ? tend : nullptr;
+  (void)tr;
+  StageArgs a{};
+  a.epi = EPI_STORE;
+  a.base_from_tend = add_mode == OMEGA_ADD_INCREMENT;
+  a.a = 1.0;
+  a.h = fio(hh, ht, nullptr, nullptr, nullptr);
+  a.vn = fio(vn, vt, nullptr, nullptr, nullptr);
+  a.tr = fio(tt, trt, nullptr, nullptr, nullptr);
+  run_stage(M, a, as_stream(stream), 1 << field, false);
+}
+} // namespace
+
+int omega_model_compute_h_tendency(omega_model_t M, double *h_tend, const double *h,
+                                   const double *vn, int add_mode, omega_stream_t stream) {
+  return guarded([&] { field_ptrs(M, 0, h_tend, h, vn, nullptr, add_mode, stream); });
+}
+
+int omega_model_compute_vn_tendency(omega_model_t M, double *vn_tend, const double *h,
+                                    const double *vn, int add_mode, omega_stream_t stream) {
+  return guarded([&] {
+    OMEGA_REQUIRE(h, "compute_vn_tendency: null h");
+    field_ptrs(M, 1, vn_tend, h, vn, nullptr, add_mode, stream);
+  });
+}
+
+int omega_model_compute_tr_tendency(omega_model_t M, double *tr_tend, const double *tr,
+                                    const double *vn, int add_mode, omega_stream_t stream) {
+  return guarded([&] { field_ptrs(M, 2, tr_tend, nullptr, vn, tr, add_mode, stream); });
+}
+
+int omega_model_aux_download(omega_model_t M, const char *name, double *out,
+                             omega_stream_t stream) {
+  return guarded([&] {
+    OMEGA_REQUIRE(M && name && out, "null argument");
+    OMEGA_REQUIRE(!M->linear, "the linear model has no auxiliary variables");
+    const DeviceMesh &dm = M->mesh->dm;
+    const size_t L = dm.host.nlayers;
+    const std::string nm(name);
+    const DiagOut d = M->diag();
+    const Real *src = nullptr;
+    const std::vector<int> *n2o = nullptr;
+    int reps = 1;
+    if (nm == "rvort_vertex") { src = d.rvort_v; n2o = &dm.vert_n2o; }
+    else if (nm == "inv_thick_vertex") { src = d.inv_thick_v; n2o = &dm.vert_n2o; }
+    else if (nm == "ke_cell") { src = d.ke_c; n2o = &dm.cell_n2o; }
+    else if (nm == "vel_div_cell") { src = d.div_c; n2o = &dm.cell_n2o; }
+    else if (nm == "tracer_del2_cell") {
+      src = d.trdel2_c;
+      n2o = &dm.cell_n2o;
+      reps = M->params.ntracers;
+    } else throw Error(OMEGA_ERR_NOTFOUND, "unknown auxiliary array: " + nm);
+    OMEGA_REQUIRE(src != nullptr && reps > 0,
+                  "auxiliary array " + nm + " is not computed for this model's terms");
+    const size_t n = n2o->size() * L * reps;
+    std::vector<double> tmp(n);
+    hipStream_t s = as_stream(stream);
+    OMEGA_HIP_CHECK(hipMemcpyAsync(tmp.data(), src, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    OMEGA_HIP_CHECK(hipStreamSynchronize(s));
+    const size_t per = n2o->size() * L;
+    for (int r = 0; r < reps; ++r)
+      permute_rows(tmp.data() + r * per, out + r * per, *n2o, L, false);
+  });
+}
+
 int omega_model_set_terms(omega_model_t M, int mask) {
   return guarded([&] {
     OMEGA_REQUIRE(M, "null model");
-    OMEGA_REQUIRE((mask & ~127) == 0, "unknown term bits");
+    OMEGA_REQUIRE((mask & ~T_ALL) == 0, "unknown term bits");
+    OMEGA_REQUIRE(!(mask & T_DRAG) || !M->linear,
+                  "bottom drag is a term of the nonlinear model only");
     M->terms = mask;
+    M->touch();
     if (mask & T_VDEL4) ensure_vdel4(M);
     if ((mask & T_TDEL4) && M->params.ntracers > 0 && !M->trdel2_c.p)
       M->trdel2_c.alloc((size_t)M->params.ntracers * M->mesh->dm.host.ncells *
@@ -567,6 +684,7 @@ int omega_model_set_launch(omega_model_t M, int rows, int lpt, int swizzle, int 
     }
     if (swizzle >= 0) M->lc.swizzle = swizzle ? 1 : 0;
     if (chunk >= 0) M->lc.chunk = chunk;
+    M->touch();
   });
 }
 
@@ -575,6 +693,7 @@ int omega_model_set_variant(omega_model_t M, int variant) {
     OMEGA_REQUIRE(M, "null model");
     OMEGA_REQUIRE(variant >= 0 && variant <= 4, "variant must be 0, 1, 2, 3 or 4");
     M->lc.variant = variant;
+    M->touch();
   });
 }
 
@@ -597,6 +716,7 @@ int omega_model_set_tiling(omega_model_t M, int cells_per_tile, int levels_per_b
                   "levels_per_block must be 8, 16 or 32");
     M->lc.tile = cells_per_tile;
     M->lc.tile_k = levels_per_block;
+    M->touch();
   });
 }
 
@@ -606,6 +726,7 @@ int omega_model_kernel_variant(omega_model_t M, int *in_use) {
     const MeshView mv = M->view();
     ensure_tiles(M);
     *in_use = M->linear                                   ? 6
+              : general_only(M)                           ? 1
               : M->lc.variant == 2                        ? 3
               : hextile_usable(mv, M->mesh->tiles, M->lc) ? 5
               : hexu_usable(mv, M->lc)                    ? 4
@@ -619,6 +740,7 @@ int omega_model_set_kernel_forms(omega_model_t M, int forms) {
     OMEGA_REQUIRE(M, "null model");
     OMEGA_REQUIRE(forms >= 0 && forms <= 3, "forms must be 0..3");
     M->forms = forms;
+    M->touch();
   });
 }
 

@@ -56,7 +56,7 @@ __device__ __forceinline__ void cell_diag(const MeshView &m, int terms, int ntr,
                                           const double *__restrict__ tr, const DiagOut &d) {
   const int L = m.L, me = m.me;
   const int n = m.noc[c];
-  if (terms & (T_KE | T_VDEL2 | T_VDEL4)) {
+  if (terms & (T_KE | T_VDEL2 | T_VDEL4 | T_DRAG)) {
     Vd<N> ke = zero<N>(), div = zero<N>();
     for (int j = 0; j < n; ++j) {
       const int e = m.eoc[c * me + j];
@@ -245,6 +245,24 @@ __device__ __forceinline__ void edge_tend(const MeshView &m, const ModelCoef &mc
     const Vd<N> h0 = ld<N>(h + (size_t)c0 * L + k), h1 = ld<N>(h + (size_t)c1 * L + k);
 #pragma unroll
     for (int i = 0; i < N; ++i) T[i] = T[i] - mc.grav * (h1[i] - h0[i]) * idc;
+  }
+  if (a.terms & T_DRAG) {
+    // bottom drag on the edge's deepest active level, the formula left as
+    // a comment at shallow_water_hierarchical.cpp:479-485:
+    // -drag_coeff * sqrt(ke_c0 + ke_c1) * vn * (1 / h_drag), with h_drag the
+    // edge-mean thickness 0.5 * (h_c0 + h_c1)
+    const int ml = m.maxlev_e[e];
+    if (ml - 1 >= k && ml - 1 < k + N) {
+      const int i = ml - 1 - k;
+      const double k0 = d.ke_c[(size_t)c0 * L + k + i], k1 = d.ke_c[(size_t)c1 * L + k + i];
+      const double h0 = h[(size_t)c0 * L + k + i], h1 = h[(size_t)c1 * L + k + i];
+      const double u = vn[idx + i];
+      const double inv_h = 1.0 / (0.5 * (h0 + h1));
+      const double drag = -mc.drag * sqrt(k0 + k1) * u * inv_h;
+#pragma unroll
+      for (int q = 0; q < N; ++q)
+        if (q == i) T[q] = T[q] + drag;
+    }
   }
   if (a.terms & (T_VDEL2 | T_VDEL4)) {
     const int b0 = m.voe[2 * e], b1 = m.voe[2 * e + 1];

@@ -181,7 +181,9 @@ struct Launch {
 
 // Enabled-term bits (same meaning as omega_model_set_terms).
 enum : int {
-  T_PV = 1, T_KE = 2, T_SSH = 4, T_VDEL2 = 8, T_VDEL4 = 16, T_TDEL2 = 32, T_TDEL4 = 64
+  T_PV = 1, T_KE = 2, T_SSH = 4, T_VDEL2 = 8, T_VDEL4 = 16, T_TDEL2 = 32, T_TDEL4 = 64,
+  T_DRAG = 128, // bottom drag, opt-in (no reference kernel applies it)
+  T_ALL = 255
 };
 
 // Stage epilogue applied to a freshly computed tendency value.
@@ -221,6 +223,7 @@ struct ModelCoef {
   // f_v, cv2, cv4, ct2, ct4 equal at every stencil slot (bit for bit) on a
   // regular hexagonal mesh, so a pair wavefront's two cells share them
   int uniform;
+  Real drag; // bottom drag coefficient (term T_DRAG, opt-in)
 };
 
 struct StageArgs {

@@ -21,6 +21,17 @@ using omega_hip::Real;
 #include "adjoint.hpp"
 #include "tiles.hpp"
 
+namespace omega_hip {
+// Destroys cached graph executables after the device has finished every
+// launch of them (a launch may be in flight on any stream).
+template <class Map> void destroy_graphs(Map &graphs) {
+  if (graphs.empty()) return;
+  (void)hipDeviceSynchronize();
+  for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+  graphs.clear();
+}
+} // namespace omega_hip
+
 struct omega_mesh_s {
   DeviceMesh dm;
   DBuf<Real> staging;             // device scratch for omega_add_host_field
@@ -46,6 +57,16 @@ struct omega_model_s {
   omega_params params{};
   int terms = 0;
   Launch lc;
+  // configuration generation: bumped by every setter that changes what a
+  // step's kernels are or read (terms, launch shape, variant, tiling, forms,
+  // f_vertex, hook), so captured graphs and ensemble tables built for an
+  // older configuration are never replayed
+  uint64_t gen = 0;
+  void touch() { ++gen; }
+  // arrays the auxiliary state (diagnostics buffers) was last computed from
+  // (ShallowWaterAuxiliaryState::compute, shallow_water_auxstate.cpp:12-50);
+  // the per-field tendency calls read the thickness and tracers through it
+  const Real *aux_h = nullptr, *aux_vn = nullptr, *aux_tr = nullptr;
   DBuf<Real> f_v, cv2, cv4, ct2, ct4;
   DBuf<Real> rvort_v, inv_thick_v, ke_c, div_c, trdel2_c, vdel2_e, vdel2_rv_v, vdel2_div_c;
   DBuf<Real> partial, result;
@@ -83,7 +104,7 @@ struct omega_model_s {
   }
   ModelCoef coef() const {
     return ModelCoef{params.grav, f_v.p, cv2.p, cv4.p, ct2.p, ct4.p,
-                     f_uniform && mix_uniform ? 1 : 0};
+                     f_uniform && mix_uniform ? 1 : 0, params.drag_coeff};
   }
   // step adjoint: diagnostics of the recomputed stage inputs, one set per
   // stage input (stage 0 uses the buffers above), kept for the reverse
@@ -104,17 +125,20 @@ struct omega_model_s {
   }
 };
 
-// A captured step is valid only for the exact device buffers it recorded:
-// the key holds dt, the buffer parity and every buffer address the step
-// touches, so a state or scratch buffer reallocated at another address
-// never replays a stale graph.
+// A captured step is valid only for the exact device buffers and model
+// configuration it recorded: the key holds dt, the buffer parity, the
+// model's configuration generation and every buffer address the step
+// touches, so a reallocated buffer or a changed setting never replays a
+// stale graph.
 struct GraphKey {
   double dt;
   int cur;
+  uint64_t gen;
   std::vector<const void *> bufs;
   bool operator<(const GraphKey &o) const {
     if (dt != o.dt) return dt < o.dt;
     if (cur != o.cur) return cur < o.cur;
+    if (gen != o.gen) return gen < o.gen;
     return bufs < o.bufs;
   }
 };
@@ -132,11 +156,7 @@ struct omega_stepper_s {
   // step adjoint: recomputed stage states (RK4 stage inputs 2..4, LSRK stage
   // inputs 2..S), their tendency, the incoming cotangent and stage cotangents
   DBuf<Real> adj_y[4][3], adj_t[3], adj_gy[3], adj_pb[3], adj_kb[3];
-  hipStream_t cap_stream = nullptr; // private stream the step is captured on
-  ~omega_stepper_s() {
-    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
-    if (cap_stream) (void)hipStreamDestroy(cap_stream);
-  }
+  ~omega_stepper_s() { omega_hip::destroy_graphs(graphs); }
 };
 
 // ---------------------------------------------------------------------------
@@ -164,6 +184,9 @@ template <class F> int guarded(F &&f) {
 
 inline hipStream_t as_stream(omega_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 bool stream_capturing(hipStream_t s);
+// The stream steps are captured on: one per device for the whole process
+// (the null stream cannot be captured), created on first use and kept.
+hipStream_t capture_stream();
 
 void model_build_coefficients(omega_model_s *M);
 void ensure_vdel4(omega_model_s *M);
@@ -173,8 +196,13 @@ void model_build_linear(omega_model_s *M);
 int tile_cells(const omega_model_s *M);
 void ensure_tiles(omega_model_s *M);
 void run_diag(omega_model_s *M, const Real *h, const Real *vn, const Real *tr, hipStream_t s);
-// fields: bit 0 h, bit 1 vn, bit 2 tracers (all by default)
-void run_stage(omega_model_s *M, StageArgs a, hipStream_t s, int fields = 7);
+// fields: bit 0 h, bit 1 vn, bit 2 tracers (all by default); with_diag =
+// false reuses the diagnostics of the last run_diag
+void run_stage(omega_model_s *M, StageArgs a, hipStream_t s, int fields = 7,
+               bool with_diag = true);
+// true when the model's terms need the general kernels (bottom drag)
+bool general_only(const omega_model_s *M);
+void carry_extra_tracers(omega_model_s *M, omega_state_s *S, hipStream_t s);
 FieldIO fio(const Real *in, Real *tend, Real *acc, const Real *yold, Real *out);
 void check_state(omega_model_s *M, omega_state_s *S, const char *what);
 void ensure3(DBuf<Real> (&b)[3], omega_state_s *S);

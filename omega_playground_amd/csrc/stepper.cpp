@@ -3,6 +3,7 @@
 // steppers and their HIP-graph capture.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "objects.hpp"
 
@@ -94,10 +95,31 @@ void ensure_tiles(omega_model_s *M) {
   if (mesh->tiles.tcells != T) build_tile_tables(mesh->dm, T, mesh->tiles);
 }
 
+hipStream_t capture_stream() {
+  static std::mutex mu;
+  static std::map<int, hipStream_t> streams;
+  int dev = 0;
+  OMEGA_HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(mu);
+  auto it = streams.find(dev);
+  if (it != streams.end()) return it->second;
+  hipStream_t cs = nullptr;
+  OMEGA_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  streams.emplace(dev, cs);
+  return cs;
+}
+
+// The opt-in bottom drag is evaluated by the general kernels only.
+bool general_only(const omega_model_s *M) { return (M->terms & T_DRAG) != 0; }
+
 // Diagnostics of a state (vertex vorticity and thickness, cell kinetic
 // energy and divergence, tracer Laplacian, momentum del4 pre-pass) into the
-// model's buffers; only what the enabled terms read.
+// model's buffers; only what the enabled terms read.  Records the arrays as
+// the auxiliary state's source.
 void run_diag(omega_model_s *M, const Real *h, const Real *vn, const Real *tr, hipStream_t s) {
+  M->aux_h = h;
+  M->aux_vn = vn;
+  M->aux_tr = tr;
   const MeshView mv = M->view();
   const int ntr = M->params.ntracers;
   const bool do_vn = !M->params.disable_vn_tendency;
@@ -106,13 +128,16 @@ void run_diag(omega_model_s *M, const Real *h, const Real *vn, const Real *tr, h
   if (M->terms & T_VDEL4) ensure_vdel4(M);
   const DiagOut d = M->diag();
   const int dterms = do_vn ? M->terms : (M->terms & T_TDEL4);
-  if (hex_usable(mv, M->lc)) launch_diag_hex(mv, M->lc, dterms, ntr, h, vn, tr, d, s);
+  if (hex_usable(mv, M->lc) && !general_only(M))
+    launch_diag_hex(mv, M->lc, dterms, ntr, h, vn, tr, d, s);
   else launch_diag(mv, M->lc, M->coef(), dterms, ntr, h, vn, tr, d, s);
   if (do_vn && (M->terms & T_VDEL4)) launch_vdel4(mv, M->lc, d, s);
 }
 
-// One tendency evaluation (+ optional fused stage update).
-void run_stage(omega_model_s *M, StageArgs a, hipStream_t s, int fields) {
+// One tendency evaluation (+ optional fused stage update).  with_diag =
+// false reads the diagnostics an earlier run_diag left in the model's
+// buffers (the per-field tendency calls after compute_aux).
+void run_stage(omega_model_s *M, StageArgs a, hipStream_t s, int fields, bool with_diag) {
   const MeshView mv = M->view();
   const ModelCoef mc = M->coef();
   a.terms = M->terms;
@@ -123,8 +148,12 @@ void run_stage(omega_model_s *M, StageArgs a, hipStream_t s, int fields) {
     launch_tend_linear(mv, M->lc, M->lin_coef(), a, s);
     return;
   }
-  run_diag(M, a.h.in, a.vn.in, a.tr.in, s);
+  if (with_diag) run_diag(M, a.h.in, a.vn.in, a.tr.in, s);
   const DiagOut d = M->diag();
+  if (general_only(M)) {
+    launch_tend(mv, M->lc, mc, a, d, s);
+    return;
+  }
   if (M->lc.variant == 2 || M->lc.variant == 4) ensure_tiles(M);
   if (M->lc.variant == 2) {
     launch_tend_tile(mv, M->mesh->tiles.view(), M->lc, mc, a, d, s);
@@ -263,11 +292,24 @@ void rk4_unfused_step(omega_stepper_s *St, omega_state_s *S, double t, double dt
   }
 }
 
+// Tracer rows of a state beyond the model's tracer count are not stepped
+// (the reference's loops run over the model's ntracers, lsrk.cpp:47-87);
+// steppers that alternate the state between two buffers copy those rows to
+// the other buffer first, so they read the same from either.
+void carry_extra_tracers(omega_model_s *M, omega_state_s *S, hipStream_t s) {
+  const size_t per = S->ntr > 0 ? S->n[2] / S->ntr : 0;
+  const size_t first = (size_t)M->params.ntracers * per;
+  if (S->ntr <= M->params.ntracers || !per) return;
+  OMEGA_HIP_CHECK(hipMemcpyAsync(S->alt(2) + first, S->ptr(2) + first,
+                                 (S->n[2] - first) * sizeof(Real), hipMemcpyDeviceToDevice, s));
+}
+
 // Low-storage RK with the state alternating between two buffers per stage
 // (lsrk.cpp:36-89 fused: tend = a*tend + F(y); y' = y + dt*b*tend).
 void lsrk_fused_step(omega_stepper_s *St, omega_state_s *S, double dt, hipStream_t s) {
   omega_model_s *M = St->model;
   ensure3(St->tend, S);
+  carry_extra_tracers(M, S, s);
   for (int stage = 0; stage < St->nstages; ++stage) {
     StageArgs a{};
     a.epi = EPI_LSRK;
@@ -331,26 +373,23 @@ void do_step_maybe_graph(omega_stepper_s *St, omega_state_s *S, double t, double
   omega_model_s *M = St->model;
   if (M->terms & T_VDEL4) ensure_vdel4(M);
   ensure_tiles(M);
-  GraphKey key{dt, S->cur, {}};
+  GraphKey key{dt, S->cur, M->gen, {}};
   for (int f = 0; f < 3; ++f)
     key.bufs.insert(key.bufs.end(), {S->buf[0][f].p, S->buf[1][f].p, St->acc[f].p,
                                      St->p1[f].p, St->p2[f].p, St->tend[f].p});
   key.bufs.insert(key.bufs.end(), {M->vdel2_e.p, M->trdel2_c.p, M->mesh->tiles.ue.p,
-                                   M->mesh->tiles.cc.p, M->mesh->tiles.hexl.p});
+                                   M->mesh->tiles.cc.p, M->mesh->tiles.hexl.p, M->f_v.p});
+  const DiagOut dk = M->diag();
+  key.bufs.insert(key.bufs.end(), {dk.rvort_v, dk.inv_thick_v, dk.ke_c, dk.div_c, dk.trdel2_c,
+                                   dk.vdel2_e, dk.vdel2_rv_v, dk.vdel2_div_c});
   auto it = St->graphs.find(key);
   if (it == St->graphs.end()) {
     // bound the cache: states created and destroyed in a loop (the adjoint's
     // work states) would otherwise leave a graph each behind
-    if (St->graphs.size() >= 16) {
-      OMEGA_HIP_CHECK(hipStreamSynchronize(s));
-      for (auto &kv : St->graphs) (void)hipGraphExecDestroy(kv.second);
-      St->graphs.clear();
-    }
-    // capture on a private stream (the null stream cannot be captured); the
-    // graph is then launched on the caller's stream
-    if (!St->cap_stream)
-      OMEGA_HIP_CHECK(hipStreamCreateWithFlags(&St->cap_stream, hipStreamNonBlocking));
-    hipStream_t cs = St->cap_stream;
+    if (St->graphs.size() >= 16) destroy_graphs(St->graphs);
+    // capture on the process's capture stream (the null stream cannot be
+    // captured); the graph is then launched on the caller's stream
+    hipStream_t cs = capture_stream();
     const int cur0 = S->cur;
     hipGraph_t g;
     OMEGA_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));

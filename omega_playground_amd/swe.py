@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(_HERE, "libomega_swe.so")
 
 ORDER = {"original": 0, "hilbert": 1, "morton": 2}
 ADD_REPLACE, ADD_INCREMENT = 0, 1
-PV, KE, SSH, VDEL2, VDEL4, TDEL2, TDEL4 = 1, 2, 4, 8, 16, 32, 64
+PV, KE, SSH, VDEL2, VDEL4, TDEL2, TDEL4, DRAG = 1, 2, 4, 8, 16, 32, 64, 128
 
 _lib = None
 
@@ -98,6 +98,11 @@ def lib():
         "omega_model_compute_tendency": ([vp, vp, vp, C.c_double, C.c_int, vp], C.c_int),
         "omega_model_set_terms": ([vp, C.c_int], C.c_int),
         "omega_model_compute_field_tendency": ([vp, C.c_int, vp, vp, C.c_int, vp], C.c_int),
+        "omega_model_compute_aux": ([vp, vp, vp, vp, vp], C.c_int),
+        "omega_model_compute_h_tendency": ([vp, vp, vp, vp, C.c_int, vp], C.c_int),
+        "omega_model_compute_vn_tendency": ([vp, vp, vp, vp, C.c_int, vp], C.c_int),
+        "omega_model_compute_tr_tendency": ([vp, vp, vp, vp, C.c_int, vp], C.c_int),
+        "omega_model_aux_download": ([vp, C.c_char_p, dp, vp], C.c_int),
         "omega_stepper_coeffs": ([vp, ip, dp, dp, dp], C.c_int),
         "omega_model_get_terms": ([vp, ip], C.c_int),
         "omega_model_set_launch": ([vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
@@ -474,6 +479,51 @@ class ShallowWaterModel:
     def compute_tr_tendency(self, tend, state, add_mode: int = ADD_REPLACE, stream=None):
         """compute_tr_tendency (shallow_water_model.cpp:221-311): only tend's tracers."""
         self._field_tendency(2, tend, state, add_mode, stream)
+
+    def compute_aux(self, h, vn, tr=None, stream=None):
+        """ShallowWaterAuxiliaryState::compute (shallow_water_auxstate.cpp:12-50)
+        from device arrays (pointers as ints, device numbering), or from a
+        ShallowWaterState passed as h."""
+        if isinstance(h, ShallowWaterState):
+            h, vn, tr = h.device_ptrs()
+        check(lib().omega_model_compute_aux(self._h, C.c_void_p(h), C.c_void_p(vn),
+                                            C.c_void_p(tr or 0), _stream(stream)))
+
+    def compute_h_tendency_dev(self, h_tend, h, vn, add_mode: int = ADD_REPLACE, stream=None):
+        """compute_h_tendency(h_tend_cell, h_cell, vn_edge, add_mode)
+        (shallow_water_model.hpp:35-36) on device pointers; reads the
+        auxiliary state of the last compute_aux / tendency / step."""
+        check(lib().omega_model_compute_h_tendency(self._h, C.c_void_p(h_tend), C.c_void_p(h),
+                                                   C.c_void_p(vn), int(add_mode),
+                                                   _stream(stream)))
+
+    def compute_vn_tendency_dev(self, vn_tend, h, vn, add_mode: int = ADD_REPLACE, stream=None):
+        """compute_vn_tendency(vn_tend_edge, h_cell, vn_edge, add_mode)
+        (shallow_water_model.hpp:37-38) on device pointers."""
+        check(lib().omega_model_compute_vn_tendency(self._h, C.c_void_p(vn_tend), C.c_void_p(h),
+                                                    C.c_void_p(vn), int(add_mode),
+                                                    _stream(stream)))
+
+    def compute_tr_tendency_dev(self, tr_tend, tr, vn, add_mode: int = ADD_REPLACE, stream=None):
+        """compute_tr_tendency(tr_tend_cell, tr_cell, vn_edge, add_mode)
+        (shallow_water_model.hpp:39-40) on device pointers."""
+        check(lib().omega_model_compute_tr_tendency(self._h, C.c_void_p(tr_tend),
+                                                    C.c_void_p(tr or 0), C.c_void_p(vn),
+                                                    int(add_mode), _stream(stream)))
+
+    def aux_array(self, name: str, stream=None) -> np.ndarray:
+        """Host copy (original numbering) of an auxiliary-state array:
+        rvort_vertex, inv_thick_vertex, ke_cell, vel_div_cell, tracer_del2_cell."""
+        m = self.mesh
+        L = m.nlayers
+        shape = {"rvort_vertex": (m.nvertices, L), "inv_thick_vertex": (m.nvertices, L),
+                 "ke_cell": (m.ncells, L), "vel_div_cell": (m.ncells, L),
+                 "tracer_del2_cell": (self.params.ntracers, m.ncells, L)}.get(name)
+        if shape is None:
+            raise OmegaError(f"unknown auxiliary array {name}")
+        out = np.empty(shape)
+        check(lib().omega_model_aux_download(self._h, name.encode(), _dp(out), _stream(stream)))
+        return out
 
     @property
     def terms(self) -> int:

@@ -80,6 +80,7 @@ def lib():
         L.orc_model_get_terms.restype = C.c_int
         L.orc_aux_compute.argtypes = [vp, dp, dp, dp]
         L.orc_compute_tendency.argtypes = [vp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_int]
+        L.orc_compute_field_tendency.argtypes = [vp, C.c_int, dp, dp, dp, C.c_int]
         L.orc_stepper_create.restype = vp
         L.orc_stepper_create.argtypes = [vp, C.c_int, C.c_int]
         L.orc_stepper_free.argtypes = [vp]
@@ -190,7 +191,7 @@ class Mesh:
             self._h = None
 
 
-PV, KE, SSH, VDEL2, VDEL4, TDEL2, TDEL4 = 1, 2, 4, 8, 16, 32, 64
+PV, KE, SSH, VDEL2, VDEL4, TDEL2, TDEL4, DRAG = 1, 2, 4, 8, 16, 32, 64, 128
 
 
 class Model:
@@ -253,6 +254,13 @@ class Model:
 
     def aux_compute(self, h, vn, tr):
         lib().orc_aux_compute(self._h, _dptr(h), _dptr(vn), _dptr(tr))
+
+    def compute_field_tendency(self, field, tend, a, vn, increment=False):
+        """compute_h_tendency (field 0, a = h), compute_vn_tendency (1, a = h)
+        or compute_tr_tendency (2, a = tr), reading the auxiliary state of
+        the last aux_compute."""
+        lib().orc_compute_field_tendency(self._h, int(field), _dptr(tend), _dptr(a), _dptr(vn),
+                                         int(bool(increment)))
 
     def compute_tendency(self, tend, state, t=0.0, increment=False):
         ht, vt, tt = tend

@@ -31,6 +31,7 @@
 //  7 kinetic energy without 1/4                       8 vertex circulation sign flipped
 //  9 linear model: SSH gradient sign                 10 linear model: h0 dropped from div
 // 11 linear model: energy weighs K with h, not h0
+// 12 bottom drag applied on the top level instead of the deepest one
 // 0 (the default) is the restatement itself.
 #ifndef ORC_MUTANT
 #define ORC_MUTANT 0
@@ -544,6 +545,17 @@ void Model::compute_vn_tendency(Real *vn_tend, const Real *h, const Real *vn,
         vt -= (mutant == 2 ? -grav : grav) * (h[(size_t)c1 * L + k] - h[(size_t)c0 * L + k]) *
               inv_dc_edge;
       }
+      // opt-in bottom drag (term bit 128): the formula left as a comment at
+      // src/shallow_water_hierarchical.cpp:479-485, on the edge's deepest
+      // active level, with h_drag_edge = mean_h_edge
+      if (en_drag && k == (mutant == 12 ? 0 : M.max_level_edge_top[ie] - 1)) {
+        const Real inv_h_drag_edge = 1. / mean_h_edge[(size_t)ie * L + k];
+        const Real drag_force =
+            -params.drag_coeff *
+            std::sqrt(ke_cell[(size_t)c0 * L + k] + ke_cell[(size_t)c1 * L + k]) *
+            vn[(size_t)ie * L + k] * inv_h_drag_edge;
+        vt += drag_force;
+      }
       if (en_vdel2) { // vel_diff_edge.hpp:61-86
         const Real del2u =
             ((vel_div_cell[(size_t)c1 * L + k] - vel_div_cell[(size_t)c0 * L + k]) * inv_dc_edge -
@@ -996,14 +1008,27 @@ void orc_model_set_terms(void *mp, int mask) {
   Model *m = static_cast<Model *>(mp);
   m->en_pv = mask & 1; m->en_ke = mask & 2; m->en_ssh = mask & 4; m->en_vdel2 = mask & 8;
   m->en_vdel4 = mask & 16; m->en_tdel2 = mask & 32; m->en_tdel4 = mask & 64;
+  m->en_drag = mask & 128;
 }
 int orc_model_get_terms(void *mp) {
   Model *m = static_cast<Model *>(mp);
   return (m->en_pv ? 1 : 0) | (m->en_ke ? 2 : 0) | (m->en_ssh ? 4 : 0) | (m->en_vdel2 ? 8 : 0) |
-         (m->en_vdel4 ? 16 : 0) | (m->en_tdel2 ? 32 : 0) | (m->en_tdel4 ? 64 : 0);
+         (m->en_vdel4 ? 16 : 0) | (m->en_tdel2 ? 32 : 0) | (m->en_tdel4 ? 64 : 0) |
+         (m->en_drag ? 128 : 0);
 }
 void orc_aux_compute(void *mp, const double *h, const double *vn, const double *tr) {
   static_cast<Model *>(mp)->aux_compute(h, vn, tr);
+}
+// one field of compute_tendency after orc_aux_compute: field 0
+// compute_h_tendency(tend, a=h, vn), 1 compute_vn_tendency(tend, a=h, vn),
+// 2 compute_tr_tendency(tend, a=tr, vn)
+void orc_compute_field_tendency(void *mp, int field, double *tend, const double *a,
+                                const double *vn, int add_mode) {
+  Model *m = static_cast<Model *>(mp);
+  const AddMode mode = add_mode ? increment : replace;
+  if (field == 0) m->compute_h_tendency(tend, a, vn, mode);
+  else if (field == 1) m->compute_vn_tendency(tend, a, vn, mode);
+  else m->compute_tr_tendency(tend, a, vn, mode);
 }
 void orc_compute_tendency(void *mp, double *h_tend, double *vn_tend, double *tr_tend,
                           const double *h, const double *vn, const double *tr, double t,

@@ -83,6 +83,9 @@ struct Model {
   Params params;
   bool en_pv = true, en_ke = true, en_ssh = true, en_vdel2 = false,
        en_vdel4 = false, en_tdel2 = false, en_tdel4 = false;
+  // opt-in bottom drag (shallow_water_hierarchical.cpp:479-485, commented
+  // out there; the active reference model never applies it)
+  bool en_drag = false;
 
   // auxiliary state (src/model/shallow_water_auxstate.hpp:10-22)
   std::vector<Real> rvort_vertex, norm_rvort_vertex, norm_pvort_vertex;

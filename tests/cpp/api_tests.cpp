@@ -6,6 +6,8 @@
 //   api_tests ensemble_adjoint   Ensemble::adjoint_run == per-member adjoint_run
 //   api_tests linear             LinearShallowWaterModel: RK4 keeps mass and
 //                                (to its own O(dt^5) dissipation) energy
+//   api_tests fields             compute_aux + compute_{h,vn,tr}_tendency on
+//                                device arrays == compute_tendency
 //
 // Results are compared bit for bit (a gradient of the shared coefficients
 // to 1e-12, since it sums members in another order).  Exit status 0 on
@@ -187,6 +189,54 @@ void linear() {
   require(std::abs(m1 - m0) <= 1e-13 * std::abs(m0), "linear mass conserved");
 }
 
+// The reference's per-field calls (shallow_water_model.hpp:35-40) on device
+// arrays after compute_aux give the fields of compute_tendency, in replace
+// and increment mode.
+void fields() {
+  auto mesh = std::make_unique<PlanarHexagonalMesh>(12, 12, 10e3, 64);
+  auto p = params(2);
+  p.m_eddy_diff2 = 1e2;
+  ShallowWaterModel model(mesh.get(), p);
+  ShallowWaterState s(mesh.get(), p), full(mesh.get(), p), part(mesh.get(), p);
+  fill(*mesh, s, 0.7);
+  model.compute_tendency(full, s, 0.0);
+  model.compute_aux(s);
+  model.compute_h_tendency(part.h_cell_device(), s.h_cell_device(), s.vn_edge_device(),
+                           AddMode::replace);
+  model.compute_vn_tendency(part.vn_edge_device(), s.h_cell_device(), s.vn_edge_device(),
+                            AddMode::replace);
+  model.compute_tr_tendency(part.tr_cell_device(), s.tr_cell_device(), s.vn_edge_device(),
+                            AddMode::replace);
+  require(same(full, part), "per-field tendencies equal compute_tendency");
+  // increment mode adds the same tendency again: exactly twice the values
+  model.compute_h_tendency(part.h_cell_device(), s.h_cell_device(), s.vn_edge_device(),
+                           AddMode::increment);
+  model.compute_vn_tendency(part.vn_edge_device(), s.h_cell_device(), s.vn_edge_device(),
+                            AddMode::increment);
+  model.compute_tr_tendency(part.tr_cell_device(), s.tr_cell_device(), s.vn_edge_device(),
+                            AddMode::increment);
+  full.copy_to_host();
+  part.copy_to_host();
+  bool twice = true;
+  for (size_t i = 0; i < full.m_h_cell.size(); ++i)
+    twice = twice && part.m_h_cell.data()[i] == 2 * full.m_h_cell.data()[i];
+  for (size_t i = 0; i < full.m_vn_edge.size(); ++i)
+    twice = twice && part.m_vn_edge.data()[i] == 2 * full.m_vn_edge.data()[i];
+  for (size_t i = 0; i < full.m_tr_cell.size(); ++i)
+    twice = twice && part.m_tr_cell.data()[i] == 2 * full.m_tr_cell.data()[i];
+  require(twice, "increment mode adds the field tendency");
+  // compute_vn_tendency with a thickness other than the auxiliary state's
+  bool threw = false;
+  try {
+    model.compute_vn_tendency(part.vn_edge_device(), part.h_cell_device(), s.vn_edge_device(),
+                              AddMode::replace);
+  } catch (const std::runtime_error &) {
+    threw = true;
+  }
+  require(threw, "vn tendency with another thickness array is rejected");
+  std::cout << "fields: per-field device-array tendencies equal compute_tendency" << std::endl;
+}
+
 } // namespace
 
 int main(int argc, char **argv) {
@@ -196,8 +246,10 @@ int main(int argc, char **argv) {
     else if (what == "ensemble") ensemble();
     else if (what == "ensemble_adjoint") ensemble_adjoint();
     else if (what == "linear") linear();
+    else if (what == "fields") fields();
     else {
-      std::cerr << "usage: api_tests adjoint|ensemble|ensemble_adjoint|linear" << std::endl;
+      std::cerr << "usage: api_tests adjoint|ensemble|ensemble_adjoint|linear|fields"
+                << std::endl;
       return 2;
     }
   } catch (const std::exception &e) {

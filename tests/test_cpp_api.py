@@ -66,6 +66,6 @@ def test_cpp_tracer_convergence(kind):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("what", ["adjoint", "ensemble", "ensemble_adjoint", "linear"])
+@pytest.mark.parametrize("what", ["adjoint", "ensemble", "ensemble_adjoint", "linear", "fields"])
 def test_cpp_adjoint_and_ensemble_classes(what):
     _run(what, exe=API_BIN)

@@ -309,20 +309,31 @@ def test_specialised_each_term_bitwise(terms):
         assert same_bits(a, b)
 
 
+@pytest.mark.parametrize("L", [4, 64])
 @pytest.mark.parametrize("field", [0, 1, 2])
-def test_field_tendency_matches_full_tendency(field):
-    """compute_{h,vn,tr}_tendency write exactly their field of compute_tendency."""
-    om, omodel, gm, gmodel, state = make_pair(9, 9, 4, 2)
+def test_field_tendency_matches_oracle(field, L):
+    """compute_{h,vn,tr}_tendency on whole states write exactly their field,
+    with the bits of the oracle's compute_{h,vn,tr}_tendency after its aux
+    pass (shallow_water_model.cpp:43-311), in replace and increment mode;
+    4 layers run the general kernels, 64 the pattern-specialised ones."""
+    om, omodel, gm, gmodel, state = make_pair(9, 9, L, 2)
+    print("field", field, "L", L, flush=True)
     gs = upload(gm, 2, state)
-    full = swe.ShallowWaterState(gm, 2)
-    gmodel.compute_tendency(full, gs, 0.0)
     rng = np.random.default_rng(field)
     junk = tuple(rng.standard_normal(a.shape) for a in state)
     part = upload(gm, 2, junk)
-    (gmodel.compute_h_tendency, gmodel.compute_vn_tendency, gmodel.compute_tr_tendency)[field](
-        part, gs)
-    for f, (a, b, j) in enumerate(zip(part.download(), full.download(), junk)):
-        assert same_bits(a, b if f == field else j), f
+    fn = (gmodel.compute_h_tendency, gmodel.compute_vn_tendency, gmodel.compute_tr_tendency)[field]
+    fn(part, gs)
+    h, vn, tr = state
+    omodel.aux_compute(h, vn, tr)
+    ref = junk[field].copy()
+    omodel.compute_field_tendency(field, ref, tr if field == 2 else h, vn)
+    for f, (a, j) in enumerate(zip(part.download(), junk)):
+        assert same_bits(a, ref if f == field else j), f
+    # increment mode adds to what the field holds
+    fn(part, gs, swe.ADD_INCREMENT)
+    omodel.compute_field_tendency(field, ref, tr if field == 2 else h, vn, increment=True)
+    assert same_bits(part.download()[field], ref)
 
 
 def test_stepper_coefficients_match_oracle():

@@ -1,5 +1,5 @@
 """Mutation check of the oracle's pinning: each build of the oracle with a
-planted plausible mistake (swe_oracle.cpp, ORC_MUTANT = 1..11: a dropped
+planted plausible mistake (swe_oracle.cpp, ORC_MUTANT = 1..12: a dropped
 term, a flipped sign, a swapped operand or coefficient) must fail at least
 one of the pinning tests.  The unmodified oracle passes them all
 (test_oracle_pinning.py, test_oracle_reference.py, test_linear_model.py)."""
@@ -25,6 +25,7 @@ EXPECTED = {
     9: ("test_linear_model.py", "test_linear_energy_is_conserved"),     # linear SSH sign
     10: ("test_linear_model.py", "test_linear_equals_nonlinear_at_rest"),  # h0 dropped
     11: ("test_linear_model.py", "test_linear_energy_uses_h0"),         # energy with h
+    12: "test_bottom_drag_uniform_flow_decay",      # drag on the top level
 }
 
 

@@ -338,3 +338,35 @@ def test_mesh_invariants():
             W[(e, m.edges_on_edge[e, j])] = W.get((e, m.edges_on_edge[e, j]), 0) + m.weights_on_edge[e, j]
     for (e, f), w in W.items():
         assert abs(w + W.get((f, e), 0.0)) < 1e-15
+
+
+# --- opt-in bottom drag (shallow_water_hierarchical.cpp:479-485) ------------
+def test_bottom_drag_uniform_flow_decay():
+    """A uniform flow of speed U in every layer with f = 0 and uniform h has
+    no vorticity, divergence, or KE and SSH gradients, and KE = U^2/2 in
+    every cell.  With the drag term on, the deepest layer's tendency is then
+    -C_d |U| u_e / H on every edge, exactly to rounding, and the other layers
+    get none; stepping the flow follows dU/dt = -C_d U^2 / H, whose solution
+    U(t) = U0 / (1 + C_d U0 t / H) RK4 matches to its truncation error.  The
+    term is off by default (the reference applies no drag)."""
+    L = 3
+    m = po.Mesh(12, 10, 1000.0, L)
+    U0, ang, H, cd = 0.8, 0.4, 50.0, 2e-3
+    vn = uniform_vn(m, L, U0 * math.cos(ang), U0 * math.sin(ang))
+    h = np.full((m.ncells, L), H)
+    trz = np.zeros((0, m.ncells, L))
+    model = po.Model(m, f0=0.0, grav=9.81, drag_coeff=cd)
+    t_off = tendency(model, (h, vn, trz))
+    assert np.max(np.abs(t_off[1])) < 1e-15
+    model.terms = model.terms | po.DRAG
+    t = tendency(model, (h, vn, trz))
+    expect = -cd * U0 * vn[:, L - 1] / H
+    assert np.max(np.abs(t[1][:, L - 1] - expect)) <= 1e-12 * np.max(np.abs(expect))
+    assert np.max(np.abs(t[1][:, :L - 1])) < 1e-15
+    assert np.max(np.abs(t[0])) < 1e-15
+    dt, n = 100.0, 50
+    st = (h.copy(), vn.copy(), trz.copy())
+    po.Stepper(model, "rk4").run(0.0, dt, n, st)
+    U = U0 / (1 + cd * U0 * n * dt / H)
+    assert np.max(np.abs(st[1][:, L - 1] - vn[:, L - 1] * (U / U0))) <= 1e-9 * U0
+    assert np.max(np.abs(st[1][:, :L - 1] - vn[:, :L - 1])) < 1e-13
