@@ -480,7 +480,7 @@ class ShallowWaterModel:
         """compute_tr_tendency (shallow_water_model.cpp:221-311): only tend's tracers."""
         self._field_tendency(2, tend, state, add_mode, stream)
 
-    def compute_aux(self, h, vn, tr=None, stream=None):
+    def compute_aux(self, h, vn=None, tr=None, stream=None):
         """ShallowWaterAuxiliaryState::compute (shallow_water_auxstate.cpp:12-50)
         from device arrays (pointers as ints, device numbering), or from a
         ShallowWaterState passed as h."""
