@@ -226,6 +226,22 @@ int omega_model_compute_field_tendency(omega_model_t model, int field, omega_sta
                                        omega_state_t state, int add_mode,
                                        omega_stream_t stream);
 
+/* omega_model_compute_tendency without any allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The diagnostics the tendency kernels pass between launches
+ * live in one static device array of 64 MiB per GPU (held until the process
+ * ends) instead of the model's buffers; models whose enabled terms need more
+ * (about (12 + ntracers) * ncells * nlayers doubles with every mixing term
+ * on) get OMEGA_ERR_ARG.  The results are the bits of
+ * omega_model_compute_tendency; the model's buffers and auxiliary-state
+ * record are not touched.  The array keeps the last call's contents and is
+ * shared by every call on the GPU, so calls that overlap on one GPU race.
+ * The additional-tendency hook, when set, runs as in compute_tendency. */
+int omega_model_compute_tendency_noalloc(omega_model_t model, omega_state_t tend,
+                                         omega_state_t state, double t, int add_mode,
+                                         omega_stream_t stream);
+
 /* Replaces ShallowWaterAuxiliaryState::compute(h, vn, tr, f_vertex, mesh)
  * (src/model/shallow_water_auxstate.cpp:12-50), the aux pass that
  * compute_tendency runs first (shallow_water_model.cpp:317-319): vertex

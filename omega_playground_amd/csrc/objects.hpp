@@ -202,6 +202,17 @@ void run_stage(omega_model_s *M, StageArgs a, hipStream_t s, int fields = 7,
                bool with_diag = true);
 // true when the model's terms need the general kernels (bottom drag)
 bool general_only(const omega_model_s *M);
+// true when the enabled terms read diagnostics (vn tendency, tracer del4)
+bool diag_needed(const omega_model_s *M);
+// diagnostics into caller-provided buffers (no model state is touched)
+void run_diag_into(omega_model_s *M, const Real *h, const Real *vn, const Real *tr,
+                   const DiagOut &d, hipStream_t s);
+// run_stage with the diagnostics in dx (the default kernels), or, with
+// dx == nullptr, exactly run_stage
+void run_stage_into(omega_model_s *M, StageArgs a, hipStream_t s, int fields,
+                    const DiagOut *dx);
+void call_hook(omega_model_s *M, Real *ht, Real *vt, const Real *h, const Real *vn, double t,
+               hipStream_t s);
 void carry_extra_tracers(omega_model_s *M, omega_state_s *S, hipStream_t s);
 FieldIO fio(const Real *in, Real *tend, Real *acc, const Real *yold, Real *out);
 void check_state(omega_model_s *M, omega_state_s *S, const char *what);

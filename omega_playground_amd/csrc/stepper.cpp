@@ -116,22 +116,54 @@ bool general_only(const omega_model_s *M) { return (M->terms & T_DRAG) != 0; }
 // energy and divergence, tracer Laplacian, momentum del4 pre-pass) into the
 // model's buffers; only what the enabled terms read.  Records the arrays as
 // the auxiliary state's source.
-void run_diag(omega_model_s *M, const Real *h, const Real *vn, const Real *tr, hipStream_t s) {
-  M->aux_h = h;
-  M->aux_vn = vn;
-  M->aux_tr = tr;
+bool diag_needed(const omega_model_s *M) {
+  // diagnostics are needed by the vn tendency and by tracer del4 only
+  return !M->params.disable_vn_tendency || ((M->terms & T_TDEL4) && M->params.ntracers > 0);
+}
+
+void run_diag_into(omega_model_s *M, const Real *h, const Real *vn, const Real *tr,
+                   const DiagOut &d, hipStream_t s) {
+  if (!diag_needed(M)) return;
   const MeshView mv = M->view();
   const int ntr = M->params.ntracers;
   const bool do_vn = !M->params.disable_vn_tendency;
-  // diagnostics are needed by the vn tendency and by tracer del4 only
-  if (!(do_vn || ((M->terms & T_TDEL4) && ntr > 0))) return;
-  if (M->terms & T_VDEL4) ensure_vdel4(M);
-  const DiagOut d = M->diag();
   const int dterms = do_vn ? M->terms : (M->terms & T_TDEL4);
   if (hex_usable(mv, M->lc) && !general_only(M))
     launch_diag_hex(mv, M->lc, dterms, ntr, h, vn, tr, d, s);
   else launch_diag(mv, M->lc, M->coef(), dterms, ntr, h, vn, tr, d, s);
   if (do_vn && (M->terms & T_VDEL4)) launch_vdel4(mv, M->lc, d, s);
+}
+
+void run_diag(omega_model_s *M, const Real *h, const Real *vn, const Real *tr, hipStream_t s) {
+  M->aux_h = h;
+  M->aux_vn = vn;
+  M->aux_tr = tr;
+  if (!diag_needed(M)) return;
+  if (M->terms & T_VDEL4) ensure_vdel4(M);
+  run_diag_into(M, h, vn, tr, M->diag(), s);
+}
+
+void run_stage_into(omega_model_s *M, StageArgs a, hipStream_t s, int fields,
+                    const DiagOut *dx) {
+  const MeshView mv = M->view();
+  const ModelCoef mc = M->coef();
+  a.terms = M->terms;
+  a.do_h = !M->params.disable_h_tendency && (fields & 1);
+  a.do_vn = !M->params.disable_vn_tendency && (fields & 2);
+  a.ntr = (fields & 4) ? M->params.ntracers : 0;
+  if (M->linear) {
+    launch_tend_linear(mv, M->lc, M->lin_coef(), a, s);
+    return;
+  }
+  if (dx) {
+    // caller-provided diagnostics buffers: the default kernels only (the
+    // tiled variants build tables on first use)
+    run_diag_into(M, a.h.in, a.vn.in, a.tr.in, *dx, s);
+    if (hex_usable(mv, M->lc) && !general_only(M)) launch_tend_hex(mv, M->lc, mc, a, *dx, s);
+    else launch_tend(mv, M->lc, mc, a, *dx, s);
+    return;
+  }
+  run_stage(M, a, s, fields, true);
 }
 
 // One tendency evaluation (+ optional fused stage update).  with_diag =

@@ -99,6 +99,7 @@ def lib():
         "omega_model_set_terms": ([vp, C.c_int], C.c_int),
         "omega_model_compute_field_tendency": ([vp, C.c_int, vp, vp, C.c_int, vp], C.c_int),
         "omega_model_compute_aux": ([vp, vp, vp, vp, vp], C.c_int),
+        "omega_model_compute_tendency_noalloc": ([vp, vp, vp, C.c_double, C.c_int, vp], C.c_int),
         "omega_model_compute_h_tendency": ([vp, vp, vp, vp, C.c_int, vp], C.c_int),
         "omega_model_compute_vn_tendency": ([vp, vp, vp, vp, C.c_int, vp], C.c_int),
         "omega_model_compute_tr_tendency": ([vp, vp, vp, vp, C.c_int, vp], C.c_int),
@@ -463,6 +464,13 @@ class ShallowWaterModel:
                          add_mode: int = ADD_REPLACE, stream=None):
         check(lib().omega_model_compute_tendency(self._h, tend._h, state._h, float(t), int(add_mode),
                                                  _stream(stream)))
+
+    def compute_tendency_noalloc(self, tend: ShallowWaterState, state: ShallowWaterState,
+                                 t: float = 0.0, add_mode: int = ADD_REPLACE, stream=None):
+        """compute_tendency with its diagnostics in the library's static scratch
+        array (no allocation; 64 MiB per GPU)."""
+        check(lib().omega_model_compute_tendency_noalloc(self._h, tend._h, state._h, float(t),
+                                                         int(add_mode), _stream(stream)))
 
     def _field_tendency(self, field, tend, state, add_mode, stream):
         check(lib().omega_model_compute_field_tendency(self._h, field, tend._h, state._h,
