@@ -135,7 +135,8 @@ def reference_benchmark(nx: int = 512, nlayers: int = 64, ntracers: int = 1, nst
     xc, yc = np.asarray(mesh.x_cell), np.asarray(mesh.y_cell)
     xe, ye, ae = np.asarray(mesh.x_edge), np.asarray(mesh.y_edge), np.asarray(mesh.angle_edge)
     h = np.repeat(dv.h(xc, yc)[:, None], nlayers, 1)
-    vn = np.repeat(wl.normal_velocity(ae, dv.vx(xe, ye), dv.vy(xe, ye))[:, None], nlayers, 1)
+    vn = np.repeat(wl.normal_velocity(ae, dv.vx(xe, ye)[:, None], dv.vy(xe, ye)[:, None]),
+                   nlayers, 1)
     tr = np.full((ntracers, mesh.ncells, nlayers), 1000.0)
     state = swe.ShallowWaterState(mesh, ntracers)
     state.upload(h, vn, tr)
@@ -248,14 +249,17 @@ def adjoint_bench(n_snapshots: int = 10, nsteps: int | None = None, graph: bool 
     }
 
 
-def batched_bench(nmembers: int = 48, nsteps: int = 50):
+def batched_bench(nmembers: int = 48, nsteps: int = 50, specs=None, label=None):
     """The ensemble workload: 48 seeded members (nx in {32..192}, 16/32/64
     layers, 1-4 tracers, double vortex with offset centres and random dH),
     LSRK5, 50 steps, del2 viscosity 1e3 and tracer del4 1e9 as the forward
     workload; one autotuned, graph-captured launch per stage phase for the
     whole ensemble against a sequential per-member loop (each member its own
-    graph-captured stepper) on the same GPU."""
-    specs = wl.ensemble_members(nmembers, seed=0)
+    graph-captured stepper) on the same GPU.  Per-stage bandwidth: the
+    members' algorithmic bytes (18 S per cell-layer-step for LSRK5, S =
+    8 (4 + ntr) B) over the time, as a fraction of the 8 TB/s peak."""
+    specs = specs if specs is not None else wl.ensemble_members(nmembers, seed=0)
+    nmembers = len(specs)
     members = []
     for sp in specs:
         mesh = swe.PlanarHexagonalMesh(sp["nx"], sp["nx"], sp["dc"], sp["nlayers"])
@@ -266,6 +270,8 @@ def batched_bench(nmembers: int = 48, nsteps: int = 50):
                                        dh=sp["dh"], centres=sp["centres"])
         members.append((mesh, model, sp, y0, wl.dt_rule(sp["dc"])))
     work = sum(m[0].ncells * m[2]["nlayers"] for m in members) * nsteps
+    alg = sum(m[0].ncells * m[2]["nlayers"] * alg_bytes_per_cell_layer_step(m[2]["ntracers"], "lsrk")
+              for m in members) * nsteps
 
     def fresh():
         out = []
@@ -322,11 +328,17 @@ def batched_bench(nmembers: int = 48, nsteps: int = 50):
         for sb, sl in zip(states_b, states_l))
     info = batch.info()
     return {
-        "workload": f"{nmembers} members, nx in {{32..192}}, 16/32/64 layers, 1-4 tracers, "
-                    f"LSRK5, {nsteps} steps, double vortex, del2 visc 1e3, tracer del4 1e9",
+        "workload": label or (f"{nmembers} members, nx in {{32..192}}, 16/32/64 layers, 1-4 "
+                              f"tracers, LSRK5, {nsteps} steps, double vortex, del2 visc 1e3, "
+                              f"tracer del4 1e9"),
         "cell_layers": info["cell_layers"],
         "blocks_per_launch": info["blocks_per_launch"],
         "launch": list(batch.get_launch()),
+        "persistent_blocks_per_cu": batch.get_persistent(),
+        "batched_GBs": alg / t_batch / 1e9,
+        "batched_frac": alg / t_batch / 1e9 / HBM_PEAK_GBS,
+        "looped_GBs": alg / t_loop / 1e9,
+        "looped_frac": alg / t_loop / 1e9 / HBM_PEAK_GBS,
         "autotune_ms_per_step": tune_ms,
         "batched_s": t_batch,
         "looped_s": t_loop,
@@ -460,9 +472,9 @@ def main():
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--launch", default="", help="rows,lpt,swizzle,levels_per_block")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--variant", type=int, default=None,
-                    help="0 specialised if possible, 1 general, 2 LDS-tiled")
-    ap.add_argument("--tiling", default="", help="cells_per_tile,levels_per_block")
+                    help="0 specialised if possible, 1 general")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -481,14 +493,16 @@ def main():
         model.set_launch(*(int(x) for x in args.launch.split(",")))
     if args.variant is not None:
         model.set_variant(args.variant)
-    if args.tiling:
-        model.set_tiling(*(int(x) for x in args.tiling.split(",")))
     ntr, L, nc = cfg["ntracers"], cfg["nlayers"], mesh.ncells
     dt = wl.dt_rule(cfg["dc"], cfg["grav"])
     state = swe.ShallowWaterState(mesh, ntr)
     state.upload(*state0)
     stepper = swe.RK4Stepper(model) if cfg["stepper"] == "rk4" else swe.LSRKStepper(model, 5)
     stepper.set_graph(not args.no_graph)
+    tune_ms = None
+    if not args.launch and not args.no_tune:
+        tune_ms = model.autotune(state, reps=3)
+        log(f"[bench] autotuned launch {model.get_launch()} ({tune_ms:.3f} ms per tendency)")
     log(f"[bench] {args.config}: nx={cfg['nx']} cells={nc} L={L} ntr={ntr} dt={dt:.4f}s "
         f"launch={model.get_launch()} kernels={model.kernel_variant()} graph={not args.no_graph}")
 
@@ -546,7 +560,16 @@ def main():
         log("[bench] ensemble workload")
         bat = batched_bench()
         log(f"[bench] ensemble: batched {bat['batched_s']:.3f} s vs looped {bat['looped_s']:.3f} s "
-            f"= {bat['speedup']:.2f}x, bit-identical {bat['bit_identical']}, launch {bat['launch']}")
+            f"= {bat['speedup']:.2f}x, bit-identical {bat['bit_identical']}, launch {bat['launch']}, "
+            f"{bat['batched_frac']:.3f} vs {bat['looped_frac']:.3f} of peak")
+        small = [dict(nx=32, dc=10e3, nlayers=16, ntracers=2,
+                      centres=((0.4, 0.5), (0.6, 0.5)), dh=75.0, seed=i) for i in range(48)]
+        bat["small_ensemble"] = batched_bench(
+            specs=small, label="48 members, nx=32 (1024 cells), 16 layers, 2 tracers, LSRK5, "
+                               "50 steps, double vortex, del2 visc 1e3, tracer del4 1e9")
+        sm = bat["small_ensemble"]
+        log(f"[bench] small ensemble: batched {sm['batched_s']:.4f} s vs looped "
+            f"{sm['looped_s']:.4f} s = {sm['speedup']:.2f}x (eager loop {sm['speedup_vs_eager_loop']:.2f}x)")
 
     eadj = None
     if not args.skip_ensemble_adjoint:
@@ -601,7 +624,8 @@ def main():
                                f"vortex, del2 visc {cfg['visc_del2']:g}, tracer del4 "
                                f"{cfg['eddy_diff4']:g}, hipGraph step",
                    "ncells": nc, "nlayers": L, "ntracers": ntr, "stepper": cfg["stepper"],
-                   "graph": not args.no_graph, "parallelism": "single GPU"},
+                   "graph": not args.no_graph, "parallelism": "single GPU",
+                   "launch": list(model.get_launch()), "autotune_ms_per_tendency": tune_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": stage_traffic,

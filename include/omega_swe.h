@@ -313,20 +313,23 @@ int omega_model_set_launch(omega_model_t model, int rows_per_block, int levels_p
                            int xcd_swizzle, int levels_per_block);
 int omega_model_get_launch(omega_model_t model, int *rows_per_block, int *levels_per_thread,
                            int *xcd_swizzle, int *levels_per_block);
+/* Times reps tendency evaluations of state for each launch shape (cells per
+ * block: the default fill, 4 or 2; levels per thread 2 or 1; XCD swizzle on
+ * or off), keeps the fastest; *best_ms = its time per evaluation.  With the
+ * environment variable OMEGA_TUNE_CACHE naming a file, a choice recorded
+ * there for the same model shape (cells, layers, tracers, terms, kernel
+ * family) is applied without timing, and a new choice is appended.  No
+ * reference counterpart (Kokkos picks its tiles, src/common.hpp:78-104);
+ * results are the same bits for every shape. */
+int omega_model_autotune(omega_model_t model, omega_state_t state, int reps, double *best_ms,
+                         omega_stream_t stream);
 
 /* Kernel family: 0 (default) uses the pattern-specialised kernels when the
  * mesh is a regular hexagonal mesh verified against the built-in stencil
- * pattern and nlayers is a multiple of 64, 1 forces the general kernels,
- * 2 selects the LDS-tiled tendency kernels (any mesh; every row a tile's
- * stencils read is staged once in LDS), 3 the union-stencil specialised
- * kernel (one wavefront per cell, rows shared by its three edges loaded
- * once), 4 the hexagonal LDS-tiled edge kernel (velocity, thickness and
- * vertex PV rows of a tile staged in LDS, stencil slots from the built-in
- * pattern) followed by the specialised cell kernel; 3 and 4 fall back to
- * the specialised kernels where they do not apply.
- * omega_model_kernel_variant reports what runs: 6 linear model,
- * 5 hexagonal tiled, 4 union-stencil, 3 tiled, 2 specialised, 1 general.  No reference
- * counterpart (performance control only; all give the same bits). */
+ * pattern and nlayers is a multiple of 64 (or 16 / 32 with two levels per
+ * thread), 1 forces the general kernels.  omega_model_kernel_variant
+ * reports what runs: 6 linear model, 2 specialised, 1 general.  No reference
+ * counterpart (performance control only; both give the same bits). */
 int omega_model_set_variant(omega_model_t model, int variant);
 int omega_model_kernel_variant(omega_model_t model, int *in_use);
 /* Forms of the two-cells-per-wavefront kernels this model runs: bit 0 set
@@ -340,18 +343,6 @@ int omega_model_kernel_flags(omega_model_t model, int *flags);
 /* Which of those forms the model may use (bits as above, default 3);
  * clearing a bit forces the per-cell or 64-bit form (tests, tuning). */
 int omega_model_set_kernel_forms(omega_model_t model, int forms);
-/* Tiled kernels: cells per tile (1..256, default 32) and levels per block
- * (8, 16 or 32, default 8; the general tiled kernels use 8 or 16 and take
- * 32 as 16, the hexagonal tiled kernel uses 16 or 32 and takes 8 as 16);
- * the tile's staged rows must fit the LDS a block may allocate (160 KiB on
- * gfx950).  The hexagonal tiled kernel needs at most 256 union edges,
- * cells and vertices per tile (one-byte stencil records). */
-int omega_model_set_tiling(omega_model_t model, int cells_per_tile, int levels_per_block);
-/* Set sizes of the tiling of a mesh (host only): stats[0..6] = tiles, and
- * the largest per-tile counts of union edges, union cells, cells of owned
- * edges, union vertices, edges of owned cells, owned + neighbour cells. */
-int omega_mesh_tile_stats(omega_mesh_t mesh, int cells_per_tile, int *stats);
-
 /* ----------------------------------------------------------------------
  * Time steppers (src/time_integration/time_stepper.hpp:8-38)
  * -------------------------------------------------------------------- */
@@ -475,11 +466,26 @@ int omega_batch_run(omega_batch_t batch, int nsteps, omega_stream_t stream);
 /* Capture the ensemble step in a HIP graph (one per buffer parity). */
 int omega_batch_set_graph(omega_batch_t batch, int enable);
 /* Launch shape of the ensemble kernels: threads per block (64, 128, 256)
- * and levels per lane (1 or 2). */
+ * and levels per lane (1 or 2).  Each stage phase (diagnostics, momentum
+ * del4 pre-pass, tendency + update) is one launch over every member; members
+ * on the regular hexagonal mesh run the pattern-specialised operators and
+ * the others the general ones inside it. */
 int omega_batch_set_launch(omega_batch_t batch, int threads, int levels_per_thread);
 int omega_batch_get_launch(omega_batch_t batch, int *threads, int *levels_per_thread);
-/* Times reps steps per admissible launch shape, keeps the fastest and
- * restores the members' states; *best_ms = that shape's time per step. */
+/* Persistent grid: 0 (default) launches one block per table entry (a
+ * block covers the cells-per-block of one member); n > 0 launches n blocks
+ * per CU, each walking a contiguous run of entries (used when that is fewer
+ * blocks than entries). */
+int omega_batch_set_persistent(omega_batch_t batch, int blocks_per_cu);
+int omega_batch_get_persistent(omega_batch_t batch, int *blocks_per_cu);
+/* Times reps steps per admissible launch shape (levels per lane 2/1,
+ * threads per block 256/128/64, which set every member's cells per block,
+ * and a grid of one block per entry or 4/8/16 persistent blocks per CU),
+ * keeps the fastest and restores the members' states; *best_ms = that
+ * shape's time per step.  When the environment variable OMEGA_TUNE_CACHE
+ * names a file, a choice recorded there for the same ensemble (stepper and
+ * each member's cells, layers, tracers, terms and kernel kind) is applied
+ * without timing, and a new choice is appended to the file. */
 int omega_batch_autotune(omega_batch_t batch, int reps, double *best_ms, omega_stream_t stream);
 int omega_batch_info(omega_batch_t batch, int *nmembers, int64_t *blocks_per_launch,
                      int64_t *cell_layers);

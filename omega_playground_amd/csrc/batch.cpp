@@ -7,7 +7,10 @@
 // epilogues of the single-model steppers (rk4.cpp:28-104, lsrk.cpp:36-89),
 // so each member advances bit for bit as it would alone.
 #include <algorithm>
+#include <cstdlib>
+#include <fstream>
 #include <map>
+#include <sstream>
 
 #include "batch.hpp"
 #include "objects.hpp"
@@ -30,12 +33,13 @@ struct omega_batch_s {
   std::vector<std::unique_ptr<Member>> mem;
   int N = 2;         // levels per lane
   int threads = 256; // threads per block
+  int per_cu = 0;    // persistent grid: blocks per CU (0: one block per entry)
   bool built = false;
   int parity = 0;
-  // two launch groups: 0 general kernels, 1 pattern-specialised kernels
-  int nblocks[2] = {0, 0};
+  // one table over all members; member kinds share each launch
+  int nblocks = 0;
   bool any_vdel4 = false;
-  DBuf<int> blk[2];
+  DBuf<int> blk;
   DBuf<BatchMember> desc; // [parity][stage][member]
   bool use_graph = false;
   std::map<int, hipGraphExec_t> graphs;
@@ -45,6 +49,18 @@ struct omega_batch_s {
 };
 
 namespace {
+
+// Blocks of the persistent grid (0: one block per table entry).
+int persistent_blocks(const omega_batch_s *B) {
+  if (B->per_cu <= 0) return 0;
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    OMEGA_HIP_CHECK(hipGetDevice(&dev));
+    OMEGA_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  }
+  return B->per_cu * cus;
+}
 
 bool odd_flip(const omega_batch_s *B) {
   return B->kind == OMEGA_STEPPER_LSRK && (B->nstages % 2 == 1);
@@ -59,7 +75,7 @@ Real *state_buf(omega_state_s *S, int c, int f) {
 void build_tables(omega_batch_s *B) {
   OMEGA_REQUIRE(!B->mem.empty(), "batch: no members");
   const int nm = (int)B->mem.size(), ns = B->nstages, N = B->N, T = B->threads;
-  std::vector<int> blk[2];
+  std::vector<int> blk;
   std::vector<BatchMember> geo(nm);
   B->any_vdel4 = false;
   for (int i = 0; i < nm; ++i) {
@@ -87,10 +103,12 @@ void build_tables(omega_batch_s *B) {
                 ? 1
                 : 0;
     if (g.hex) {
-      // one row of lanes covers 64 levels (or all 16 / 32)
+      // one row of lanes covers 64 levels (or all 16 / 32); the pair form
+      // (64-level rows, N = 2) needs an even row count, which every
+      // admissible block size gives
       const int span = hm.nlayers % 64 == 0 ? 64 : hm.nlayers;
       g.lx = span / N;
-      g.rows = 256 / g.lx;
+      g.rows = std::max(1, T / g.lx);
       g.chunks = hm.nlayers / span;
     } else {
       int lanes = (hm.nlayers + N - 1) / N;
@@ -101,10 +119,9 @@ void build_tables(omega_batch_s *B) {
       g.rows = std::max(1, T / lx);
       g.chunks = (hm.nlayers + lx * N - 1) / (lx * N);
     }
-    std::vector<int> &bl = blk[g.hex];
-    g.block0 = (int)bl.size();
+    g.block0 = (int)blk.size();
     const int nb = ((hm.ncells + g.rows - 1) / g.rows) * g.chunks;
-    bl.insert(bl.end(), nb, i);
+    blk.insert(blk.end(), nb, i);
     const bool do_vn = !M->params.disable_vn_tendency;
     const int ntr = M->params.ntracers;
     const bool need_diag = do_vn || ((M->terms & T_TDEL4) && ntr > 0);
@@ -114,7 +131,7 @@ void build_tables(omega_batch_s *B) {
     mb.cur0 = S->cur;
     mb.gen = M->gen;
   }
-  for (int q = 0; q < 2; ++q) B->nblocks[q] = (int)blk[q].size();
+  B->nblocks = (int)blk.size();
   std::vector<BatchMember> tab((size_t)2 * ns * nm);
   for (int p = 0; p < 2; ++p)
     for (int s = 0; s < ns; ++s)
@@ -172,7 +189,7 @@ void build_tables(omega_batch_s *B) {
         g.a = a;
         tab[((size_t)p * ns + s) * nm + i] = g;
       }
-  for (int q = 0; q < 2; ++q) B->blk[q].upload(blk[q]);
+  B->blk.upload(blk);
   if (B->desc.n != tab.size()) B->desc.alloc(tab.size());
   OMEGA_HIP_CHECK(hipMemcpy(B->desc.p, tab.data(), tab.size() * sizeof(BatchMember),
                             hipMemcpyHostToDevice));
@@ -208,18 +225,14 @@ void step_launches(omega_batch_s *B, int parity, hipStream_t s) {
       carry_extra_tracers(mb->M, S, s);
       S->cur = keep;
     }
+  const int pers = persistent_blocks(B);
   for (int st = 0; st < B->nstages; ++st) {
     const BatchMember *d = B->desc.p + ((size_t)parity * B->nstages + st) * nm;
-    const int *g0 = B->blk[0].p, *g1 = B->blk[1].p;
-    const int n0 = B->nblocks[0], n1 = B->nblocks[1];
-    if (n0) launch_batch_diag(d, g0, n0, B->threads, B->N, s);
-    if (n1) launch_batch_diag_hex(d, g1, n1, B->N, s);
-    if (B->any_vdel4) {
-      if (n0) launch_batch_vdel4(d, g0, n0, B->threads, B->N, s);
-      if (n1) launch_batch_vdel4(d, g1, n1, 256, B->N, s);
-    }
-    if (n0) launch_batch_tend(d, g0, n0, B->threads, B->N, s);
-    if (n1) launch_batch_tend_hex(d, g1, n1, B->N, s);
+    const int *g = B->blk.p;
+    const int n = B->nblocks;
+    launch_batch_diag(d, g, n, B->threads, B->N, pers, s);
+    if (B->any_vdel4) launch_batch_vdel4(d, g, n, B->threads, B->N, pers, s);
+    launch_batch_tend(d, g, n, B->threads, B->N, pers, s);
   }
 }
 
@@ -356,14 +369,51 @@ int omega_batch_get_launch(omega_batch_t B, int *threads, int *levels_per_thread
   });
 }
 
-// Times `reps` steps for every launch shape the ensemble admits and keeps
-// the fastest.  The members' states are restored afterwards, so the
-// ensemble continues from where it was.
+// Times `reps` steps for every launch shape the ensemble admits -- levels
+// per lane (2, 1), threads per block (256, 128, 64: the cells per block of
+// every member) and the grid (one block per table entry, or a persistent
+// grid of 4, 8 or 16 blocks per CU) -- and keeps the fastest.  The members'
+// states are restored afterwards, so the ensemble continues from where it
+// was.  With OMEGA_TUNE_CACHE naming a file, a choice recorded there for
+// the same ensemble signature (stepper, and per member: cells, layers,
+// tracers, terms, kernel kind) is applied without timing, and a new choice
+// is appended to it.
 int omega_batch_autotune(omega_batch_t B, int reps, double *best_ms, omega_stream_t stream) {
   return guarded([&] {
     OMEGA_REQUIRE(B && !B->mem.empty(), "batch: no members");
     hipStream_t s = as_stream(stream);
     reps = std::max(1, reps);
+    bool even_layers = true;
+    for (auto &mb : B->mem) even_layers = even_layers && mb->M->mesh->dm.host.nlayers % 2 == 0;
+    // the ensemble's signature for the tuning cache
+    std::string sig = std::to_string(B->kind) + "/" + std::to_string(B->nstages);
+    for (auto &mb : B->mem) {
+      const HostMesh &hm = mb->M->mesh->dm.host;
+      const bool hx = hex_usable(mb->M->mesh->dm.view(), mb->M->lc) && !general_only(mb->M);
+      sig += ";" + std::to_string(hm.ncells) + ":" + std::to_string(hm.nlayers) + ":" +
+             std::to_string(mb->M->params.ntracers) + ":" + std::to_string(mb->M->terms) + ":" +
+             (hx ? "h" : "g");
+    }
+    const char *cache = std::getenv("OMEGA_TUNE_CACHE");
+    if (cache && *cache) {
+      std::ifstream in(cache);
+      std::string line;
+      while (std::getline(in, line)) {
+        std::istringstream ls(line);
+        std::string key;
+        int n = 0, t = 0, pc = -1;
+        double ms = -1;
+        if (!(ls >> key >> n >> t >> pc >> ms) || key != sig) continue;
+        if ((n == 1 || (n == 2 && even_layers)) && (t == 64 || t == 128 || t == 256) && pc >= 0) {
+          B->N = n;
+          B->threads = t;
+          B->per_cu = pc;
+          B->built = false;
+          if (best_ms) *best_ms = ms;
+          return;
+        }
+      }
+    }
     // save every member's state
     std::vector<std::array<DBuf<Real>, 3>> saved(B->mem.size());
     std::vector<int> cur(B->mem.size());
@@ -377,27 +427,28 @@ int omega_batch_autotune(omega_batch_t B, int reps, double *best_ms, omega_strea
                                          hipMemcpyDeviceToDevice, s));
       }
     }
-    bool even_layers = true;
-    for (auto &mb : B->mem) even_layers = even_layers && mb->M->mesh->dm.host.nlayers % 2 == 0;
-    const int keepN = B->N, keepT = B->threads;
     double best = -1;
-    int bestN = keepN, bestT = keepT;
+    int bestN = B->N, bestT = B->threads, bestP = B->per_cu;
     for (int N : {2, 1}) {
       if (N == 2 && !even_layers) continue;
-      for (int T : {256, 128, 64}) {
-        B->N = N;
-        B->threads = T;
-        B->built = false;
-        const double ms = time_steps(B, reps, s);
-        if (best < 0 || ms < best) {
-          best = ms;
-          bestN = N;
-          bestT = T;
+      for (int T : {256, 128, 64})
+        for (int P : {0, 4, 8, 16}) {
+          B->N = N;
+          B->threads = T;
+          B->per_cu = P;
+          B->built = false;
+          const double ms = time_steps(B, reps, s);
+          if (best < 0 || ms < best) {
+            best = ms;
+            bestN = N;
+            bestT = T;
+            bestP = P;
+          }
         }
-      }
     }
     B->N = bestN;
     B->threads = bestT;
+    B->per_cu = bestP;
     B->built = false;
     // restore the states
     for (size_t i = 0; i < B->mem.size(); ++i) {
@@ -409,7 +460,26 @@ int omega_batch_autotune(omega_batch_t B, int reps, double *best_ms, omega_strea
                                          hipMemcpyDeviceToDevice, s));
     }
     OMEGA_HIP_CHECK(hipStreamSynchronize(s));
+    if (cache && *cache) {
+      std::ofstream out(cache, std::ios::app);
+      if (out) out << sig << " " << bestN << " " << bestT << " " << bestP << " " << best << "\n";
+    }
     if (best_ms) *best_ms = best;
+  });
+}
+
+int omega_batch_set_persistent(omega_batch_t B, int blocks_per_cu) {
+  return guarded([&] {
+    OMEGA_REQUIRE(B, "null batch");
+    OMEGA_REQUIRE(blocks_per_cu >= 0 && blocks_per_cu <= 64, "blocks_per_cu must be in 0..64");
+    B->per_cu = blocks_per_cu;
+  });
+}
+
+int omega_batch_get_persistent(omega_batch_t B, int *blocks_per_cu) {
+  return guarded([&] {
+    OMEGA_REQUIRE(B && blocks_per_cu, "null argument");
+    *blocks_per_cu = B->per_cu;
   });
 }
 
@@ -419,7 +489,7 @@ int omega_batch_info(omega_batch_t B, int *nmembers, int64_t *blocks_per_launch,
     OMEGA_REQUIRE(B, "null batch");
     if (!B->built && !B->mem.empty()) build_tables(B);
     if (nmembers) *nmembers = (int)B->mem.size();
-    if (blocks_per_launch) *blocks_per_launch = B->nblocks[0] + B->nblocks[1];
+    if (blocks_per_launch) *blocks_per_launch = B->nblocks;
     if (cell_layers) {
       int64_t n = 0;
       for (auto &mb : B->mem)

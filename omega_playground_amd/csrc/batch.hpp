@@ -22,19 +22,15 @@ struct BatchMember {
   int hex;      // member runs the pattern-specialised kernels
 };
 
-// One launch each over all members' blocks; blk[b] = member of block b.
+// One launch per stage phase over every member's table entries; blk[b] =
+// member of entry b.  persistent > 0 launches that many blocks (when fewer
+// than the entries), each walking a contiguous run of entries; 0 launches
+// one block per entry.
 void launch_batch_diag(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
-                       hipStream_t s);
+                       int persistent, hipStream_t s);
 void launch_batch_vdel4(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
-                        hipStream_t s);
+                        int persistent, hipStream_t s);
 void launch_batch_tend(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
-                       hipStream_t s);
-// The same phases for members on the regular hexagonal mesh with a layer
-// count that is a multiple of 64 (pattern-specialised operators; 256
-// threads, 64/N lanes per row).
-void launch_batch_diag_hex(const BatchMember *desc, const int *blk, int nblocks, int N,
-                           hipStream_t s);
-void launch_batch_tend_hex(const BatchMember *desc, const int *blk, int nblocks, int N,
-                           hipStream_t s);
+                       int persistent, hipStream_t s);
 
 } // namespace omega_hip

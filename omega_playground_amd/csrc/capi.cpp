@@ -2,7 +2,10 @@
 // diagnostics and timing.  Stage orchestration lives in stepper.cpp.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <fstream>
 #include <mutex>
+#include <sstream>
 
 #include "objects.hpp"
 
@@ -688,34 +691,96 @@ int omega_model_set_launch(omega_model_t M, int rows, int lpt, int swizzle, int 
   });
 }
 
+// Times reps tendency evaluations of `state` (diagnostics + tendency
+// launches) for each launch shape -- cells per block (the default fill, 4,
+// 2), levels per thread (2 when the layer count is even, 1) and the XCD
+// swizzle (on, off) -- and keeps the fastest.  OMEGA_TUNE_CACHE works as for
+// omega_batch_autotune, keyed by the model's cells, layers, tracers, terms
+// and kernel family.
+int omega_model_autotune(omega_model_t M, omega_state_t state, int reps, double *best_ms,
+                         omega_stream_t stream) {
+  return guarded([&] {
+    OMEGA_REQUIRE(M, "null model");
+    check_state(M, state, "autotune");
+    reps = std::max(1, reps);
+    hipStream_t s = as_stream(stream);
+    const HostMesh &hm = M->mesh->dm.host;
+    const MeshView mv = M->view();
+    const std::string sig = "model;" + std::to_string(hm.ncells) + ":" +
+                            std::to_string(hm.nlayers) + ":" +
+                            std::to_string(M->params.ntracers) + ":" + std::to_string(M->terms) +
+                            ":" + (M->linear ? "l" : hex_usable(mv, M->lc) ? "h" : "g");
+    const char *cache = std::getenv("OMEGA_TUNE_CACHE");
+    if (cache && *cache) {
+      std::ifstream in(cache);
+      std::string line;
+      while (std::getline(in, line)) {
+        std::istringstream ls(line);
+        std::string key;
+        int r = 0, l = 0, w = 0;
+        double ms = -1;
+        if (!(ls >> key >> r >> l >> w >> ms) || key != sig) continue;
+        if (r >= 0 && r <= 256 && (l == 1 || (l == 2 && hm.nlayers % 2 == 0))) {
+          M->lc.rows = r;
+          M->lc.lpt = l;
+          M->lc.swizzle = w ? 1 : 0;
+          M->touch();
+          if (best_ms) *best_ms = ms;
+          return;
+        }
+      }
+    }
+    DBuf<Real> t0(state->n[0]), t1(state->n[1]), t2(state->n[2]);
+    hipEvent_t a, b;
+    OMEGA_HIP_CHECK(hipEventCreate(&a));
+    OMEGA_HIP_CHECK(hipEventCreate(&b));
+    const Launch keep = M->lc;
+    double best = -1;
+    Launch pick = keep;
+    for (int lpt : {2, 1}) {
+      if (lpt == 2 && hm.nlayers % 2 != 0) continue;
+      for (int rows : {0, 4, 2})
+        for (int swz : {1, 0}) {
+          M->lc = keep;
+          M->lc.rows = rows;
+          M->lc.lpt = lpt;
+          M->lc.swizzle = swz;
+          compute_tendency_ptrs(M, t0.p, t1.p, t2.p, state->ptr(0), state->ptr(1),
+                                state->ptr(2), 0.0, OMEGA_ADD_REPLACE, s, 7);
+          OMEGA_HIP_CHECK(hipEventRecord(a, s));
+          for (int i = 0; i < reps; ++i)
+            compute_tendency_ptrs(M, t0.p, t1.p, t2.p, state->ptr(0), state->ptr(1),
+                                  state->ptr(2), 0.0, OMEGA_ADD_REPLACE, s, 7);
+          OMEGA_HIP_CHECK(hipEventRecord(b, s));
+          OMEGA_HIP_CHECK(hipEventSynchronize(b));
+          float ms = 0;
+          OMEGA_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+          const double per = ms / reps;
+          if (best < 0 || per < best) {
+            best = per;
+            pick = M->lc;
+          }
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    M->lc = pick;
+    M->touch();
+    if (cache && *cache) {
+      std::ofstream out(cache, std::ios::app);
+      if (out)
+        out << sig << " " << pick.rows << " " << pick.lpt << " " << pick.swizzle << " " << best
+            << "\n";
+    }
+    if (best_ms) *best_ms = best;
+  });
+}
+
 int omega_model_set_variant(omega_model_t M, int variant) {
   return guarded([&] {
     OMEGA_REQUIRE(M, "null model");
-    OMEGA_REQUIRE(variant >= 0 && variant <= 4, "variant must be 0, 1, 2, 3 or 4");
+    OMEGA_REQUIRE(variant == 0 || variant == 1, "variant must be 0 or 1");
     M->lc.variant = variant;
-    M->touch();
-  });
-}
-
-int omega_mesh_tile_stats(omega_mesh_t mesh, int cells_per_tile, int *stats) {
-  return guarded([&] {
-    OMEGA_REQUIRE(mesh && stats, "null argument");
-    omega_hip::TileTables t;
-    build_tile_tables(mesh->dm, cells_per_tile, t, false);
-    const int v[7] = {t.ntiles, t.max_ue, t.max_uc, t.max_nkc, t.max_uv, t.max_ce, t.max_cc};
-    for (int i = 0; i < 7; ++i) stats[i] = v[i];
-  });
-}
-
-int omega_model_set_tiling(omega_model_t M, int cells_per_tile, int levels_per_block) {
-  return guarded([&] {
-    OMEGA_REQUIRE(M, "null model");
-    OMEGA_REQUIRE(cells_per_tile >= 1 && cells_per_tile <= 256,
-                  "cells_per_tile must be in 1..256");
-    OMEGA_REQUIRE(levels_per_block == 8 || levels_per_block == 16 || levels_per_block == 32,
-                  "levels_per_block must be 8, 16 or 32");
-    M->lc.tile = cells_per_tile;
-    M->lc.tile_k = levels_per_block;
     M->touch();
   });
 }
@@ -724,14 +789,7 @@ int omega_model_kernel_variant(omega_model_t M, int *in_use) {
   return guarded([&] {
     OMEGA_REQUIRE(M && in_use, "null argument");
     const MeshView mv = M->view();
-    ensure_tiles(M);
-    *in_use = M->linear                                   ? 6
-              : general_only(M)                           ? 1
-              : M->lc.variant == 2                        ? 3
-              : hextile_usable(mv, M->mesh->tiles, M->lc) ? 5
-              : hexu_usable(mv, M->lc)                    ? 4
-              : hex_usable(mv, M->lc)  ? 2
-                                       : 1;
+    *in_use = M->linear ? 6 : general_only(M) ? 1 : hex_usable(mv, M->lc) ? 2 : 1;
   });
 }
 

@@ -173,10 +173,7 @@ struct Launch {
   int lpt = 1;       // levels per thread (1 or 2)
   int swizzle = 1;   // XCD-contiguous block order
   int chunk = 0;     // levels per block (0: all levels, up to 64 lanes)
-  int variant = 0;   // 0: pattern-specialised kernels when the mesh allows, 1: general,
-                     // 2: LDS-tiled tendency kernels, 3: union-stencil specialised kernel
-  int tile = 32;     // tiled kernels: cells per tile
-  int tile_k = 8;    // tiled kernels: levels per block (8 or 16)
+  int variant = 0;   // 0: pattern-specialised kernels when the mesh allows, 1: general
 };
 
 // Enabled-term bits (same meaning as omega_model_set_terms).
@@ -264,10 +261,6 @@ void launch_diag_hex(const MeshView &m, const Launch &lc, int terms, int ntr, co
                      const Real *vn, const Real *tr, const DiagOut &d, hipStream_t s);
 void launch_tend_hex(const MeshView &m, const Launch &lc, const ModelCoef &mc, const StageArgs &a,
                      const DiagOut &d, hipStream_t s);
-// Union-stencil variant (one wavefront per cell, shared rows loaded once)
-bool hexu_usable(const MeshView &m, const Launch &lc);
-void launch_tend_hexu(const MeshView &m, const Launch &lc, const ModelCoef &mc,
-                      const StageArgs &a, const DiagOut &d, hipStream_t s);
 // y += alpha*x elementwise
 void launch_axpy(Real *y, Real alpha, const Real *x, size_t n, hipStream_t s);
 // y = x + alpha*z elementwise

@@ -1,10 +1,15 @@
-// Ensemble kernels: the forward operators of fwd_ops.hpp applied to every
-// member of an ensemble in one launch.  Block b belongs to member blk[b]
-// (wavefront-uniform, so the member's descriptor comes through scalar
-// loads); inside the member the block covers `rows` cells of its
-// space-filling-curve order and `lx` lanes x N levels of one level chunk,
-// exactly as the single-model kernels do, so each member's results are the
-// same bits as stepping it alone.
+// Ensemble kernels: the forward operators of fwd_ops.hpp and hex_ops.hpp
+// applied to every member of an ensemble in one launch per stage phase.
+// Table entry b belongs to member blk[b] (wavefront-uniform, so the
+// member's descriptor comes through scalar loads); inside the member the
+// entry covers `rows` cells of its space-filling-curve order and `lx`
+// lanes x N levels of one level chunk, exactly as the single-model kernels
+// do, so each member's results are the same bits as stepping it alone.
+// Members on the regular hexagonal mesh run the pattern-specialised
+// operators, the others the general ones, in the same launch.  With a
+// persistent grid each block walks a contiguous run of table entries.
+#include <algorithm>
+
 #include "batch.hpp"
 #include "fwd_ops.hpp"
 #include "hex_ops.hpp"
@@ -34,13 +39,26 @@ __device__ __forceinline__ Place place(const BatchMember &B, int lb, int N) {
   return p;
 }
 
+// Table entries a block handles: one entry per block, or (persistent grid,
+// gridDim.x below the entry count) a contiguous run of entries per block,
+// the runs dealt so that each XCD walks a contiguous part of the table.
+template <class F>
+__device__ __forceinline__ void for_entries(int nblocks, int persistent, F &&f) {
+  if (!persistent) {
+    f((int)blockIdx.x);
+    return;
+  }
+  const int G = gridDim.x;
+  const int lb = logical_block(blockIdx.x, G, 1);
+  const int b0 = (int)((long long)lb * nblocks / G);
+  const int b1 = (int)((long long)(lb + 1) * nblocks / G);
+  for (int b = b0; b < b1; ++b) f(b);
+}
+
+// general-kernel phases of one table entry (member B, block lb of it)
 template <int N>
-__global__ void __launch_bounds__(256)
-    k_batch_diag(const BatchMember *__restrict__ desc, const int *__restrict__ blk) {
-  const int id = uniform(blk[blockIdx.x]);
-  const BatchMember &B = desc[id];
-  if (!B.dterms) return;
-  const Place p = place(B, (int)blockIdx.x - B.block0, N);
+__device__ __forceinline__ void gen_diag(const BatchMember &B, int lb) {
+  const Place p = place(B, lb, N);
   if (!p.active) return;
   const MeshView &m = B.m;
   const DiagOut &d = B.d;
@@ -54,40 +72,8 @@ __global__ void __launch_bounds__(256)
 }
 
 template <int N>
-__global__ void __launch_bounds__(256)
-    k_batch_vdel4_edge(const BatchMember *__restrict__ desc, const int *__restrict__ blk) {
-  const int id = uniform(blk[blockIdx.x]);
-  const BatchMember &B = desc[id];
-  if (!B.vdel4) return;
-  const Place p = place(B, (int)blockIdx.x - B.block0, N);
-  if (!p.active) return;
-  const MeshView &m = B.m;
-  const DiagOut &d = B.d;
-  const int eb = m.eofs[p.c0], ee = m.eofs[p.c1];
-  for (int e = eb + p.ty; e < ee; e += p.R) vdel4_edge<N>(m, d, e, p.k);
-}
-
-template <int N>
-__global__ void __launch_bounds__(256)
-    k_batch_vdel4_vc(const BatchMember *__restrict__ desc, const int *__restrict__ blk) {
-  const int id = uniform(blk[blockIdx.x]);
-  const BatchMember &B = desc[id];
-  if (!B.vdel4) return;
-  const Place p = place(B, (int)blockIdx.x - B.block0, N);
-  if (!p.active) return;
-  const MeshView &m = B.m;
-  const DiagOut &d = B.d;
-  const int vb = m.vofs[p.c0], ve = m.vofs[p.c1];
-  for (int v = vb + p.ty; v < ve; v += p.R) vdel4_vertex<N>(m, d, v, p.k);
-  for (int c = p.c0 + p.ty; c < p.c1; c += p.R) vdel4_cell<N>(m, d, c, p.k);
-}
-
-template <int N>
-__global__ void __launch_bounds__(256)
-    k_batch_tend(const BatchMember *__restrict__ desc, const int *__restrict__ blk) {
-  const int id = uniform(blk[blockIdx.x]);
-  const BatchMember &B = desc[id];
-  const Place p = place(B, (int)blockIdx.x - B.block0, N);
+__device__ __forceinline__ void gen_tend(const BatchMember &B, int lb) {
+  const Place p = place(B, lb, N);
   if (!p.active) return;
   // the descriptor is read in place (global memory, wavefront-uniform
   // address, scalar loads); copying it out would put it on the stack
@@ -100,50 +86,73 @@ __global__ void __launch_bounds__(256)
   for (int c = p.c0 + p.ty; c < p.c1; c += p.R) cell_tend<N>(m, mc, a, d, c, p.k);
 }
 
-// Pattern-specialised phases: one cell (its 3 owned edges and 2 owned
-// vertices) per row of 64/N lanes.
 template <int N>
 __global__ void __launch_bounds__(256)
-    k_batch_diag_hex(const BatchMember *__restrict__ desc, const int *__restrict__ blk) {
-  const int id = uniform(blk[blockIdx.x]);
-  const BatchMember &B = desc[id];
-  if (!B.dterms) return;
-  const int lb = (int)blockIdx.x - B.block0;
-  const int chunk = lb % B.chunks, seg = lb / B.chunks;
-  const int c = hexops::ent<N>(seg * B.rows + (int)threadIdx.x / B.lx);
-  if (c >= B.m.nc) return;
-  const int k = (chunk * B.lx + (int)threadIdx.x % B.lx) * N;
-  const MeshView &m = B.m;
-  const DiagOut &d = B.d;
-  const double *h = B.a.h.in, *vn = B.a.vn.in, *tr = B.a.tr.in;
-  if (B.dterms & (T_PV | T_VDEL2 | T_VDEL4)) {
-    hexops::vertex_diag_u<N>(m, 2 * c, k, h, vn, d);
-    hexops::vertex_diag_u<N>(m, 2 * c + 1, k, h, vn, d);
-  }
-  hexops::cell_diag_u<N>(m, B.dterms, B.a.ntr, c, k, h, vn, tr, d);
+    k_batch_vdel4_edge(const BatchMember *__restrict__ desc, const int *__restrict__ blk,
+                       int nblocks, int persistent) {
+  for_entries(nblocks, persistent, [&](int b) {
+    const int id = uniform(blk[b]);
+    const BatchMember &B = desc[id];
+    if (!B.vdel4) return;
+    const Place p = place(B, b - B.block0, N);
+    if (!p.active) return;
+    const MeshView &m = B.m;
+    const DiagOut &d = B.d;
+    const int eb = m.eofs[p.c0], ee = m.eofs[p.c1];
+    for (int e = eb + p.ty; e < ee; e += p.R) vdel4_edge<N>(m, d, e, p.k);
+  });
 }
 
 template <int N>
 __global__ void __launch_bounds__(256)
-    k_batch_tend_hex(const BatchMember *__restrict__ desc, const int *__restrict__ blk) {
-  const int id = uniform(blk[blockIdx.x]);
-  const BatchMember &B = desc[id];
-  const int lb = (int)blockIdx.x - B.block0;
+    k_batch_vdel4_vc(const BatchMember *__restrict__ desc, const int *__restrict__ blk,
+                     int nblocks, int persistent) {
+  for_entries(nblocks, persistent, [&](int b) {
+    const int id = uniform(blk[b]);
+    const BatchMember &B = desc[id];
+    if (!B.vdel4) return;
+    const Place p = place(B, b - B.block0, N);
+    if (!p.active) return;
+    const MeshView &m = B.m;
+    const DiagOut &d = B.d;
+    const int vb = m.vofs[p.c0], ve = m.vofs[p.c1];
+    for (int v = vb + p.ty; v < ve; v += p.R) vdel4_vertex<N>(m, d, v, p.k);
+    for (int c = p.c0 + p.ty; c < p.c1; c += p.R) vdel4_cell<N>(m, d, c, p.k);
+  });
+}
+
+// Pattern-specialised phases, N = 1: one cell (its 3 owned edges and 2
+// owned vertices) per row of 64 lanes.
+__device__ __forceinline__ void hex1_diag(const BatchMember &B, int lb) {
   const int chunk = lb % B.chunks, seg = lb / B.chunks;
-  const int c = hexops::ent<N>(seg * B.rows + (int)threadIdx.x / B.lx);
+  const int c = hexops::ent<1>(seg * B.rows + (int)threadIdx.x / B.lx);
   if (c >= B.m.nc) return;
-  const int k = (chunk * B.lx + (int)threadIdx.x % B.lx) * N;
+  const int k = chunk * B.lx + (int)threadIdx.x % B.lx;
+  const MeshView &m = B.m;
+  const DiagOut &d = B.d;
+  const double *h = B.a.h.in, *vn = B.a.vn.in, *tr = B.a.tr.in;
+  if (B.dterms & (T_PV | T_VDEL2 | T_VDEL4)) {
+    hexops::vertex_diag_u<1>(m, 2 * c, k, h, vn, d);
+    hexops::vertex_diag_u<1>(m, 2 * c + 1, k, h, vn, d);
+  }
+  hexops::cell_diag_u<1>(m, B.dterms, B.a.ntr, c, k, h, vn, tr, d);
+}
+
+__device__ __forceinline__ void hex1_tend(const BatchMember &B, int lb) {
+  const int chunk = lb % B.chunks, seg = lb / B.chunks;
+  const int c = hexops::ent<1>(seg * B.rows + (int)threadIdx.x / B.lx);
+  if (c >= B.m.nc) return;
+  const int k = chunk * B.lx + (int)threadIdx.x % B.lx;
   const MeshView &m = B.m;
   const ModelCoef &mc = B.mc;
   const StageArgs &a = B.a;
   const DiagOut &d = B.d;
   const int e0 = 3 * c;
-  hexops::edge_tend_hex<N, 0>(m, mc, a, d, e0, k);
-  hexops::edge_tend_hex<N, 1>(m, mc, a, d, e0 + 1, k);
-  hexops::edge_tend_hex<N, 2>(m, mc, a, d, e0 + 2, k);
-  hexops::cell_tend_hex<N>(m, mc, a, d, c, k);
+  hexops::edge_tend_hex<1, 0>(m, mc, a, d, e0, k);
+  hexops::edge_tend_hex<1, 1>(m, mc, a, d, e0 + 1, k);
+  hexops::edge_tend_hex<1, 2>(m, mc, a, d, e0 + 2, k);
+  hexops::cell_tend_hex<1>(m, mc, a, d, c, k);
 }
-
 // N = 2 forms: the two rows of 32 lanes of a wavefront carry consecutive
 // cells cA, cA + 1 of the member, whose tables come through the scalar
 // cache (hex_ops.hpp pair operators), as in the single-model kernels.
@@ -191,28 +200,6 @@ __device__ __forceinline__ Row row_of(const BatchMember &B, int lb) {
   return Row{seg * B.rows + (int)threadIdx.x / B.lx, (chunk * B.lx + (int)threadIdx.x % B.lx) * 2};
 }
 
-__global__ void __launch_bounds__(256)
-    k_batch_diag_hex2(const BatchMember *__restrict__ desc, const int *__restrict__ blk) {
-  const int id = uniform(blk[blockIdx.x]);
-  const BatchMember &B = desc[id];
-  if (!B.dterms) return;
-  if (B.m.L % 64 != 0) {
-    const Row r = row_of(B, (int)blockIdx.x - B.block0);
-    if (r.c >= B.m.nc) return;
-    const double *h = B.a.h.in, *vn = B.a.vn.in, *tr = B.a.tr.in;
-    if (B.dterms & (T_PV | T_VDEL2 | T_VDEL4)) {
-      hexops::vertex_diag_u<2>(B.m, 2 * r.c, r.k, h, vn, B.d);
-      hexops::vertex_diag_u<2>(B.m, 2 * r.c + 1, r.k, h, vn, B.d);
-    }
-    hexops::cell_diag_u<2>(B.m, B.dterms, B.a.ntr, r.c, r.k, h, vn, tr, B.d);
-    return;
-  }
-  const Pair q = pair_of(B, (int)blockIdx.x - B.block0);
-  if (!q.ok) return;
-  if (B.m.coef_uni && B.m.off32) batch_diag_pair<true, false>(B, q);
-  else batch_diag_pair<false, true>(B, q);
-}
-
 template <bool U, bool W>
 __device__ __forceinline__ void batch_tend_pair(const BatchMember &B, const Pair &q) {
   const MeshView &m = B.m;
@@ -234,12 +221,27 @@ __device__ __forceinline__ void batch_tend_pair(const BatchMember &B, const Pair
   hexops::cell_tend_hex2<U, W>(m, mc, a, d, q.cA, q.cB, q.hi, q.k);
 }
 
-__global__ void __launch_bounds__(256)
-    k_batch_tend_hex2(const BatchMember *__restrict__ desc, const int *__restrict__ blk) {
-  const int id = uniform(blk[blockIdx.x]);
-  const BatchMember &B = desc[id];
+__device__ __forceinline__ void hex2_diag(const BatchMember &B, int lb) {
   if (B.m.L % 64 != 0) {
-    const Row r = row_of(B, (int)blockIdx.x - B.block0);
+    const Row r = row_of(B, lb);
+    if (r.c >= B.m.nc) return;
+    const double *h = B.a.h.in, *vn = B.a.vn.in, *tr = B.a.tr.in;
+    if (B.dterms & (T_PV | T_VDEL2 | T_VDEL4)) {
+      hexops::vertex_diag_u<2>(B.m, 2 * r.c, r.k, h, vn, B.d);
+      hexops::vertex_diag_u<2>(B.m, 2 * r.c + 1, r.k, h, vn, B.d);
+    }
+    hexops::cell_diag_u<2>(B.m, B.dterms, B.a.ntr, r.c, r.k, h, vn, tr, B.d);
+    return;
+  }
+  const Pair q = pair_of(B, lb);
+  if (!q.ok) return;
+  if (B.m.coef_uni && B.m.off32) batch_diag_pair<true, false>(B, q);
+  else batch_diag_pair<false, true>(B, q);
+}
+
+__device__ __forceinline__ void hex2_tend(const BatchMember &B, int lb) {
+  if (B.m.L % 64 != 0) {
+    const Row r = row_of(B, lb);
     if (r.c >= B.m.nc) return;
     hexops::edge_tend_hex<2, 0>(B.m, B.mc, B.a, B.d, 3 * r.c, r.k);
     hexops::edge_tend_hex<2, 1>(B.m, B.mc, B.a, B.d, 3 * r.c + 1, r.k);
@@ -247,62 +249,79 @@ __global__ void __launch_bounds__(256)
     hexops::cell_tend_hex<2>(B.m, B.mc, B.a, B.d, r.c, r.k);
     return;
   }
-  const Pair q = pair_of(B, (int)blockIdx.x - B.block0);
+  const Pair q = pair_of(B, lb);
   if (!q.ok) return;
   if (B.m.coef_uni && B.mc.uniform && B.m.off32) batch_tend_pair<true, false>(B, q);
   else batch_tend_pair<false, true>(B, q);
+}
+
+// One launch per stage phase over every member: each table entry runs the
+// pattern-specialised or the general operators its member was built for.
+template <int N>
+__global__ void __launch_bounds__(256)
+    k_batch_diag(const BatchMember *__restrict__ desc, const int *__restrict__ blk, int nblocks,
+                 int persistent) {
+  for_entries(nblocks, persistent, [&](int b) {
+    const int id = uniform(blk[b]);
+    const BatchMember &B = desc[id];
+    if (!B.dterms) return;
+    const int lb = b - B.block0;
+    if (B.hex) {
+      if constexpr (N == 2) hex2_diag(B, lb);
+      else hex1_diag(B, lb);
+    } else {
+      gen_diag<N>(B, lb);
+    }
+  });
+}
+
+template <int N>
+__global__ void __launch_bounds__(256)
+    k_batch_tend(const BatchMember *__restrict__ desc, const int *__restrict__ blk, int nblocks,
+                 int persistent) {
+  for_entries(nblocks, persistent, [&](int b) {
+    const int id = uniform(blk[b]);
+    const BatchMember &B = desc[id];
+    const int lb = b - B.block0;
+    if (B.hex) {
+      if constexpr (N == 2) hex2_tend(B, lb);
+      else hex1_tend(B, lb);
+    } else {
+      gen_tend<N>(B, lb);
+    }
+  });
 }
 
 } // namespace
 
 #define OMEGA_BATCH_LAUNCH(K)                                                               \
   do {                                                                                      \
-    if (N == 2) hipLaunchKernelGGL(K<2>, dim3(nblocks), dim3(threads), 0, s, desc, blk);    \
-    else hipLaunchKernelGGL(K<1>, dim3(nblocks), dim3(threads), 0, s, desc, blk);           \
+    const int grid = persistent ? std::min(persistent, nblocks) : nblocks;                  \
+    const int pers = persistent && persistent < nblocks ? 1 : 0;                            \
+    if (N == 2)                                                                             \
+      hipLaunchKernelGGL(K<2>, dim3(grid), dim3(threads), 0, s, desc, blk, nblocks, pers);  \
+    else                                                                                    \
+      hipLaunchKernelGGL(K<1>, dim3(grid), dim3(threads), 0, s, desc, blk, nblocks, pers);  \
     OMEGA_HIP_CHECK(hipGetLastError());                                                     \
   } while (0)
 
 void launch_batch_diag(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
-                       hipStream_t s) {
+                       int persistent, hipStream_t s) {
   TimerScope ts("batch_diag", s);
   OMEGA_BATCH_LAUNCH(k_batch_diag);
 }
 
 void launch_batch_vdel4(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
-                        hipStream_t s) {
+                        int persistent, hipStream_t s) {
   TimerScope ts("batch_vdel4", s);
   OMEGA_BATCH_LAUNCH(k_batch_vdel4_edge);
   OMEGA_BATCH_LAUNCH(k_batch_vdel4_vc);
 }
 
 void launch_batch_tend(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
-                       hipStream_t s) {
+                       int persistent, hipStream_t s) {
   TimerScope ts("batch_tend", s);
   OMEGA_BATCH_LAUNCH(k_batch_tend);
-}
-
-void launch_batch_diag_hex(const BatchMember *desc, const int *blk, int nblocks, int N,
-                           hipStream_t s) {
-  TimerScope ts("batch_diag", s);
-  const int threads = 256;
-  if (N == 2) {
-    hipLaunchKernelGGL(k_batch_diag_hex2, dim3(nblocks), dim3(threads), 0, s, desc, blk);
-    OMEGA_HIP_CHECK(hipGetLastError());
-  } else {
-    OMEGA_BATCH_LAUNCH(k_batch_diag_hex);
-  }
-}
-
-void launch_batch_tend_hex(const BatchMember *desc, const int *blk, int nblocks, int N,
-                           hipStream_t s) {
-  TimerScope ts("batch_tend", s);
-  const int threads = 256;
-  if (N == 2) {
-    hipLaunchKernelGGL(k_batch_tend_hex2, dim3(nblocks), dim3(threads), 0, s, desc, blk);
-    OMEGA_HIP_CHECK(hipGetLastError());
-  } else {
-    OMEGA_BATCH_LAUNCH(k_batch_tend_hex);
-  }
 }
 
 #undef OMEGA_BATCH_LAUNCH

@@ -19,7 +19,6 @@ using omega_hip::Real;
 // Opaque objects
 // ---------------------------------------------------------------------------
 #include "adjoint.hpp"
-#include "tiles.hpp"
 
 namespace omega_hip {
 // Destroys cached graph executables after the device has finished every
@@ -36,7 +35,6 @@ struct omega_mesh_s {
   DeviceMesh dm;
   DBuf<Real> staging;             // device scratch for omega_add_host_field
   omega_hip::AdjLayout adj;       // transposed tables, built on first adjoint use
-  omega_hip::TileTables tiles;    // LDS tile tables, built on first tiled launch
 };
 
 struct omega_state_s {
@@ -58,7 +56,7 @@ struct omega_model_s {
   int terms = 0;
   Launch lc;
   // configuration generation: bumped by every setter that changes what a
-  // step's kernels are or read (terms, launch shape, variant, tiling, forms,
+  // step's kernels are or read (terms, launch shape, kernel family, forms,
   // f_vertex, hook), so captured graphs and ensemble tables built for an
   // older configuration are never replayed
   uint64_t gen = 0;
@@ -191,10 +189,6 @@ hipStream_t capture_stream();
 void model_build_coefficients(omega_model_s *M);
 void ensure_vdel4(omega_model_s *M);
 void model_build_linear(omega_model_s *M);
-// Cells per tile of the tiled kernels for this model's launch shape, and
-// the tables for it (built before any capture).
-int tile_cells(const omega_model_s *M);
-void ensure_tiles(omega_model_s *M);
 void run_diag(omega_model_s *M, const Real *h, const Real *vn, const Real *tr, hipStream_t s);
 // fields: bit 0 h, bit 1 vn, bit 2 tracers (all by default); with_diag =
 // false reuses the diagnostics of the last run_diag

@@ -86,15 +86,6 @@ void ensure_vdel4(omega_model_s *M) {
   }
 }
 
-int tile_cells(const omega_model_s *M) { return M->lc.tile; }
-
-void ensure_tiles(omega_model_s *M) {
-  if (M->lc.variant != 2 && M->lc.variant != 4) return;
-  omega_mesh_s *mesh = M->mesh;
-  const int T = tile_cells(M);
-  if (mesh->tiles.tcells != T) build_tile_tables(mesh->dm, T, mesh->tiles);
-}
-
 hipStream_t capture_stream() {
   static std::mutex mu;
   static std::map<int, hipStream_t> streams;
@@ -156,8 +147,7 @@ void run_stage_into(omega_model_s *M, StageArgs a, hipStream_t s, int fields,
     return;
   }
   if (dx) {
-    // caller-provided diagnostics buffers: the default kernels only (the
-    // tiled variants build tables on first use)
+    // caller-provided diagnostics buffers
     run_diag_into(M, a.h.in, a.vn.in, a.tr.in, *dx, s);
     if (hex_usable(mv, M->lc) && !general_only(M)) launch_tend_hex(mv, M->lc, mc, a, *dx, s);
     else launch_tend(mv, M->lc, mc, a, *dx, s);
@@ -186,18 +176,8 @@ void run_stage(omega_model_s *M, StageArgs a, hipStream_t s, int fields, bool wi
     launch_tend(mv, M->lc, mc, a, d, s);
     return;
   }
-  if (M->lc.variant == 2 || M->lc.variant == 4) ensure_tiles(M);
-  if (M->lc.variant == 2) {
-    launch_tend_tile(mv, M->mesh->tiles.view(), M->lc, mc, a, d, s);
-  } else if (hextile_usable(mv, M->mesh->tiles, M->lc)) {
-    launch_tend_hextile(mv, M->mesh->tiles.view(), M->lc, mc, a, d, s);
-  } else if (hexu_usable(mv, M->lc)) {
-    launch_tend_hexu(mv, M->lc, mc, a, d, s);
-  } else if (hex_usable(mv, M->lc)) {
-    launch_tend_hex(mv, M->lc, mc, a, d, s);
-  } else {
-    launch_tend(mv, M->lc, mc, a, d, s);
-  }
+  if (hex_usable(mv, M->lc)) launch_tend_hex(mv, M->lc, mc, a, d, s);
+  else launch_tend(mv, M->lc, mc, a, d, s);
 }
 
 FieldIO fio(const Real *in, Real *tend, Real *acc, const Real *yold, Real *out) {
@@ -404,13 +384,11 @@ void do_step_maybe_graph(omega_stepper_s *St, omega_state_s *S, double t, double
   }
   omega_model_s *M = St->model;
   if (M->terms & T_VDEL4) ensure_vdel4(M);
-  ensure_tiles(M);
   GraphKey key{dt, S->cur, M->gen, {}};
   for (int f = 0; f < 3; ++f)
     key.bufs.insert(key.bufs.end(), {S->buf[0][f].p, S->buf[1][f].p, St->acc[f].p,
                                      St->p1[f].p, St->p2[f].p, St->tend[f].p});
-  key.bufs.insert(key.bufs.end(), {M->vdel2_e.p, M->trdel2_c.p, M->mesh->tiles.ue.p,
-                                   M->mesh->tiles.cc.p, M->mesh->tiles.hexl.p, M->f_v.p});
+  key.bufs.insert(key.bufs.end(), {M->vdel2_e.p, M->trdel2_c.p, M->f_v.p});
   const DiagOut dk = M->diag();
   key.bufs.insert(key.bufs.end(), {dk.rvort_v, dk.inv_thick_v, dk.ke_c, dk.div_c, dk.trdel2_c,
                                    dk.vdel2_e, dk.vdel2_rv_v, dk.vdel2_div_c});

@@ -108,12 +108,11 @@ def lib():
         "omega_model_get_terms": ([vp, ip], C.c_int),
         "omega_model_set_launch": ([vp, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
         "omega_model_get_launch": ([vp, ip, ip, ip, ip], C.c_int),
+        "omega_model_autotune": ([vp, vp, C.c_int, dp, vp], C.c_int),
         "omega_model_set_variant": ([vp, C.c_int], C.c_int),
         "omega_model_kernel_variant": ([vp, ip], C.c_int),
         "omega_model_kernel_flags": ([vp, ip], C.c_int),
         "omega_model_set_kernel_forms": ([vp, C.c_int], C.c_int),
-        "omega_model_set_tiling": ([vp, C.c_int, C.c_int], C.c_int),
-        "omega_mesh_tile_stats": ([vp, C.c_int, ip], C.c_int),
         "omega_stepper_create": ([vp, C.c_int, C.c_int, pp], C.c_int),
         "omega_stepper_destroy": ([vp], C.c_int),
         "omega_stepper_do_step": ([vp, C.c_double, C.c_double, vp, vp], C.c_int),
@@ -147,6 +146,8 @@ def lib():
         "omega_batch_set_launch": ([vp, C.c_int, C.c_int], C.c_int),
         "omega_batch_get_launch": ([vp, ip, ip], C.c_int),
         "omega_batch_autotune": ([vp, C.c_int, dp, vp], C.c_int),
+        "omega_batch_set_persistent": ([vp, C.c_int], C.c_int),
+        "omega_batch_get_persistent": ([vp, ip], C.c_int),
         "omega_batch_info": ([vp, ip, i64p, i64p], C.c_int),
         "omega_batch_adjoint_run": ([vp, C.c_double, C.c_int, C.c_int, C.c_int64, pp, dp, dp, pp,
                                      dp, vp], C.c_int),
@@ -215,14 +216,6 @@ class MPASMesh:
             check(lib().omega_mesh_permutation(self._h, kind, out.ctypes.data_as(C.POINTER(C.c_int))))
             self._perm[kind] = out
         return self._perm[kind]
-
-    def tile_stats(self, cells_per_tile: int) -> dict:
-        """Per-tile set sizes of the LDS-tiled kernels (host only)."""
-        st = (C.c_int * 7)()
-        check(lib().omega_mesh_tile_stats(self._h, int(cells_per_tile), st))
-        keys = ("tiles", "union_edges", "union_cells", "owned_edge_cells", "union_vertices",
-                "cell_edges", "cell_cells")
-        return dict(zip(keys, list(st)))
 
     @property
     def regular_hex(self) -> bool:
@@ -553,16 +546,17 @@ class ShallowWaterModel:
         check(lib().omega_model_get_launch(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
         return a.value, b.value, c.value, d.value
 
-    def set_variant(self, variant: int):
-        """0: pattern-specialised kernels when possible, 1: general kernels,
-        2: LDS-tiled kernels, 3: union-stencil specialised kernel,
-        4: hexagonal LDS-tiled edge kernel + specialised cell kernel."""
-        check(lib().omega_model_set_variant(self._h, int(variant)))
+    def autotune(self, state: "ShallowWaterState", reps: int = 3, stream=None) -> float:
+        """Times the launch shapes on tendency evaluations of state and keeps the
+        fastest; returns its milliseconds per evaluation."""
+        ms = C.c_double()
+        check(lib().omega_model_autotune(self._h, state._h, int(reps), C.byref(ms),
+                                         _stream(stream)))
+        return ms.value
 
-    def set_tiling(self, cells_per_tile: int = 32, levels_per_block: int = 8):
-        """Tile shape of the LDS-tiled kernels (variants 2 and 4): cells per
-        tile and levels per block (8/16 for variant 2, 16/32 for variant 4)."""
-        check(lib().omega_model_set_tiling(self._h, int(cells_per_tile), int(levels_per_block)))
+    def set_variant(self, variant: int):
+        """0: pattern-specialised kernels when possible, 1: general kernels."""
+        check(lib().omega_model_set_variant(self._h, int(variant)))
 
     def kernel_flags(self) -> int:
         """bit 0: coefficient tables uniform per stencil slot (shared by a
@@ -577,8 +571,7 @@ class ShallowWaterModel:
         check(lib().omega_model_set_kernel_forms(self._h, int(forms)))
 
     def kernel_variant(self) -> int:
-        """6 linear model, 5 hexagonal tiled, 4 union-stencil, 3 tiled, 2 pattern-specialised,
-        1 general kernels."""
+        """6 linear model, 2 pattern-specialised, 1 general kernels."""
         v = C.c_int()
         check(lib().omega_model_kernel_variant(self._h, C.byref(v)))
         return v.value
@@ -806,6 +799,15 @@ class Batch:
         t, n = C.c_int(), C.c_int()
         check(lib().omega_batch_get_launch(self._h, C.byref(t), C.byref(n)))
         return t.value, n.value
+
+    def set_persistent(self, blocks_per_cu: int = 0):
+        """0: one block per table entry; n > 0: a persistent grid of n blocks per CU."""
+        check(lib().omega_batch_set_persistent(self._h, int(blocks_per_cu)))
+
+    def get_persistent(self) -> int:
+        v = C.c_int()
+        check(lib().omega_batch_get_persistent(self._h, C.byref(v)))
+        return v.value
 
     def autotune(self, reps: int = 3, stream=None) -> float:
         ms = C.c_double()

@@ -161,3 +161,77 @@ def test_batch_rejects_hook_and_shared_state():
     b.add(model, s, dt)
     with pytest.raises(swe.OmegaError, match="one member"):
         b.add(model, s, dt)
+
+
+@pytest.mark.parametrize("threads", [256, 128, 64])
+@pytest.mark.parametrize("persistent", [0, 1, 4])
+def test_batch_launch_shapes_and_persistent_grid(threads, persistent):
+    """Every cells-per-block choice and the persistent grid (blocks walking
+    runs of table entries; 1 block per CU is far fewer blocks than entries)
+    give each member the bits of its solo run; general-kernel members (odd
+    layer counts, 2-, 4-, 6- and 8-layer meshes) and pattern-specialised ones
+    share each launch."""
+    print("launch", threads, persistent, flush=True)
+    members = make_members(MEMBERS)
+    ref = solo(members, "lsrk", 5, 2)
+    b = swe.Batch("lsrk", 5)
+    states = []
+    for mesh, model, ntr, y0, dt, _ in members:
+        s = state_of(mesh, ntr, y0)
+        b.add(model, s, dt)
+        states.append(s)
+    b.set_launch(threads, 2)
+    b.set_persistent(persistent)
+    b.set_graph(True)
+    b.run(2)
+    for i, (s, r) in enumerate(zip(states, ref)):
+        for name, a, c in zip(("h", "vn", "tr"), s.download(), r):
+            assert same_bits(a, c), (i, name)
+
+
+def test_batch_autotune_cache(tmp_path, monkeypatch):
+    """A tuned choice is written to OMEGA_TUNE_CACHE and applied from it by
+    the next autotune of the same ensemble without timing."""
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("OMEGA_TUNE_CACHE", str(cache))
+    members = make_members(MEMBERS[:3])
+
+    def batch():
+        b = swe.Batch("lsrk", 5)
+        for mesh, model, ntr, y0, dt, _ in members:
+            b.add(model, state_of(mesh, ntr, y0), dt)
+        return b
+
+    b1 = batch()
+    ms1 = b1.autotune(reps=1)
+    choice = (b1.get_launch(), b1.get_persistent())
+    assert cache.read_text().count("\n") == 1
+    b2 = batch()
+    b2.set_launch(64, 1)
+    ms2 = b2.autotune(reps=1)
+    assert (b2.get_launch(), b2.get_persistent()) == choice
+    assert ms2 == pytest.approx(ms1)
+    assert cache.read_text().count("\n") == 1
+
+
+def test_model_autotune_keeps_results():
+    """The single-model tuner picks a launch shape; tendencies and steps stay
+    the oracle's bits under it."""
+    nx, L, ntr, dc = 12, 64, 2, 10e3
+    pd = params_dict(dc, ntr)
+    om = po.Mesh(nx, nx, dc, L)
+    omodel = po.Model(om, **pd)
+    gm = swe.PlanarHexagonalMesh(nx, nx, dc, L)
+    gmodel = swe.ShallowWaterModel(gm, swe.ShallowWaterParams(**pd))
+    y0 = rich_state(om, L, ntr, 3)
+    gs = state_of(gm, ntr, y0)
+    ms = gmodel.autotune(gs, reps=2)
+    assert ms > 0
+    print("model autotune", gmodel.get_launch(), ms, flush=True)
+    st = swe.RK4Stepper(gmodel)
+    st.set_graph(True)
+    st.run(0.0, 20.0, 2, gs)
+    ref = tuple(a.copy() for a in y0)
+    po.Stepper(omodel, "rk4").run(0.0, 20.0, 2, ref)
+    for a, c in zip(gs.download(), ref):
+        assert same_bits(a, c)

@@ -1,8 +1,7 @@
 """GPU tests on general MPAS meshes: the icosahedral sphere mesh (twelve
 pentagons among hexagons, so nedges_on_cell 5 or 6 and nedges_on_edge 9 or
 10) runs the general kernels, which must give the oracle's bits for the
-tendency, every term, RK4 and low-storage RK steps, the LDS-tiled variant
-and ensembles; the diagnostics match to 1e-13; the discrete adjoint
+tendency, every term, RK4 and low-storage RK steps and ensembles; the diagnostics match to 1e-13; the discrete adjoint
 matches torch.autograd on the CPU restatement to 1e-10.  Meshes read from
 MPAS NetCDF files behave exactly like the same arrays passed in memory,
 including 8-column rows on the specialised hexagonal kernels."""
@@ -88,21 +87,6 @@ def test_icosahedral_steps_bitwise(kind, nstages, graph):
         st.set_graph(True)
     gs = upload(gm, 2, state)
     st.run(0.0, DT, 3, gs)
-    for name, a, b in zip(("h", "vn", "tr"), gs.download(), ref):
-        assert same_bits(a, b), name
-
-
-@pytest.mark.parametrize("tiling", [(8, 8), (32, 16)])
-def test_icosahedral_tiled_kernels_bitwise(tiling):
-    print("tiling", tiling, flush=True)
-    om, omodel, gm, gmodel, pd, state = make(3, 16, 2, seed=6)
-    gmodel.set_variant(2)
-    gmodel.set_tiling(*tiling)
-    check_tendency(omodel, gm, gmodel, 2, state)
-    ref = tuple(a.copy() for a in state)
-    po.Stepper(omodel, "lsrk", 5).run(0.0, DT, 2, ref)
-    gs = upload(gm, 2, state)
-    swe.LSRKStepper(gmodel).run(0.0, DT, 2, gs)
     for name, a, b in zip(("h", "vn", "tr"), gs.download(), ref):
         assert same_bits(a, b), name
 

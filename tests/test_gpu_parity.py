@@ -385,153 +385,3 @@ def test_general_mpas_mesh_masks_and_scalings_bitwise(ordering):
         assert same_bits(a, b), name
 
 
-@pytest.mark.parametrize("ordering", ["original", "hilbert", "morton"])
-@pytest.mark.parametrize("tiling", [(32, 8), (16, 16), (5, 8), (64, 8)])
-@pytest.mark.parametrize("lpt", [1, 2])
-def test_tiled_kernels_bitwise(ordering, tiling, lpt):
-    """The LDS-tiled tendency kernels give the oracle's bits (layer counts
-    that are not a multiple of the level chunk included)."""
-    om, omodel, gm, gmodel, state = make_pair(14, 10, 12, 5, ordering=ordering)
-    print("case", ordering, tiling, lpt, flush=True)
-    gmodel.set_variant(2)
-    gmodel.set_tiling(*tiling)
-    gmodel.set_launch(0, lpt, 1, 0)
-    assert gmodel.kernel_variant() == 3
-    tend_o = omodel.new_state()
-    omodel.compute_tendency(tend_o, state, 0.0)
-    gs = upload(gm, 5, state)
-    gt = swe.ShallowWaterState(gm, 5)
-    gmodel.compute_tendency(gt, gs, 0.0)
-    for name, a, b in zip(("h", "vn", "tr"), gt.download(), tend_o):
-        assert same_bits(a, b), name
-
-
-@pytest.mark.parametrize("terms", [1, 2, 4, 8, 16, 32, 64, ALL_TERMS])
-@pytest.mark.parametrize("L", [64, 7])
-def test_tiled_each_term_bitwise(terms, L):
-    om, omodel, gm, gmodel, state = make_pair(12, 12, L, 2)
-    omodel.terms = terms
-    gmodel.terms = terms
-    gmodel.set_variant(2)
-    tend_o = omodel.new_state()
-    omodel.compute_tendency(tend_o, state, 0.0)
-    gt = swe.ShallowWaterState(gm, 2)
-    gmodel.compute_tendency(gt, upload(gm, 2, state), 0.0)
-    for a, b in zip(gt.download(), tend_o):
-        assert same_bits(a, b)
-
-
-@pytest.mark.parametrize("kind,nstages", [("rk4", 4), ("lsrk", 5)])
-@pytest.mark.parametrize("graph", [False, True])
-@pytest.mark.parametrize("disable", [(False, False), (True, False), (False, True)])
-def test_tiled_steps_bitwise(kind, nstages, graph, disable):
-    om, omodel, gm, gmodel, state = make_pair(12, 10, 64, 3, disable_h=disable[0],
-                                              disable_vn=disable[1])
-    gmodel.set_variant(2)
-    ref = tuple(a.copy() for a in state)
-    po.Stepper(omodel, kind, nstages).run(0.0, 30.0, 3, ref)
-    gs = upload(gm, 3, state)
-    st = swe.RK4Stepper(gmodel) if kind == "rk4" else swe.LSRKStepper(gmodel, nstages)
-    st.set_graph(graph)
-    st.run(0.0, 30.0, 3, gs)
-    for name, a, b in zip(("h", "vn", "tr"), gs.download(), ref):
-        assert same_bits(a, b), name
-
-
-def test_tiled_general_mpas_mesh_bitwise():
-    nx, ny, L, ntr, dc = 10, 8, 6, 2, 10e3
-    om = po.Mesh(nx, ny, dc, L)
-    rng = np.random.default_rng(8)
-    om.mesh_scaling_del2[:] = rng.uniform(0.5, 1.5, om.nedges)
-    om.mesh_scaling_del4[:] = rng.uniform(0.5, 1.5, om.nedges)
-    top = rng.integers(2, L + 1, om.nedges)
-    om.max_level_edge_top[:] = top
-    om.edge_mask[:] = (np.arange(L)[None, :] < top[:, None]).astype(np.float64)
-    pd = params_dict(dc, ntr)
-    omodel = po.Model(om, **pd)
-    gm = swe.MPASMesh.from_arrays({f: np.array(getattr(om, f)) for f in MPAS_FIELDS}, L, "hilbert")
-    gmodel = swe.ShallowWaterModel(gm, swe.ShallowWaterParams(**pd))
-    gmodel.set_variant(2)
-    gmodel.set_tiling(8, 8)
-    state = rich_state(om, L, ntr, 4)
-    ref = tuple(a.copy() for a in state)
-    po.Stepper(omodel, "lsrk", 5).run(0.0, 20.0, 2, ref)
-    gs = upload(gm, ntr, state)
-    swe.LSRKStepper(gmodel).run(0.0, 20.0, 2, gs)
-    for name, a, b in zip(("h", "vn", "tr"), gs.download(), ref):
-        assert same_bits(a, b), name
-
-
-@pytest.mark.parametrize("ordering", ["original", "hilbert", "morton"])
-@pytest.mark.parametrize("terms", [1, 2, 4, 8, 16, 32, 64, ALL_TERMS])
-def test_union_stencil_kernel_bitwise(ordering, terms):
-    """The union-stencil specialised kernel (one wavefront per cell, shared
-    rows loaded once) gives the oracle's bits."""
-    om, omodel, gm, gmodel, state = make_pair(12, 10, 128, 3, ordering=ordering)
-    print("case", ordering, terms, flush=True)
-    omodel.terms = terms
-    gmodel.terms = terms
-    gmodel.set_variant(3)
-    assert gmodel.kernel_variant() == 4
-    tend_o = omodel.new_state()
-    omodel.compute_tendency(tend_o, state, 0.0)
-    gt = swe.ShallowWaterState(gm, 3)
-    gmodel.compute_tendency(gt, upload(gm, 3, state), 0.0)
-    for name, a, b in zip(("h", "vn", "tr"), gt.download(), tend_o):
-        assert same_bits(a, b), name
-
-
-@pytest.mark.parametrize("kind,nstages", [("rk4", 4), ("lsrk", 5)])
-@pytest.mark.parametrize("disable", [(False, False), (True, False), (False, True)])
-def test_union_stencil_steps_bitwise(kind, nstages, disable):
-    om, omodel, gm, gmodel, state = make_pair(12, 10, 64, 2, disable_h=disable[0],
-                                              disable_vn=disable[1])
-    gmodel.set_variant(3)
-    ref = tuple(a.copy() for a in state)
-    po.Stepper(omodel, kind, nstages).run(0.0, 30.0, 3, ref)
-    gs = upload(gm, 2, state)
-    st = swe.RK4Stepper(gmodel) if kind == "rk4" else swe.LSRKStepper(gmodel, nstages)
-    st.set_graph(True)
-    st.run(0.0, 30.0, 3, gs)
-    for name, a, b in zip(("h", "vn", "tr"), gs.download(), ref):
-        assert same_bits(a, b), name
-
-
-@pytest.mark.parametrize("ordering", ["original", "hilbert", "morton"])
-@pytest.mark.parametrize("terms", [1, 2, 4, 8, 16, 32, 64, ALL_TERMS])
-@pytest.mark.parametrize("tiling", [(32, 16), (16, 32), (7, 16), (32, 8)])
-def test_hex_tiled_kernel_bitwise(ordering, terms, tiling):
-    """The hexagonal LDS-tiled edge kernel (tile rows staged once, stencil
-    slots from the built-in pattern, partial last tile) gives the oracle's
-    bits."""
-    om, omodel, gm, gmodel, state = make_pair(12, 10, 128, 3, ordering=ordering)
-    print("case", ordering, terms, tiling, flush=True)
-    omodel.terms = terms
-    gmodel.terms = terms
-    gmodel.set_variant(4)
-    gmodel.set_tiling(*tiling)
-    assert gmodel.kernel_variant() == 5
-    tend_o = omodel.new_state()
-    omodel.compute_tendency(tend_o, state, 0.0)
-    gt = swe.ShallowWaterState(gm, 3)
-    gmodel.compute_tendency(gt, upload(gm, 3, state), 0.0)
-    for name, a, b in zip(("h", "vn", "tr"), gt.download(), tend_o):
-        assert same_bits(a, b), name
-
-
-@pytest.mark.parametrize("kind,nstages", [("rk4", 4), ("lsrk", 5)])
-@pytest.mark.parametrize("disable", [(False, False), (True, False), (False, True)])
-def test_hex_tiled_steps_bitwise(kind, nstages, disable):
-    om, omodel, gm, gmodel, state = make_pair(12, 10, 64, 2, disable_h=disable[0],
-                                              disable_vn=disable[1])
-    print("case", kind, disable, flush=True)
-    gmodel.set_variant(4)
-    gmodel.set_tiling(32, 16)
-    ref = tuple(a.copy() for a in state)
-    po.Stepper(omodel, kind, nstages).run(0.0, 30.0, 3, ref)
-    gs = upload(gm, 2, state)
-    st = swe.RK4Stepper(gmodel) if kind == "rk4" else swe.LSRKStepper(gmodel, nstages)
-    st.set_graph(True)
-    st.run(0.0, 30.0, 3, gs)
-    for name, a, b in zip(("h", "vn", "tr"), gs.download(), ref):
-        assert same_bits(a, b), name
