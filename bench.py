@@ -249,7 +249,8 @@ def adjoint_bench(n_snapshots: int = 10, nsteps: int | None = None, graph: bool 
     }
 
 
-def batched_bench(nmembers: int = 48, nsteps: int = 50, specs=None, label=None):
+def batched_bench(nmembers: int = 48, nsteps: int = 50, specs=None, label=None, graph=True,
+                  tune=True):
     """The ensemble workload: 48 seeded members (nx in {32..192}, 16/32/64
     layers, 1-4 tracers, double vortex with offset centres and random dH),
     LSRK5, 50 steps, del2 viscosity 1e3 and tracer del4 1e9 as the forward
@@ -286,8 +287,8 @@ def batched_bench(nmembers: int = 48, nsteps: int = 50, specs=None, label=None):
     batch = swe.Batch("lsrk", 5)
     for (mesh, model, sp, y0, dt), s in zip(members, states_b):
         batch.add(model, s, dt)
-    batch.set_graph(True)
-    tune_ms = batch.autotune(reps=3)
+    batch.set_graph(graph)
+    tune_ms = batch.autotune(reps=3) if tune else None
     batch.run(2)   # capture the graphs of both buffer parities
     swe.synchronize()
     for (mesh, model, sp, y0, dt), s in zip(members, states_b):
@@ -301,7 +302,7 @@ def batched_bench(nmembers: int = 48, nsteps: int = 50, specs=None, label=None):
     steppers = []
     for mesh, model, sp, y0, dt in members:
         st = swe.LSRKStepper(model, 5)
-        st.set_graph(True)
+        st.set_graph(graph)
         steppers.append(st)
     for st, (mesh, model, sp, y0, dt), s in zip(steppers, members, states_l):
         st.run(0.0, dt, 2, s)   # capture both buffer parities
@@ -469,10 +470,12 @@ def main():
     ap.add_argument("--c5-steps", type=int, default=500,
                     help="steps of the ensemble adjoint leg (the workload's length)")
     ap.add_argument("--skip-reference-shape", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=50)
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--launch", default="", help="rows,lpt,swizzle,levels_per_block")
-    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default launch shapes (model and ensembles)")
     ap.add_argument("--variant", type=int, default=None,
                     help="0 specialised if possible, 1 general")
     args = ap.parse_args()
@@ -558,15 +561,18 @@ def main():
     bat = None
     if not args.skip_batched:
         log("[bench] ensemble workload")
-        bat = batched_bench()
+        bat = batched_bench(nsteps=args.batched_steps, graph=not args.no_graph,
+                            tune=not args.no_tune)
         log(f"[bench] ensemble: batched {bat['batched_s']:.3f} s vs looped {bat['looped_s']:.3f} s "
             f"= {bat['speedup']:.2f}x, bit-identical {bat['bit_identical']}, launch {bat['launch']}, "
             f"{bat['batched_frac']:.3f} vs {bat['looped_frac']:.3f} of peak")
         small = [dict(nx=32, dc=10e3, nlayers=16, ntracers=2,
                       centres=((0.4, 0.5), (0.6, 0.5)), dh=75.0, seed=i) for i in range(48)]
         bat["small_ensemble"] = batched_bench(
-            specs=small, label="48 members, nx=32 (1024 cells), 16 layers, 2 tracers, LSRK5, "
-                               "50 steps, double vortex, del2 visc 1e3, tracer del4 1e9")
+            nsteps=args.batched_steps, specs=small, graph=not args.no_graph,
+            tune=not args.no_tune,
+            label=f"48 members, nx=32 (1024 cells), 16 layers, 2 tracers, LSRK5, "
+                  f"{args.batched_steps} steps, double vortex, del2 visc 1e3, tracer del4 1e9")
         sm = bat["small_ensemble"]
         log(f"[bench] small ensemble: batched {sm['batched_s']:.4f} s vs looped "
             f"{sm['looped_s']:.4f} s = {sm['speedup']:.2f}x (eager loop {sm['speedup_vs_eager_loop']:.2f}x)")

@@ -38,6 +38,7 @@ struct omega_batch_s {
   int parity = 0;
   // one table over all members; member kinds share each launch
   int nblocks = 0;
+  int kinds = 0; // bit 0 general-kernel members, bit 1 pattern-specialised
   bool any_vdel4 = false;
   DBuf<int> blk;
   DBuf<BatchMember> desc; // [parity][stage][member]
@@ -132,6 +133,8 @@ void build_tables(omega_batch_s *B) {
     mb.gen = M->gen;
   }
   B->nblocks = (int)blk.size();
+  B->kinds = 0;
+  for (const BatchMember &g : geo) B->kinds |= g.hex ? 2 : 1;
   std::vector<BatchMember> tab((size_t)2 * ns * nm);
   for (int p = 0; p < 2; ++p)
     for (int s = 0; s < ns; ++s)
@@ -230,9 +233,9 @@ void step_launches(omega_batch_s *B, int parity, hipStream_t s) {
     const BatchMember *d = B->desc.p + ((size_t)parity * B->nstages + st) * nm;
     const int *g = B->blk.p;
     const int n = B->nblocks;
-    launch_batch_diag(d, g, n, B->threads, B->N, pers, s);
+    launch_batch_diag(d, g, n, B->threads, B->N, pers, B->kinds, s);
     if (B->any_vdel4) launch_batch_vdel4(d, g, n, B->threads, B->N, pers, s);
-    launch_batch_tend(d, g, n, B->threads, B->N, pers, s);
+    launch_batch_tend(d, g, n, B->threads, B->N, pers, B->kinds, s);
   }
 }
 

@@ -25,12 +25,13 @@ struct BatchMember {
 // One launch per stage phase over every member's table entries; blk[b] =
 // member of entry b.  persistent > 0 launches that many blocks (when fewer
 // than the entries), each walking a contiguous run of entries; 0 launches
-// one block per entry.
+// one block per entry.  kinds: bit 0 the ensemble has general-kernel
+// members, bit 1 pattern-specialised members.
 void launch_batch_diag(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
-                       int persistent, hipStream_t s);
+                       int persistent, int kinds, hipStream_t s);
 void launch_batch_vdel4(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
                         int persistent, hipStream_t s);
 void launch_batch_tend(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
-                       int persistent, hipStream_t s);
+                       int persistent, int kinds, hipStream_t s);
 
 } // namespace omega_hip

@@ -40,19 +40,20 @@ __device__ __forceinline__ Place place(const BatchMember &B, int lb, int N) {
 }
 
 // Table entries a block handles: one entry per block, or (persistent grid,
-// gridDim.x below the entry count) a contiguous run of entries per block,
-// the runs dealt so that each XCD walks a contiguous part of the table.
-template <class F>
-__device__ __forceinline__ void for_entries(int nblocks, int persistent, F &&f) {
-  if (!persistent) {
+// PERS) a contiguous run of entries per block, the runs dealt so that each
+// XCD walks a contiguous part of the table.  PERS is a template parameter:
+// one code path per kernel keeps the register budget of the entry body.
+template <bool PERS, class F>
+__device__ __forceinline__ void for_entries(int nblocks, F &&f) {
+  if constexpr (!PERS) {
     f((int)blockIdx.x);
-    return;
+  } else {
+    const int G = gridDim.x;
+    const int lb = logical_block(blockIdx.x, G, 1);
+    const int b0 = (int)((long long)lb * nblocks / G);
+    const int b1 = (int)((long long)(lb + 1) * nblocks / G);
+    for (int b = b0; b < b1; ++b) f(b);
   }
-  const int G = gridDim.x;
-  const int lb = logical_block(blockIdx.x, G, 1);
-  const int b0 = (int)((long long)lb * nblocks / G);
-  const int b1 = (int)((long long)(lb + 1) * nblocks / G);
-  for (int b = b0; b < b1; ++b) f(b);
 }
 
 // general-kernel phases of one table entry (member B, block lb of it)
@@ -86,11 +87,11 @@ __device__ __forceinline__ void gen_tend(const BatchMember &B, int lb) {
   for (int c = p.c0 + p.ty; c < p.c1; c += p.R) cell_tend<N>(m, mc, a, d, c, p.k);
 }
 
-template <int N>
+template <int N, bool PERS>
 __global__ void __launch_bounds__(256)
     k_batch_vdel4_edge(const BatchMember *__restrict__ desc, const int *__restrict__ blk,
-                       int nblocks, int persistent) {
-  for_entries(nblocks, persistent, [&](int b) {
+                       int nblocks) {
+  for_entries<PERS>(nblocks, [&](int b) {
     const int id = uniform(blk[b]);
     const BatchMember &B = desc[id];
     if (!B.vdel4) return;
@@ -103,11 +104,11 @@ __global__ void __launch_bounds__(256)
   });
 }
 
-template <int N>
+template <int N, bool PERS>
 __global__ void __launch_bounds__(256)
     k_batch_vdel4_vc(const BatchMember *__restrict__ desc, const int *__restrict__ blk,
-                     int nblocks, int persistent) {
-  for_entries(nblocks, persistent, [&](int b) {
+                     int nblocks) {
+  for_entries<PERS>(nblocks, [&](int b) {
     const int id = uniform(blk[b]);
     const BatchMember &B = desc[id];
     if (!B.vdel4) return;
@@ -257,36 +258,37 @@ __device__ __forceinline__ void hex2_tend(const BatchMember &B, int lb) {
 
 // One launch per stage phase over every member: each table entry runs the
 // pattern-specialised or the general operators its member was built for.
-template <int N>
+// KINDS (bit 0 general members, bit 1 pattern-specialised members) compiles
+// only the paths the ensemble has, so an ensemble of one kind keeps the
+// register budget (and occupancy) of that kind's operators alone.
+template <int N, int KINDS, bool PERS>
 __global__ void __launch_bounds__(256)
-    k_batch_diag(const BatchMember *__restrict__ desc, const int *__restrict__ blk, int nblocks,
-                 int persistent) {
-  for_entries(nblocks, persistent, [&](int b) {
+    k_batch_diag(const BatchMember *__restrict__ desc, const int *__restrict__ blk, int nblocks) {
+  for_entries<PERS>(nblocks, [&](int b) {
     const int id = uniform(blk[b]);
     const BatchMember &B = desc[id];
     if (!B.dterms) return;
     const int lb = b - B.block0;
-    if (B.hex) {
-      if constexpr (N == 2) hex2_diag(B, lb);
-      else hex1_diag(B, lb);
-    } else {
+    if (KINDS == 2 || (KINDS == 3 && B.hex)) {
+      if constexpr ((KINDS & 2) && N == 2) hex2_diag(B, lb);
+      else if constexpr (KINDS & 2) hex1_diag(B, lb);
+    } else if constexpr (KINDS & 1) {
       gen_diag<N>(B, lb);
     }
   });
 }
 
-template <int N>
+template <int N, int KINDS, bool PERS>
 __global__ void __launch_bounds__(256)
-    k_batch_tend(const BatchMember *__restrict__ desc, const int *__restrict__ blk, int nblocks,
-                 int persistent) {
-  for_entries(nblocks, persistent, [&](int b) {
+    k_batch_tend(const BatchMember *__restrict__ desc, const int *__restrict__ blk, int nblocks) {
+  for_entries<PERS>(nblocks, [&](int b) {
     const int id = uniform(blk[b]);
     const BatchMember &B = desc[id];
     const int lb = b - B.block0;
-    if (B.hex) {
-      if constexpr (N == 2) hex2_tend(B, lb);
-      else hex1_tend(B, lb);
-    } else {
+    if (KINDS == 2 || (KINDS == 3 && B.hex)) {
+      if constexpr ((KINDS & 2) && N == 2) hex2_tend(B, lb);
+      else if constexpr (KINDS & 2) hex1_tend(B, lb);
+    } else if constexpr (KINDS & 1) {
       gen_tend<N>(B, lb);
     }
   });
@@ -294,36 +296,85 @@ __global__ void __launch_bounds__(256)
 
 } // namespace
 
-#define OMEGA_BATCH_LAUNCH(K)                                                               \
-  do {                                                                                      \
-    const int grid = persistent ? std::min(persistent, nblocks) : nblocks;                  \
-    const int pers = persistent && persistent < nblocks ? 1 : 0;                            \
-    if (N == 2)                                                                             \
-      hipLaunchKernelGGL(K<2>, dim3(grid), dim3(threads), 0, s, desc, blk, nblocks, pers);  \
-    else                                                                                    \
-      hipLaunchKernelGGL(K<1>, dim3(grid), dim3(threads), 0, s, desc, blk, nblocks, pers);  \
-    OMEGA_HIP_CHECK(hipGetLastError());                                                     \
-  } while (0)
+namespace {
+// grid of a launch: one block per table entry, or the persistent grid
+void grid_of(int nblocks, int persistent, int &grid, bool &pers) {
+  grid = persistent ? std::min(persistent, nblocks) : nblocks;
+  pers = persistent && persistent < nblocks;
+}
+
+template <int N, int KINDS, bool PERS>
+void launch_phase(int phase, const BatchMember *desc, const int *blk, int nblocks, int grid,
+                  int threads, hipStream_t s) {
+  if (phase == 0)
+    hipLaunchKernelGGL((k_batch_diag<N, KINDS, PERS>), dim3(grid), dim3(threads), 0, s, desc,
+                       blk, nblocks);
+  else
+    hipLaunchKernelGGL((k_batch_tend<N, KINDS, PERS>), dim3(grid), dim3(threads), 0, s, desc,
+                       blk, nblocks);
+}
+
+template <int KINDS, bool PERS>
+void launch_n(int phase, const BatchMember *desc, const int *blk, int nblocks, int grid,
+              int threads, int N, hipStream_t s) {
+  if (N == 2) launch_phase<2, KINDS, PERS>(phase, desc, blk, nblocks, grid, threads, s);
+  else launch_phase<1, KINDS, PERS>(phase, desc, blk, nblocks, grid, threads, s);
+}
+
+template <bool PERS>
+void launch_k(int phase, const BatchMember *desc, const int *blk, int nblocks, int grid,
+              int threads, int N, int kinds, hipStream_t s) {
+  if (kinds == 1) launch_n<1, PERS>(phase, desc, blk, nblocks, grid, threads, N, s);
+  else if (kinds == 2) launch_n<2, PERS>(phase, desc, blk, nblocks, grid, threads, N, s);
+  else launch_n<3, PERS>(phase, desc, blk, nblocks, grid, threads, N, s);
+}
+
+void launch_kinds(int phase, const BatchMember *desc, const int *blk, int nblocks, int threads,
+                  int N, int persistent, int kinds, hipStream_t s) {
+  int grid;
+  bool pers;
+  grid_of(nblocks, persistent, grid, pers);
+  if (pers) launch_k<true>(phase, desc, blk, nblocks, grid, threads, N, kinds, s);
+  else launch_k<false>(phase, desc, blk, nblocks, grid, threads, N, kinds, s);
+  OMEGA_HIP_CHECK(hipGetLastError());
+}
+
+template <int N, bool PERS>
+void launch_vdel4_n(const BatchMember *desc, const int *blk, int nblocks, int grid, int threads,
+                    hipStream_t s) {
+  hipLaunchKernelGGL((k_batch_vdel4_edge<N, PERS>), dim3(grid), dim3(threads), 0, s, desc, blk,
+                     nblocks);
+  hipLaunchKernelGGL((k_batch_vdel4_vc<N, PERS>), dim3(grid), dim3(threads), 0, s, desc, blk,
+                     nblocks);
+}
+} // namespace
 
 void launch_batch_diag(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
-                       int persistent, hipStream_t s) {
+                       int persistent, int kinds, hipStream_t s) {
   TimerScope ts("batch_diag", s);
-  OMEGA_BATCH_LAUNCH(k_batch_diag);
+  launch_kinds(0, desc, blk, nblocks, threads, N, persistent, kinds, s);
 }
 
 void launch_batch_vdel4(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
                         int persistent, hipStream_t s) {
   TimerScope ts("batch_vdel4", s);
-  OMEGA_BATCH_LAUNCH(k_batch_vdel4_edge);
-  OMEGA_BATCH_LAUNCH(k_batch_vdel4_vc);
+  int grid;
+  bool pers;
+  grid_of(nblocks, persistent, grid, pers);
+  if (N == 2) {
+    if (pers) launch_vdel4_n<2, true>(desc, blk, nblocks, grid, threads, s);
+    else launch_vdel4_n<2, false>(desc, blk, nblocks, grid, threads, s);
+  } else {
+    if (pers) launch_vdel4_n<1, true>(desc, blk, nblocks, grid, threads, s);
+    else launch_vdel4_n<1, false>(desc, blk, nblocks, grid, threads, s);
+  }
+  OMEGA_HIP_CHECK(hipGetLastError());
 }
 
 void launch_batch_tend(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
-                       int persistent, hipStream_t s) {
+                       int persistent, int kinds, hipStream_t s) {
   TimerScope ts("batch_tend", s);
-  OMEGA_BATCH_LAUNCH(k_batch_tend);
+  launch_kinds(1, desc, blk, nblocks, threads, N, persistent, kinds, s);
 }
-
-#undef OMEGA_BATCH_LAUNCH
 
 } // namespace omega_hip
