@@ -471,6 +471,8 @@ def main():
                     help="steps of the ensemble adjoint leg (the workload's length)")
     ap.add_argument("--skip-reference-shape", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=50)
+    ap.add_argument("--adjoint-steps", type=int, default=None,
+                    help="trajectory length of the adjoint leg (default: the workload's 200)")
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--launch", default="", help="rows,lpt,swizzle,levels_per_block")
@@ -554,7 +556,7 @@ def main():
     adj = None
     if not args.skip_adjoint:
         log("[bench] adjoint workload")
-        adj = adjoint_bench(graph=not args.no_graph)
+        adj = adjoint_bench(nsteps=args.adjoint_steps, graph=not args.no_graph)
         log(f"[bench] adjoint: step ratio {adj['ratio_per_step']:.2f}, trajectory ratio "
             f"{adj['ratio_total']:.2f} ({adj['adjoint_run_s']:.3f} s)")
 
