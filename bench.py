@@ -442,15 +442,23 @@ def copy_calibration():
 
 
 def load_traffic(kernel: str, workload: str):
-    """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 counter summary, when it was measured on this workload."""
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter
+    summary, measured on this workload: at this launch shape when recorded,
+    else at another launch shape of the same workload and ordering (the
+    kernels and their data are the same; `launch` says which was measured)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             rec = json.load(f)
     except (OSError, ValueError):
         return None
-    ent = rec.get(workload, {}).get(kernel)
+    key = workload if kernel in rec.get(workload, {}) else next(
+        (k for k in rec if k.rsplit(":", 1)[0] == workload.rsplit(":", 1)[0]
+         and kernel in rec[k]), None)
+    if key is None:
+        return None
+    ent = dict(rec[key][kernel])
+    ent["launch"] = key.rsplit(":", 1)[1]
     return ent
 
 
@@ -638,7 +646,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": stage_traffic,
                      "traffic_source": "profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE, "
-                                       "WRITE_SIZE; scaled by a plain copy)"
+                                       "WRITE_SIZE; scaled by a plain copy), measured at "
+                                       f"launch {traffic['launch']}"
                      if stage_traffic else None,
                      "kernel": "one RK stage: k_diag (diagnostics) + k_tend (fused tendency + "
                                "RK4 stage update)",

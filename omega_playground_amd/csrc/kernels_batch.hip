@@ -40,13 +40,15 @@ __device__ __forceinline__ Place place(const BatchMember &B, int lb, int N) {
 }
 
 // Table entries a block handles: one entry per block, or (persistent grid,
-// PERS) a contiguous run of entries per block, the runs dealt so that each
-// XCD walks a contiguous part of the table.  PERS is a template parameter:
+// PERS) a contiguous run of entries per block.  Either way the entries are
+// dealt so that each XCD (workgroups go to the 8 XCDs round-robin) walks a
+// contiguous part of the table, which keeps a member's neighbouring cells,
+// and their halo re-reads, in one XCD's L2.  PERS is a template parameter:
 // one code path per kernel keeps the register budget of the entry body.
 template <bool PERS, class F>
 __device__ __forceinline__ void for_entries(int nblocks, F &&f) {
   if constexpr (!PERS) {
-    f((int)blockIdx.x);
+    f(logical_block(blockIdx.x, gridDim.x, 1));
   } else {
     const int G = gridDim.x;
     const int lb = logical_block(blockIdx.x, G, 1);
