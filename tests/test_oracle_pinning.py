@@ -370,3 +370,39 @@ def test_bottom_drag_uniform_flow_decay():
     U = U0 / (1 + cd * U0 * n * dt / H)
     assert np.max(np.abs(st[1][:, L - 1] - vn[:, L - 1] * (U / U0))) <= 1e-9 * U0
     assert np.max(np.abs(st[1][:, :L - 1] - vn[:, :L - 1])) < 1e-13
+
+
+# --- per-field entry points (shallow_water_model.cpp:43-165) -----------------
+def test_field_tendencies_are_the_fused_tendency_blocks():
+    """compute_h/vn/tr_tendency after the aux pass give the three blocks of
+    compute_tendency bit for bit (the reference's compute_tendency is those
+    three calls after compute_auxiliary_variables), in replace and in
+    increment mode; a field tendency that skipped a term or read the wrong
+    auxiliary array would differ."""
+    L, ntr = 4, 2
+    m = po.Mesh(10, 8, 1e5, L)
+    model = po.Model(m, f0=1e-4, grav=9.81, visc_del2=3.0, visc_del4=1e9, eddy_diff2=2.0,
+                     eddy_diff4=1e8, ntracers=ntr)
+    rng = np.random.default_rng(7)
+    h = 100.0 + rng.random((m.ncells, L))
+    vn = rng.standard_normal((m.nedges, L))
+    tr = rng.random((ntr, m.ncells, L))
+    fused = tendency(model, (h, vn, tr))
+    model.aux_compute(h, vn, tr)
+    parts = [np.full_like(fused[0], 3.0), np.full_like(fused[1], 3.0), np.full_like(fused[2], 3.0)]
+    model.compute_field_tendency(0, parts[0], h, vn)
+    model.compute_field_tendency(1, parts[1], h, vn)
+    model.compute_field_tendency(2, parts[2], tr, vn)
+    for a, b in zip(parts, fused):
+        assert np.array_equal(a, b)
+        assert np.max(np.abs(b)) > 0
+    # increment mode adds term by term onto what is there, as compute_tendency
+    # in increment mode does
+    inc = [np.full_like(p, 0.5) for p in parts]
+    whole = [np.full_like(p, 0.5) for p in parts]
+    model.compute_tendency(whole, (h, vn, tr), 0.0, increment=True)
+    model.aux_compute(h, vn, tr)
+    for f, a in enumerate((h, h, tr)):
+        model.compute_field_tendency(f, inc[f], a, vn, increment=True)
+        assert np.array_equal(inc[f], whole[f])
+        assert np.allclose(inc[f], 0.5 + fused[f], rtol=0, atol=1e-14)
