@@ -336,9 +336,13 @@ int omega_model_kernel_variant(omega_model_t model, int *in_use);
  * when every coefficient table (mesh and model) is bit-identical at each
  * stencil slot, so a wavefront's two cells share one scalar read of each;
  * bit 1 set when all [rows][levels] arrays have < 2^24 rows and < 4 GiB,
- * so row addresses are 32-bit offsets from the array base.  No reference
- * counterpart (introspection for tests and tuning; results are the same
- * bits either way). */
+ * so row addresses are 32-bit offsets from the array base; bit 2 set when
+ * the tendency VJP runs its slot-uniform form (regular hexagonal
+ * numbering, an even layer count, every adjoint table bit-identical at
+ * each stencil slot; known after the model's first adjoint call).  No
+ * reference counterpart (introspection for tests and tuning; the forward
+ * results are the same bits either way, and so are the VJP's state
+ * cotangents). */
 int omega_model_kernel_flags(omega_model_t model, int *flags);
 /* Which of those forms the model may use (bits as above, default 3);
  * clearing a bit forces the per-cell or 64-bit form (tests, tuning). */

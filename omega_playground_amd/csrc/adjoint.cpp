@@ -71,7 +71,8 @@ void build_adj_layout(const DeviceMesh &dm, AdjLayout &a) {
     }
   }
   // transposed edges_on_edge: for each edge x, the edges t listing x, in
-  // increasing (t, slot) order
+  // increasing (t, slot) order (neighbouring edges' lists then start with
+  // the same rows, which a wavefront's lanes load together)
   std::vector<int> tofs((size_t)NE + 1, 0);
   for (int t = 0; t < NE; ++t)
     for (int j = 0; j < dm.noe_h[t]; ++j) ++tofs[dm.eoe_h[(size_t)t * ME2 + j] + 1];
@@ -85,11 +86,43 @@ void build_adj_layout(const DeviceMesh &dm, AdjLayout &a) {
       tw[fill[x]] = dm.w_h[(size_t)t * ME2 + j];
       ++fill[x];
     }
-  (void)NV;
   a.fixed = true;
   for (int c = 0; c < NC && a.fixed; ++c) a.fixed = dm.noc_h[c] == 6;
   for (int e = 0; e < NE && a.fixed; ++e)
     a.fixed = dm.noe_h[e] == 10 && tofs[e + 1] - tofs[e] == 10;
+  // slot-uniform form: the hexagonal numbering (cell c owns edges 3c..3c+2
+  // and vertices 2c, 2c+1) and every coefficient the VJP reads equal at each
+  // stencil slot, bit for bit, so the kernels read them once per slot
+  bool uni = a.fixed && dm.hex_ok && dm.coef_uni && ME == 6 && ME2 == 10 &&
+             dm.coc_h.size() == (size_t)NC * 6 && rows_equal(dm.inv_dv4_h, 1) &&
+             rows_equal(ecd, 6) && rows_equal(ecke, 6) && rows_equal(ecrv, 6) &&
+             rows_equal(ccth, 6);
+  for (int c = 0; c <= NC && uni; ++c)
+    uni = dm.eofs_h[c] == 3 * c && dm.vofs_h[c] == 2 * c;
+  int ufirst[6] = {0}, ukslot[6] = {0}, uvsg[6] = {0};
+  for (int c = 0; c < NC && uni; ++c)
+    for (int j = 0; j < 6 && uni; ++j) {
+      const int e = dm.eoc_h[(size_t)c * 6 + j];
+      const int first = dm.coe_h[2 * (size_t)e] == c ? 1 : 0;
+      const int other = dm.coe_h[2 * (size_t)e + first];
+      if (c == 0) {
+        ufirst[j] = first;
+        ukslot[j] = e % 3;
+      }
+      uni = first == ufirst[j] && e % 3 == ukslot[j] && other == dm.coc_h[(size_t)c * 6 + j] &&
+            dm.coe_h[2 * (size_t)e + 1 - first] == c;
+    }
+  for (int v = 0; v < NV && uni; ++v)
+    for (int j = 0; j < 3 && uni; ++j) {
+      const int e = dm.eov_h[3 * (size_t)v + j];
+      const int sg = dm.voe_h[2 * (size_t)e + 1] == v ? 1 : 0;
+      if (v < 2) uvsg[3 * v + j] = sg;
+      uni = sg == uvsg[3 * (v & 1) + j];
+    }
+  a.uni = uni;
+  std::memcpy(a.ufirst, ufirst, sizeof ufirst);
+  std::memcpy(a.ukslot, ukslot, sizeof ukslot);
+  std::memcpy(a.uvsg, uvsg, sizeof uvsg);
   a.ecd.upload(ecd);
   a.ecke.upload(ecke);
   a.ecrv.upload(ecrv);
@@ -129,6 +162,7 @@ void ensure_adjoint(omega_model_s *M) {
       }
     M->ect2.upload(e2);
     M->ect4.upload(e4);
+    M->ect_uniform = rows_equal(e2, 6) && rows_equal(e4, 6);
   }
   if (!M->a_sbar.p) {
     M->a_se.alloc(NE * L);
@@ -179,8 +213,8 @@ void tendency_vjp_ptrs(omega_model_s *M, const Real *const y[3], const Real *con
   a.c = CotIO{k[0], k[1], k[2], out[0], out[1], out[2]};
   a.nparts = M->a_nparts;
   a.epi = epi;
-  const AdjCoef ac{M->ect2.p, M->ect4.p};
-  launch_tendency_vjp(M->mesh->dm.view(), M->mesh->adj.view(), M->lc, M->coef(), ac, a,
+  const AdjCoef ac{M->ect2.p, M->ect4.p, M->ect_uniform ? 1 : 0};
+  launch_tendency_vjp(M->view(), M->mesh->adj.view(), M->lc, M->coef(), ac, a,
                       M->diag(), scratch(M), s);
 }
 

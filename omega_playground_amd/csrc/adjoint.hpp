@@ -19,6 +19,13 @@ struct AdjMeshView {
   // every cell has 6 edges, every edge 10 edges-on-edge and appears in 10
   // such lists (the regular hexagonal mesh): loops get constant trip counts
   int fixed;
+  // slot-uniform form (regular hexagonal numbering, ecd/ecke/ecrv repeating
+  // every 3 edges, ccth the same for every cell, the mesh's coefficients
+  // equal at each stencil slot; lists of 10 transposed entries per edge)
+  int uni;
+  int ufirst[6]; // cell slot j: the cell is cells_on_edge[0] of its edge
+  int ukslot[6]; // cell slot j: the edge's class (edge index mod 3)
+  int uvsg[6];   // vertex 2c + x, slot j: the vertex is vertices_on_edge[1]
 };
 
 struct AdjLayout {
@@ -28,9 +35,17 @@ struct AdjLayout {
   std::vector<int> cslot_h;   // [ne][2] slot of the edge in cells_on_edge[0..1]
   bool built = false;
   bool fixed = false;
+  bool uni = false;
+  int ufirst[6] = {0}, ukslot[6] = {0}, uvsg[6] = {0};
   AdjMeshView view() const {
-    return AdjMeshView{ecd.p, ecke.p, ecrv.p, voc.p, ccth.p, tofs.p, tedge.p, tw.p,
-                       fixed ? 1 : 0};
+    AdjMeshView v{ecd.p, ecke.p, ecrv.p, voc.p, ccth.p, tofs.p, tedge.p, tw.p,
+                  fixed ? 1 : 0, uni ? 1 : 0, {0}, {0}, {0}};
+    for (int j = 0; j < 6; ++j) {
+      v.ufirst[j] = ufirst[j];
+      v.ukslot[j] = ukslot[j];
+      v.uvsg[j] = uvsg[j];
+    }
+    return v;
   }
 };
 
@@ -40,6 +55,7 @@ void build_adj_layout(const DeviceMesh &dm, AdjLayout &a);
 struct AdjCoef {
   const Real *ect2;   // [ne][2] ct2 of the edge's slot in cells_on_edge[0..1]
   const Real *ect4;   // [ne][2] ct4 likewise
+  int uniform;        // both repeat every 3 edges, bit for bit
 };
 
 // Cotangent buffers of one prognostic state (device numbering).

@@ -807,7 +807,11 @@ int omega_model_kernel_flags(omega_model_t M, int *flags) {
     OMEGA_REQUIRE(M && flags, "null argument");
     const MeshView mv = M->view();
     const ModelCoef mc = M->coef();
-    *flags = (mv.coef_uni && mc.uniform ? 1 : 0) | (mv.off32 ? 2 : 0);
+    const AdjLayout &al = M->mesh->adj;
+    // the VJP's slot-uniform form (known once the adjoint tables exist)
+    const bool adj_uni = al.built && al.fixed && al.uni && M->ect_uniform && mv.coef_uni &&
+                         mc.uniform && mv.L % 2 == 0;
+    *flags = (mv.coef_uni && mc.uniform ? 1 : 0) | (mv.off32 ? 2 : 0) | (adj_uni ? 4 : 0);
   });
 }
 

@@ -128,6 +128,66 @@ __device__ __forceinline__ double mask(int k, int i, int ml, double c) {
   return (k + i < ml) ? c : 0.0;
 }
 
+// Entities a block row visits: the owned edges (vertices) of the block's
+// cells, strided over the rows, so a wavefront's rows are consecutive edges
+// (vertices) that share most of their neighbours' rows.  In the
+// slot-uniform form U (regular hexagonal numbering: cell c owns edges
+// 3c + K and vertices 2c + x) the callee also gets the edge's class K
+// (vertex's x), which selects among per-class coefficients read once per
+// wavefront.
+template <int U, class F>
+__device__ __forceinline__ void for_edges(const MeshView &m, const Seg &g, F &&f) {
+  const int e0 = U ? 3 * g.c0 : m.eofs[g.c0], e1 = U ? 3 * g.c1 : m.eofs[g.c1];
+  for (int e = e0 + (int)threadIdx.y; e < e1; e += blockDim.y) f(e, U ? e % 3 : 0);
+}
+
+template <int U, class F>
+__device__ __forceinline__ void for_vertices(const MeshView &m, const Seg &g, F &&f) {
+  const int v0 = U ? 2 * g.c0 : m.vofs[g.c0], v1 = U ? 2 * g.c1 : m.vofs[g.c1];
+  for (int v = v0 + (int)threadIdx.y; v < v1; v += blockDim.y) f(v, U ? v & 1 : 0);
+}
+
+// Read-only table entry at a wavefront-uniform index, through the constant
+// address space: the tables are never written while a kernel runs, and such
+// loads go through the scalar cache even after the kernel's first stores.
+template <class T> __device__ __forceinline__ T uld(const T *p, int i) {
+  return ((const __attribute__((address_space(4))) T *)p)[i];
+}
+// per-edge coefficient (one value for the whole mesh under U)
+template <int U, class T> __device__ __forceinline__ T ecoef(const T *t, int e) {
+  if constexpr (U) return uld(t, 0);
+  else return t[e];
+}
+// [ne][2] coefficient at slot s of edge e (repeats every 3 edges under U:
+// the three classes' values are read once and the lane's one picked)
+template <int U> __device__ __forceinline__ double eslot(const double *t, int e, int K, int s) {
+  if constexpr (U) {
+    const double a = uld(t, s), b = uld(t, 2 + s), c = uld(t, 4 + s);
+    return K == 0 ? a : (K == 1 ? b : c);
+  } else {
+    return t[2 * e + s];
+  }
+}
+// [rows][n] coefficient at slot j of row r (the same for every row under U)
+template <int U> __device__ __forceinline__ double rslot(const double *t, int r, int n, int j) {
+  if constexpr (U) return uld(t, j);
+  else return t[(size_t)r * n + j];
+}
+// two consecutive ints (cells_on_edge, vertices_on_edge) in one 8-byte load
+__device__ __forceinline__ int2 pair_at(const int *t, int e) {
+  return *reinterpret_cast<const int2 *>(t + 2 * (size_t)e);
+}
+// n ints of a row starting at an 8-byte aligned offset, in 8-byte loads
+template <int n> __device__ __forceinline__ void row_ints(const int *t, size_t off, int *out) {
+  const int2 *p = reinterpret_cast<const int2 *>(t + off);
+#pragma unroll
+  for (int i = 0; i < n / 2; ++i) {
+    const int2 v = p[i];
+    out[2 * i] = v.x;
+    out[2 * i + 1] = v.y;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Momentum del4 chain: cotangents of the del2 divergence / vorticity, then
 // of the del2 edge field (vel_hyperdiff_edge.hpp:140-237 transposed)
@@ -193,17 +253,18 @@ __global__ void __launch_bounds__(kAB)
 // Per edge at the linearisation state: the PV sum S_e = Q_e + P_e (PV term
 // only) and the thickness flux F_e u_e, read by the PV pass for every edge
 // whose edges-on-edge list holds e, by the edge pass and by the cell pass.
-template <int N, int H>
+template <int N, int H, int U>
 __global__ void __launch_bounds__(kAB)
     k_adj_pre(MeshView m, ModelCoef mc, AdjArgs a, DiagOut d, AdjScratch s, int swz) {
   const Seg g = segment<N>(m, swz);
   if (!g.active) return;
   const int L = m.L, k = g.k;
   const bool pv = a.do_vn && (a.terms & T_PV);
-  for (int e = m.eofs[g.c0] + (int)threadIdx.y; e < m.eofs[g.c1]; e += blockDim.y) {
+  for_edges<U>(m, g, [&](int e, int K) {
     if (pv) {
-      const int v0 = m.voe[2 * e], v1 = m.voe[2 * e + 1];
-      const double fv0 = mc.f_v[v0], fv1 = mc.f_v[v1];
+      const int2 vv = pair_at(m.voe, e);
+      const int v0 = vv.x, v1 = vv.y;
+      const double fv0 = ecoef<U>(mc.f_v, v0), fv1 = ecoef<U>(mc.f_v, v1);
       const Vd<N> i0 = rd<N>(d.inv_thick_v, v0, L, k), i1 = rd<N>(d.inv_thick_v, v1, L, k);
       const Vd<N> z0 = rd<N>(d.rvort_v, v0, L, k), z1 = rd<N>(d.rvort_v, v1, L, k);
       Vd<N> Se;
@@ -212,18 +273,21 @@ __global__ void __launch_bounds__(kAB)
         Se[i] = 0.5 * (i0[i] * z0[i] + i1[i] * z1[i]) + 0.5 * (i0[i] * fv0 + i1[i] * fv1);
       st<N>(s.se_e + (size_t)e * L + k, Se);
     }
-    const Vd<N> h0 = rd<N>(a.h, m.coe[2 * e], L, k), h1 = rd<N>(a.h, m.coe[2 * e + 1], L, k);
+    const int2 cc = pair_at(m.coe, e);
+    const Vd<N> h0 = rd<N>(a.h, cc.x, L, k), h1 = rd<N>(a.h, cc.y, L, k);
     const Vd<N> ue = rd<N>(a.vn, e, L, k);
     Vd<N> FU;
 #pragma unroll
     for (int i = 0; i < N; ++i) FU[i] = 0.5 * (h0[i] + h1[i]) * ue[i];
     st<N>(s.fu_e + (size_t)e * L + k, FU);
-  }
+  });
 }
 
 // PV-sum cotangent of each edge, and the two transposed sums the edge pass
 // needs: A_e = sum_t w_te kbar_t S_t, B_e = sum_t w_te kbar_t over the edges
 // t whose edges-on-edge list holds e (kbar = velocity-tendency cotangent).
+// This pass keeps its per-edge tables on every mesh: a slot-uniform form
+// (weights and lists read per class) measured slower on MI355X.
 template <int N, int H>
 __global__ void __launch_bounds__(kAB)
     k_adj_pv(MeshView m, AdjMeshView am, ModelCoef mc, AdjArgs a, DiagOut d, AdjScratch s,
@@ -270,9 +334,10 @@ __global__ void __launch_bounds__(kAB)
 // ---------------------------------------------------------------------------
 // Vertex and cell diagnostics cotangents
 // ---------------------------------------------------------------------------
-template <int N, int H>
+template <int N, int H, int U>
 __global__ void __launch_bounds__(kAB)
-    k_adj_vc(MeshView m, ModelCoef mc, AdjCoef ac, AdjArgs a, DiagOut d, AdjScratch s, int swz) {
+    k_adj_vc(MeshView m, AdjMeshView am, ModelCoef mc, AdjCoef ac, AdjArgs a, DiagOut d,
+             AdjScratch s, int swz) {
   const Seg g = segment<N>(m, swz);
   const int L = m.L, k = g.k, NC = m.nc;
   const bool pv = a.do_vn && (a.terms & T_PV), vd2 = a.do_vn && (a.terms & T_VDEL2),
@@ -282,12 +347,13 @@ __global__ void __launch_bounds__(kAB)
   // zs formed beside the del2 cotangents (no edge rows re-read)
   double p_v2 = 0.0;
   if (g.active && (pv || vd2 || vd4)) {
-    for (int v = m.vofs[g.c0] + (int)threadIdx.y; v < m.vofs[g.c1]; v += blockDim.y) {
+    for_vertices<U>(m, g, [&](int v, int x) {
       Vd<N> qb = zero<N>(), zb = zero<N>(), zs = zero<N>();
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         const int e = m.eov[3 * v + j];
-        const double sg = (m.voe[2 * e + 1] == v) ? -1.0 : 1.0;
+        const double sg =
+            (U ? (x ? am.uvsg[3 + j] : am.uvsg[j]) != 0 : m.voe[2 * e + 1] == v) ? -1.0 : 1.0;
         if (pv) {
           const Vd<N> sb = rd<N>(s.sbar_e, e, L, k);
 #pragma unroll
@@ -295,17 +361,20 @@ __global__ void __launch_bounds__(kAB)
         }
         if (vd2) {
           const Vd<N> ut = rd<N>(a.c.vn, e, L, k);
-          const int ml = m.maxlev_e[e];
+          const int ml = ecoef<U>(m.maxlev_e, e);
+          const double cv2 = ecoef<U>(mc.cv2, e), s2 = ecoef<U>(m.s2, e),
+                       idv = ecoef<U>(m.inv_dv, e);
 #pragma unroll
           for (int i = 0; i < N; ++i) {
-            zb[i] += sg * mask<N>(k, i, ml, mc.cv2[e]) * ut[i] * m.inv_dv[e];
-            zs[i] += sg * mask<N>(k, i, ml, m.s2[e]) * ut[i] * m.inv_dv[e];
+            zb[i] += sg * mask<N>(k, i, ml, cv2) * ut[i] * idv;
+            zs[i] += sg * mask<N>(k, i, ml, s2) * ut[i] * idv;
           }
         }
         if (vd4) {
           const Vd<N> eb = rd<N>(s.e2bar_e, e, L, k);
+          const double idv4 = ecoef<U>(m.inv_dv4, e);
 #pragma unroll
-          for (int i = 0; i < N; ++i) zb[i] += sg * eb[i] * m.inv_dv4[e];
+          for (int i = 0; i < N; ++i) zb[i] += sg * eb[i] * idv4;
         }
       }
       Vd<N> tb = zero<N>(), z = zero<N>();
@@ -316,7 +385,7 @@ __global__ void __launch_bounds__(kAB)
       }
       if (pv) {
         const Vd<N> io = rd<N>(d.inv_thick_v, v, L, k);
-        const double f = mc.f_v[v];
+        const double f = ecoef<U>(mc.f_v, v);
 #pragma unroll
         for (int i = 0; i < N; ++i) {
           zb[i] += io[i] * qb[i];
@@ -325,32 +394,36 @@ __global__ void __launch_bounds__(kAB)
       }
       st<N>(s.zbar_v + (size_t)v * L + k, zb);
       st<N>(s.tbar_v + (size_t)v * L + k, tb);
-    }
+    });
   }
   const bool td4 = (a.terms & T_TDEL4) && a.ntr > 0;
   for (int c = g.c0 + (int)threadIdx.y; g.active && c < g.c1; c += blockDim.y) {
     Vd<N> kb = zero<N>(), db = zero<N>(), ds = zero<N>();
     const int n = H ? 6 : m.noc[c];
+    int ec[6];
+    if constexpr (U) row_ints<6>(m.eoc, (size_t)c * 6, ec);
 #pragma unroll
     for (int j = 0; j < n; ++j) {
-      const int e = m.eoc[c * m.me + j];
-      const bool first = m.coe[2 * e] == c;
+      const int e = U ? ec[j] : m.eoc[c * m.me + j];
+      const bool first = U ? am.ufirst[j] != 0 : m.coe[2 * e] == c;
+      const double idc = ecoef<U>(m.inv_dc, e);
       if (ke || vd2) {
         const Vd<N> ut = rd<N>(a.c.vn, e, L, k);
-        const int ml = m.maxlev_e[e];
+        const int ml = ecoef<U>(m.maxlev_e, e);
+        const double cv2 = ecoef<U>(mc.cv2, e), s2 = ecoef<U>(m.s2, e);
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-          if (ke) kb[i] += (first ? 1.0 : -1.0) * ut[i] * m.inv_dc[e];
+          if (ke) kb[i] += (first ? 1.0 : -1.0) * ut[i] * idc;
           if (vd2) {
-            db[i] += (first ? -1.0 : 1.0) * mask<N>(k, i, ml, mc.cv2[e]) * ut[i] * m.inv_dc[e];
-            ds[i] += (first ? -1.0 : 1.0) * mask<N>(k, i, ml, m.s2[e]) * ut[i] * m.inv_dc[e];
+            db[i] += (first ? -1.0 : 1.0) * mask<N>(k, i, ml, cv2) * ut[i] * idc;
+            ds[i] += (first ? -1.0 : 1.0) * mask<N>(k, i, ml, s2) * ut[i] * idc;
           }
         }
       }
       if (vd4) {
         const Vd<N> eb = rd<N>(s.e2bar_e, e, L, k);
 #pragma unroll
-        for (int i = 0; i < N; ++i) db[i] += (first ? -1.0 : 1.0) * eb[i] * m.inv_dc[e];
+        for (int i = 0; i < N; ++i) db[i] += (first ? -1.0 : 1.0) * eb[i] * idc;
       }
     }
     if (ke) st<N>(s.kbar_c + (size_t)c * L + k, kb);
@@ -367,11 +440,22 @@ __global__ void __launch_bounds__(kAB)
         Vd<N> gb = zero<N>();
 #pragma unroll
         for (int j = 0; j < n; ++j) {
-          const int e = m.eoc[c * m.me + j];
-          const int a0 = m.coe[2 * e], a1 = m.coe[2 * e + 1];
-          const bool second = a1 == c;
+          const int e = U ? ec[j] : m.eoc[c * m.me + j];
+          bool second;
+          int o;
+          if constexpr (U) {
+            second = am.ufirst[j] == 0;
+            o = m.coc[c * 6 + j];
+          } else {
+            const int a0 = m.coe[2 * e], a1 = m.coe[2 * e + 1];
+            second = a1 == c;
+            o = second ? a0 : a1;
+          }
           const double sg = second ? -1.0 : 1.0;
-          const Vd<N> to = rd<N>(tl, second ? a0 : a1, L, k);
+          const int Kj = U ? am.ukslot[j] : 0;
+          const double c40 = eslot<U>(ac.ect4, e, Kj, 0), c41 = eslot<U>(ac.ect4, e, Kj, 1);
+          const double idc = ecoef<U>(m.inv_dc, e);
+          const Vd<N> to = rd<N>(tl, o, L, k);
           Vd<N> t0, t1;
 #pragma unroll
           for (int i = 0; i < N; ++i) {
@@ -379,8 +463,7 @@ __global__ void __launch_bounds__(kAB)
             t1[i] = second ? tc[i] : to[i];
           }
 #pragma unroll
-          for (int i = 0; i < N; ++i)
-            gb[i] += sg * (ac.ect4[2 * e] * t0[i] + ac.ect4[2 * e + 1] * t1[i]) * m.inv_dc[e];
+          for (int i = 0; i < N; ++i) gb[i] += sg * (c40 * t0[i] + c41 * t1[i]) * idc;
         }
         st<N>(s.gbar_c + ((size_t)l * NC + c) * L + k, gb);
       }
@@ -393,7 +476,7 @@ __global__ void __launch_bounds__(kAB)
 // ---------------------------------------------------------------------------
 // Edge pass: velocity cotangent (accumulated) and edge-thickness cotangent
 // ---------------------------------------------------------------------------
-template <int N, int H, int X>
+template <int N, int H, int X, int U>
 // four waves per SIMD (at most 128 VGPRs; fits without spilling)
 __global__ void __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4)))
     k_adj_edge(MeshView m, AdjMeshView am, ModelCoef mc, AdjCoef ac, AdjArgs a, DiagOut d,
@@ -405,9 +488,11 @@ __global__ void __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4)))
   const bool td2 = X && (a.terms & T_TDEL2), td4 = X && (a.terms & T_TDEL4);
   double p_v4 = 0.0;
   if (g.active) {
-    for (int e = m.eofs[g.c0] + (int)threadIdx.y; e < m.eofs[g.c1]; e += blockDim.y) {
-      const int c0 = m.coe[2 * e], c1 = m.coe[2 * e + 1], v0 = m.voe[2 * e], v1 = m.voe[2 * e + 1];
-      const double cd0 = am.ecd[2 * e], cd1 = am.ecd[2 * e + 1], idc = m.inv_dc[e];
+    for_edges<U>(m, g, [&](int e, int K) {
+      const int2 cc = pair_at(m.coe, e), vv = pair_at(m.voe, e);
+      const int c0 = cc.x, c1 = cc.y, v0 = vv.x, v1 = vv.y;
+      const double cd0 = eslot<U>(am.ecd, e, K, 0), cd1 = eslot<U>(am.ecd, e, K, 1),
+                   idc = ecoef<U>(m.inv_dc, e);
       const Vd<N> ue = rd<N>(a.vn, e, L, k);
       const Vd<N> h0 = rd<N>(a.h, c0, L, k), h1 = rd<N>(a.h, c1, L, k);
       Vd<N> F, ih0, ih1, ub = zero<N>(), fb = zero<N>();
@@ -446,7 +531,9 @@ __global__ void __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4)))
           const double w = cd0 * b0[i] + cd1 * b1[i];
           ub[i] -= w * F[i] * ne;
           fb[i] -= w * ue[i] * ne;
-          if (td2) fb[i] += (ac.ect2[2 * e] * b0[i] + ac.ect2[2 * e + 1] * b1[i]) * m.s2[e] * grad;
+          if (td2)
+            fb[i] += (eslot<U>(ac.ect2, e, K, 0) * b0[i] + eslot<U>(ac.ect2, e, K, 1) * b1[i]) *
+                     ecoef<U>(m.s2, e) * grad;
           if (td4) fb[i] += (cd0 * g0[i] + cd1 * g1[i]) * grad;
         }
         if (!mix) {
@@ -475,14 +562,15 @@ __global__ void __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4)))
         }
         if (pv || vd2 || vd4) {
           const Vd<N> zb0 = rd<N>(s.zbar_v, v0, L, k), zb1 = rd<N>(s.zbar_v, v1, L, k);
+          const double r0 = eslot<U>(am.ecrv, e, K, 0), r1 = eslot<U>(am.ecrv, e, K, 1);
 #pragma unroll
-          for (int i = 0; i < N; ++i) ub[i] += am.ecrv[2 * e] * zb0[i] + am.ecrv[2 * e + 1] * zb1[i];
+          for (int i = 0; i < N; ++i) ub[i] += r0 * zb0[i] + r1 * zb1[i];
         }
         if (ke) {
           const Vd<N> k0 = rd<N>(s.kbar_c, c0, L, k), k1 = rd<N>(s.kbar_c, c1, L, k);
+          const double q0 = eslot<U>(am.ecke, e, K, 0), q1 = eslot<U>(am.ecke, e, K, 1);
 #pragma unroll
-          for (int i = 0; i < N; ++i)
-            ub[i] += (am.ecke[2 * e] * k0[i] + am.ecke[2 * e + 1] * k1[i]) * ue[i] * 0.5;
+          for (int i = 0; i < N; ++i) ub[i] += (q0 * k0[i] + q1 * k1[i]) * ue[i] * 0.5;
         }
         if (vd2 || vd4) {
           const Vd<N> d0 = rd<N>(s.dbar_c, c0, L, k), d1 = rd<N>(s.dbar_c, c1, L, k);
@@ -491,18 +579,18 @@ __global__ void __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4)))
         }
         if (vd4) {
           const Vd<N> ut = rd<N>(a.c.vn, e, L, k);
-          const int ml = m.maxlev_e[e];
+          const int ml = ecoef<U>(m.maxlev_e, e);
+          const double s4 = ecoef<U>(m.s4, e), idv = ecoef<U>(m.inv_dv, e);
           const Vd<N> D0 = rd<N>(d.vdel2_div_c, c0, L, k), D1 = rd<N>(d.vdel2_div_c, c1, L, k);
           const Vd<N> Z0 = rd<N>(d.vdel2_rv_v, v0, L, k), Z1 = rd<N>(d.vdel2_rv_v, v1, L, k);
 #pragma unroll
           for (int i = 0; i < N; ++i)
-            p_v4 -= ut[i] * mask<N>(k, i, ml, m.s4[e]) *
-                    ((D1[i] - D0[i]) * idc - (Z1[i] - Z0[i]) * m.inv_dv[e]);
+            p_v4 -= ut[i] * mask<N>(k, i, ml, s4) * ((D1[i] - D0[i]) * idc - (Z1[i] - Z0[i]) * idv);
         }
       }
       adj_out<N>(a, 1, a.c.ovn, (size_t)e * L + k, ub);
       st<N>(s.fbar_e + (size_t)e * L + k, fb);
-    }
+    });
   }
   const int b = blockIdx.y * gridDim.x + blockIdx.x;
   block_partial(p_v4, s.partial + a.nparts + b);
@@ -511,7 +599,7 @@ __global__ void __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4)))
 // ---------------------------------------------------------------------------
 // Cell pass: thickness and tracer cotangents (accumulated)
 // ---------------------------------------------------------------------------
-template <int N, int H, int X>
+template <int N, int H, int X, int U>
 __global__ void __launch_bounds__(kAB)
     k_adj_cell(MeshView m, AdjMeshView am, ModelCoef mc, AdjCoef ac, AdjArgs a, DiagOut d,
                AdjScratch s, int swz) {
@@ -525,23 +613,32 @@ __global__ void __launch_bounds__(kAB)
       const int n = H ? 6 : m.noc[c];
       const Vd<N> hc = rd<N>(a.h, c, L, k);
       Vd<N> hb = zero<N>();
+      // U: the cell's six edges and vertices in 8-byte loads; per-slot
+      // coefficients are the same for every cell
+      int ec[6], vc[6];
+      if constexpr (U) {
+        row_ints<6>(m.eoc, (size_t)c * 6, ec);
+        if (pv) row_ints<6>(am.voc, (size_t)c * 6, vc);
+      }
 #pragma unroll
       for (int j = 0; j < n; ++j) {
-        const int e = m.eoc[c * me + j];
+        const int e = U ? ec[j] : m.eoc[c * me + j];
         const Vd<N> fb = rd<N>(s.fbar_e, e, L, k);
 #pragma unroll
         for (int i = 0; i < N; ++i) hb[i] += 0.5 * fb[i];
         if (ssh) {
-          const double sg = (m.coe[2 * e] == c) ? 1.0 : -1.0;
+          const double sg = (U ? am.ufirst[j] != 0 : m.coe[2 * e] == c) ? 1.0 : -1.0;
+          const double idc = ecoef<U>(m.inv_dc, e);
           const Vd<N> ut = rd<N>(a.c.vn, e, L, k);
 #pragma unroll
-          for (int i = 0; i < N; ++i) hb[i] += mc.grav * sg * ut[i] * m.inv_dc[e];
+          for (int i = 0; i < N; ++i) hb[i] += mc.grav * sg * ut[i] * idc;
         }
         if (pv) {
-          const int v = am.voc[c * me + j];
+          const int v = U ? vc[j] : am.voc[c * me + j];
+          const double th = rslot<U>(am.ccth, c, me, j);
           const Vd<N> tb = rd<N>(s.tbar_v, v, L, k);
 #pragma unroll
-          for (int i = 0; i < N; ++i) hb[i] += am.ccth[c * me + j] * tb[i];
+          for (int i = 0; i < N; ++i) hb[i] += th * tb[i];
         }
       }
       // the cell's own rows are loaded once; each edge then loads only the
@@ -564,7 +661,7 @@ __global__ void __launch_bounds__(kAB)
         Vd<N> nb = zero<N>(), t2 = zero<N>(), t4 = zero<N>();
 #pragma unroll
         for (int j = 0; j < n; ++j) {
-          const int e = m.eoc[c * me + j];
+          const int e = U ? ec[j] : m.eoc[c * me + j];
           if (!mix) {
             // advection only: the edge pass stored this edge's share
             const Vd<N> xa = ld<N>(s.xa_e + ((size_t)l * m.ne + e) * L + k);
@@ -572,11 +669,22 @@ __global__ void __launch_bounds__(kAB)
             for (int i = 0; i < N; ++i) nb[i] += xa[i];
             continue;
           }
-          const int a0 = m.coe[2 * e], a1 = m.coe[2 * e + 1];
-          const bool second = a1 == c;
-          const int o = second ? a0 : a1;
+          bool second;
+          int o;
+          if constexpr (U) {
+            second = am.ufirst[j] == 0;
+            o = m.coc[c * 6 + j];
+          } else {
+            const int a0 = m.coe[2 * e], a1 = m.coe[2 * e + 1];
+            second = a1 == c;
+            o = second ? a0 : a1;
+          }
           const double sg = second ? 1.0 : -1.0;
-          const double cd0 = am.ecd[2 * e], cd1 = am.ecd[2 * e + 1], idc = m.inv_dc[e];
+          const int Kj = U ? am.ukslot[j] : 0;
+          const double cd0 = eslot<U>(am.ecd, e, Kj, 0), cd1 = eslot<U>(am.ecd, e, Kj, 1),
+                       idc = ecoef<U>(m.inv_dc, e);
+          const double c20 = eslot<U>(ac.ect2, e, Kj, 0), c21 = eslot<U>(ac.ect2, e, Kj, 1);
+          const double s2 = ecoef<U>(m.s2, e), s4 = ecoef<U>(m.s4, e);
           const Vd<N> fu = rd<N>(s.fu_e, e, L, k);
           const Vd<N> bo = rd<N>(a.c.tr + off, o, L, k);
           Vd<N> b0, b1, h0 = zero<N>(), h1 = zero<N>(), tt0 = zero<N>(), tt1 = zero<N>(),
@@ -616,7 +724,7 @@ __global__ void __launch_bounds__(kAB)
               G1[i] = second ? Gcl[i] : Go[i];
             }
           }
-          const double cdc = m.cdiv[c * me + j];
+          const double cdc = rslot<U>(m.cdiv, c, me, j);
 #pragma unroll
           for (int i = 0; i < N; ++i) {
             const double F = 0.5 * (h0[i] + h1[i]);
@@ -624,14 +732,14 @@ __global__ void __launch_bounds__(kAB)
             nb[i] += 0.5 * (-w * fu[i]);
             if (td2) {
               const double grad = (tt1[i] * ih1[i] - tt0[i] * ih0[i]) * idc;
-              const double w2 = ac.ect2[2 * e] * b0[i] + ac.ect2[2 * e + 1] * b1[i];
-              nb[i] += sg * w2 * F * m.s2[e] * idc;
-              t2[i] += cdc * F * m.s2[e] * grad;
+              const double w2 = c20 * b0[i] + c21 * b1[i];
+              nb[i] += sg * w2 * F * s2 * idc;
+              t2[i] += cdc * F * s2 * grad;
             }
             if (td4) {
               const double wg = cd0 * g0[i] + cd1 * g1[i];
               nb[i] += sg * wg * F * idc;
-              t4[i] -= cdc * m.s4[e] * (G1[i] - G0[i]) * idc;
+              t4[i] -= cdc * s4 * (G1[i] - G0[i]) * idc;
             }
           }
         }
@@ -800,8 +908,23 @@ void launch_tendency_vjp(const MeshView &m, const AdjMeshView &am, const Launch 
   const int swz = lc.swizzle;
   const bool vd4 = a.do_vn && (a.terms & T_VDEL4);
   const bool pv = a.do_vn && (a.terms & T_PV);
-  // regular hexagonal meshes: constant neighbour counts, unrolled loops
+  // regular hexagonal meshes: constant neighbour counts, unrolled loops;
+  // with slot-uniform tables and coefficients also the per-slot reads (U)
+  const bool U = g.N == 2 && am.fixed && am.uni && m.coef_uni && mc.uniform && ac.uniform;
 #define OMEGA_ADJ_LAUNCH(K, ...)                                                                \
+  do {                                                                                         \
+    if (U)                                                                                     \
+      hipLaunchKernelGGL((K<2, 1, 1>), g.grid, g.block, 0, st, __VA_ARGS__);                    \
+    else if (g.N == 2 && am.fixed)                                                             \
+      hipLaunchKernelGGL((K<2, 1, 0>), g.grid, g.block, 0, st, __VA_ARGS__);                    \
+    else if (g.N == 2)                                                                         \
+      hipLaunchKernelGGL((K<2, 0, 0>), g.grid, g.block, 0, st, __VA_ARGS__);                    \
+    else if (am.fixed)                                                                         \
+      hipLaunchKernelGGL((K<1, 1, 0>), g.grid, g.block, 0, st, __VA_ARGS__);                    \
+    else                                                                                       \
+      hipLaunchKernelGGL((K<1, 0, 0>), g.grid, g.block, 0, st, __VA_ARGS__);                    \
+  } while (0)
+#define OMEGA_ADJ_LAUNCH2(K, ...)                                                               \
   do {                                                                                         \
     if (g.N == 2 && am.fixed)                                                                  \
       hipLaunchKernelGGL((K<2, 1>), g.grid, g.block, 0, st, __VA_ARGS__);                       \
@@ -813,23 +936,25 @@ void launch_tendency_vjp(const MeshView &m, const AdjMeshView &am, const Launch 
       hipLaunchKernelGGL((K<1, 0>), g.grid, g.block, 0, st, __VA_ARGS__);                       \
   } while (0)
   if (vd4) {
-    OMEGA_ADJ_LAUNCH(k_adj_v4a, m, mc, a, s, swz);
-    OMEGA_ADJ_LAUNCH(k_adj_v4b, m, am, s, swz);
+    OMEGA_ADJ_LAUNCH2(k_adj_v4a, m, mc, a, s, swz);
+    OMEGA_ADJ_LAUNCH2(k_adj_v4b, m, am, s, swz);
   }
   // edge PV sums (PV term) and thickness fluxes (PV term, tracer advection)
   if (pv || a.ntr > 0) OMEGA_ADJ_LAUNCH(k_adj_pre, m, mc, a, d, s, swz);
-  if (pv) OMEGA_ADJ_LAUNCH(k_adj_pv, m, am, mc, a, d, s, swz);
-  OMEGA_ADJ_LAUNCH(k_adj_vc, m, mc, ac, a, d, s, swz);
+  if (pv) OMEGA_ADJ_LAUNCH2(k_adj_pv, m, am, mc, a, d, s, swz);
+  OMEGA_ADJ_LAUNCH(k_adj_vc, m, am, mc, ac, a, d, s, swz);
 #define OMEGA_ADJ_LAUNCH_X(K, X, ...)                                                           \
   do {                                                                                         \
-    if (g.N == 2 && am.fixed)                                                                  \
-      hipLaunchKernelGGL((K<2, 1, X>), g.grid, g.block, 0, st, __VA_ARGS__);                    \
+    if (U)                                                                                     \
+      hipLaunchKernelGGL((K<2, 1, X, 1>), g.grid, g.block, 0, st, __VA_ARGS__);                 \
+    else if (g.N == 2 && am.fixed)                                                             \
+      hipLaunchKernelGGL((K<2, 1, X, 0>), g.grid, g.block, 0, st, __VA_ARGS__);                 \
     else if (g.N == 2)                                                                         \
-      hipLaunchKernelGGL((K<2, 0, X>), g.grid, g.block, 0, st, __VA_ARGS__);                    \
+      hipLaunchKernelGGL((K<2, 0, X, 0>), g.grid, g.block, 0, st, __VA_ARGS__);                 \
     else if (am.fixed)                                                                         \
-      hipLaunchKernelGGL((K<1, 1, X>), g.grid, g.block, 0, st, __VA_ARGS__);                    \
+      hipLaunchKernelGGL((K<1, 1, X, 0>), g.grid, g.block, 0, st, __VA_ARGS__);                 \
     else                                                                                       \
-      hipLaunchKernelGGL((K<1, 0, X>), g.grid, g.block, 0, st, __VA_ARGS__);                    \
+      hipLaunchKernelGGL((K<1, 0, X, 0>), g.grid, g.block, 0, st, __VA_ARGS__);                 \
   } while (0)
   // X = 0: no tracer diffusion term (edge and cell passes without its code)
   if (a.terms & (T_TDEL2 | T_TDEL4)) {
@@ -840,6 +965,7 @@ void launch_tendency_vjp(const MeshView &m, const AdjMeshView &am, const Launch 
     OMEGA_ADJ_LAUNCH_X(k_adj_cell, 0, m, am, mc, ac, a, d, s, swz);
   }
 #undef OMEGA_ADJ_LAUNCH_X
+#undef OMEGA_ADJ_LAUNCH2
 #undef OMEGA_ADJ_LAUNCH
   hipLaunchKernelGGL(k_adj_param_reduce, dim3(4), dim3(kAB), 0, st, s.partial, a.nparts,
                      (int)(g.grid.x * g.grid.y), a.epi.vs, s.pgrad);

@@ -560,7 +560,8 @@ class ShallowWaterModel:
 
     def kernel_flags(self) -> int:
         """bit 0: coefficient tables uniform per stencil slot (shared by a
-        wavefront's two cells); bit 1: 32-bit row offsets."""
+        wavefront's two cells); bit 1: 32-bit row offsets; bit 2: the
+        tendency VJP's slot-uniform form (after the first adjoint call)."""
         v = C.c_int()
         check(lib().omega_model_kernel_flags(self._h, C.byref(v)))
         return v.value
