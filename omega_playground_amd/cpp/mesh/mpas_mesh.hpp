@@ -172,6 +172,28 @@ struct MPASMesh {
     return m_handle;
   }
 
+  // The arrays additional_tendency() receives are in the library's device
+  // numbering, not in the numbering of the host fields above (m_x_cell...).
+  // A HIP kernel launched from the hook indexes these device copies of the
+  // geometry, which are in device numbering ("x_cell", "y_cell", "x_edge",
+  // "y_edge", "angle_edge", "x_vertex", "y_vertex", "area_cell"; no
+  // reference counterpart: the reference's views share one numbering).
+  const Real *device_geometry(const char *name) const {
+    const double *p = nullptr;
+    int64_t n = 0;
+    check(omega_mesh_device_array(handle(), name, &p, &n));
+    return p;
+  }
+
+  // original[i] = index in the host fields of device entity i (kind 0 cells,
+  // 1 edges, 2 vertices).
+  std::vector<Int> device_to_original(Int kind) const {
+    const Int n = kind == 0 ? m_ncells : kind == 1 ? m_nedges : m_nvertices;
+    std::vector<int> p((size_t)n);
+    check(omega_mesh_permutation(handle(), (int)kind, p.data()));
+    return std::vector<Int>(p.begin(), p.end());
+  }
+
 protected:
   omega_mesh_t m_handle = nullptr;
 

@@ -302,6 +302,7 @@ int omega_mesh_device_array(omega_mesh_t mesh, const char *name, const double **
   return guarded([&] {
     OMEGA_REQUIRE(mesh && name && dptr && n, "null argument");
     DeviceMesh &dm = mesh->dm;
+    dm.upload(); // the device copies are made on first use
     const std::string nm(name);
     DBuf<Real> *b = nullptr;
     if (nm == "x_cell") b = &dm.x_cell;

@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(kAB)
   for (int c = g.c0 + (int)threadIdx.y; c < g.c1; c += blockDim.y) {
     Vd<N> d2 = zero<N>();
     const int n = H ? 6 : m.noc[c];
-#pragma unroll
+#pragma unroll 6
     for (int j = 0; j < n; ++j) {
       const int e = m.eoc[c * m.me + j];
       const double sg = (m.coe[2 * e + 1] == c) ? -1.0 : 1.0;
@@ -402,7 +402,7 @@ __global__ void __launch_bounds__(kAB)
     const int n = H ? 6 : m.noc[c];
     int ec[6];
     if constexpr (U) row_ints<6>(m.eoc, (size_t)c * 6, ec);
-#pragma unroll
+#pragma unroll 6
     for (int j = 0; j < n; ++j) {
       const int e = U ? ec[j] : m.eoc[c * m.me + j];
       const bool first = U ? am.ufirst[j] != 0 : m.coe[2 * e] == c;
@@ -438,7 +438,7 @@ __global__ void __launch_bounds__(kAB)
         const double *tl = a.c.tr + (size_t)l * NC * L;
         const Vd<N> tc = rd<N>(tl, c, L, k);
         Vd<N> gb = zero<N>();
-#pragma unroll
+#pragma unroll 6
         for (int j = 0; j < n; ++j) {
           const int e = U ? ec[j] : m.eoc[c * m.me + j];
           bool second;
@@ -620,7 +620,7 @@ __global__ void __launch_bounds__(kAB)
         row_ints<6>(m.eoc, (size_t)c * 6, ec);
         if (pv) row_ints<6>(am.voc, (size_t)c * 6, vc);
       }
-#pragma unroll
+#pragma unroll 6
       for (int j = 0; j < n; ++j) {
         const int e = U ? ec[j] : m.eoc[c * me + j];
         const Vd<N> fb = rd<N>(s.fbar_e, e, L, k);
@@ -659,7 +659,7 @@ __global__ void __launch_bounds__(kAB)
           Gcl = rd<N>(d.trdel2_c + off, c, L, k);
         }
         Vd<N> nb = zero<N>(), t2 = zero<N>(), t4 = zero<N>();
-#pragma unroll
+#pragma unroll 6
         for (int j = 0; j < n; ++j) {
           const int e = U ? ec[j] : m.eoc[c * me + j];
           if (!mix) {

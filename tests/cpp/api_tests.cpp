@@ -230,6 +230,17 @@ void fields() {
     threw = true;
   }
   require(threw, "vn tendency with another thickness array is rejected");
+  // the numbering of the device arrays: a permutation of the host fields'
+  for (Int kind = 0; kind < 3; ++kind) {
+    const std::vector<Int> o = mesh->device_to_original(kind);
+    std::vector<Int> seen(o.size(), 0);
+    for (Int i : o) {
+      require(i >= 0 && i < (Int)o.size() && !seen[i], "device_to_original is a permutation");
+      seen[i] = 1;
+    }
+  }
+  require(mesh->device_geometry("x_cell") != nullptr && mesh->device_geometry("angle_edge"),
+          "device geometry available");
   std::cout << "fields: per-field device-array tendencies equal compute_tendency" << std::endl;
 }
 

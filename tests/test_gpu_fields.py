@@ -252,3 +252,22 @@ def test_lsrk_keeps_extra_state_tracers(graph):
     b.set_graph(graph)
     b.run(3)
     assert same_bits(gb.download()[2][1:], tr[1:])
+
+
+# --- the numbering hooks see: device geometry and the permutations ---------
+def test_device_geometry_is_in_device_numbering():
+    """The arrays an additional-tendency hook receives are in device
+    numbering; the mesh's device geometry (what a hook's own kernel indexes)
+    is the host geometry permuted by the same map, and is available before
+    any model or state exists."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    gm = swe.PlanarHexagonalMesh(12, 10, 10e3, 4)
+    for name, kind in (("x_cell", 0), ("y_cell", 0), ("area_cell", 0), ("x_edge", 1),
+                       ("y_edge", 1), ("angle_edge", 1), ("x_vertex", 2), ("y_vertex", 2)):
+        ptr, n = gm.device_array(name)
+        assert ptr and n == (gm.ncells, gm.nedges, gm.nvertices)[kind], name
+        out = np.empty(n)
+        assert hip.hipMemcpy(C.c_void_p(out.ctypes.data), C.c_void_p(ptr), C.c_size_t(8 * n), 2) == 0
+        host = np.asarray(getattr(gm, name), dtype=np.float64)
+        assert same_bits(out, host[gm.permutation(kind)]), name
