@@ -385,3 +385,36 @@ def test_general_mpas_mesh_masks_and_scalings_bitwise(ordering):
         assert same_bits(a, b), name
 
 
+
+
+@pytest.mark.parametrize("nx,ny,L,ntr", [(4, 4, 1, 0), (5, 4, 3, 1), (7, 6, 48, 2), (8, 6, 65, 0),
+                                         (6, 6, 100, 9), (8, 6, 128, 2), (6, 4, 192, 1)])
+def test_mesh_layer_and_tracer_counts_bitwise(nx, ny, L, ntr):
+    """Edge shapes: the smallest periodic meshes, odd mesh sizes, one layer,
+    layer counts that are not a multiple of the 64-level row (partial level
+    chunks, general kernels) and several rows of 64 (pattern kernels over
+    level chunks), no tracers and many.  Tendency, one RK4 step replayed
+    from a graph and one LSRK5 step, each bit for bit."""
+    print("case", nx, ny, L, ntr, flush=True)
+    om, omodel, gm, gmodel, state = make_pair(nx, ny, L, ntr, seed=L + ntr)
+    tend_o = omodel.new_state()
+    omodel.compute_tendency(tend_o, state, 0.0)
+    gt = swe.ShallowWaterState(gm, ntr)
+    gmodel.compute_tendency(gt, upload(gm, ntr, state), 0.0)
+    for name, a, b in zip(("h", "vn", "tr"), gt.download(), tend_o):
+        assert same_bits(a, b), name
+    dt = 20.0
+    for kind in ("rk4", "lsrk"):
+        gs = upload(gm, ntr, state)
+        if kind == "rk4":
+            gst = swe.RK4Stepper(gmodel)
+            gst.set_graph(True)
+            ost = po.Stepper(omodel, "rk4")
+        else:
+            gst = swe.LSRKStepper(gmodel, 5)
+            ost = po.Stepper(omodel, "lsrk", 5)
+        ref = tuple(a.copy() for a in state)
+        gst.run(0.0, dt, 2, gs)
+        ost.run(0.0, dt, 2, ref)
+        for name, a, b in zip(("h", "vn", "tr"), gs.download(), ref):
+            assert same_bits(a, b), (kind, name)
