@@ -858,9 +858,10 @@ __device__ __forceinline__ void union_pv2(const MeshView &m, const ModelCoef &mc
   for (int x = 0; x < hexu::kNV; ++x) {
     const int va = cld(uA, x), vb = cld(uB, x);
     const int v = hi ? vb : va;
-    const double fa = cld(mc.f_v, va), fb = cld(mc.f_v, vb);
+    // U: f is one value for the whole mesh
+    const double fa = cld(mc.f_v, U ? 0 : va), fb = cld(mc.f_v, vb);
     const double f = U ? fa : (hi ? fb : fa);
-    const Vd<2> rv = rowv<2>(d.rvort_v, v, L, k), it = rowv<2>(d.inv_thick_v, v, L, k);
+    const Vd<2> rv = rowv<2, W>(d.rvort_v, v, L, k), it = rowv<2, W>(d.inv_thick_v, v, L, k);
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       P[x][i] = it[i] * rv[i];
