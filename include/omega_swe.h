@@ -339,13 +339,15 @@ int omega_model_kernel_variant(omega_model_t model, int *in_use);
  * so row addresses are 32-bit offsets from the array base; bit 2 set when
  * the tendency VJP runs its slot-uniform form (regular hexagonal
  * numbering, an even layer count, every adjoint table bit-identical at
- * each stencil slot; known after the model's first adjoint call).  No
- * reference counterpart (introspection for tests and tuning; the forward
- * results are the same bits either way, and so are the VJP's state
- * cotangents). */
+ * each stencil slot; known after the model's first adjoint call); bit 3
+ * set when the pair tendency kernel runs a form compiled for the model's
+ * term set (7, 15, 79 or 127, both tendencies on).  No reference
+ * counterpart (introspection for tests and tuning; the forward results are
+ * the same bits either way, and so are the VJP's state cotangents). */
 int omega_model_kernel_flags(omega_model_t model, int *flags);
-/* Which of those forms the model may use (bits as above, default 3);
- * clearing a bit forces the per-cell or 64-bit form (tests, tuning). */
+/* Which of those forms the model may use (bit 0 shared coefficients, bit 1
+ * 32-bit offsets, bit 2 the compiled-in term sets; default 7); clearing a
+ * bit forces the per-cell, 64-bit or run-time-terms form (tests, tuning). */
 int omega_model_set_kernel_forms(omega_model_t model, int forms);
 /* ----------------------------------------------------------------------
  * Time steppers (src/time_integration/time_stepper.hpp:8-38)

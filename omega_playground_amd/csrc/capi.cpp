@@ -797,7 +797,7 @@ int omega_model_kernel_variant(omega_model_t M, int *in_use) {
 int omega_model_set_kernel_forms(omega_model_t M, int forms) {
   return guarded([&] {
     OMEGA_REQUIRE(M, "null model");
-    OMEGA_REQUIRE(forms >= 0 && forms <= 3, "forms must be 0..3");
+    OMEGA_REQUIRE(forms >= 0 && forms <= 7, "forms must be 0..7");
     M->forms = forms;
     M->touch();
   });
@@ -812,7 +812,13 @@ int omega_model_kernel_flags(omega_model_t M, int *flags) {
     // the VJP's slot-uniform form (known once the adjoint tables exist)
     const bool adj_uni = al.built && al.fixed && al.uni && M->ect_uniform && mv.coef_uni &&
                          mc.uniform && mv.L % 2 == 0;
-    *flags = (mv.coef_uni && mc.uniform ? 1 : 0) | (mv.off32 ? 2 : 0) | (adj_uni ? 4 : 0);
+    // the pair tendency kernel's compiled-in term set runs for this model
+    const bool tset = mv.term_sets && mv.coef_uni && mc.uniform && mv.off32 && mv.hexu_ok &&
+                      !M->linear && !general_only(M) && hex_usable(mv, M->lc) &&
+                      M->lc.lpt != 1 && mv.L % 64 == 0 && hex_term_set(M->terms) &&
+                      !M->params.disable_h_tendency && !M->params.disable_vn_tendency;
+    *flags = (mv.coef_uni && mc.uniform ? 1 : 0) | (mv.off32 ? 2 : 0) | (adj_uni ? 4 : 0) |
+             (tset ? 8 : 0);
   });
 }
 

@@ -749,16 +749,19 @@ __device__ __forceinline__ void diag_union_p(const MeshView &m, int terms, int n
 // union vertices (hex_pattern.hpp, namespace hexu) are formed once per cell
 // by the caller and shared by its three edges; the rows per edge are then
 // 10 thicknesses and 10 velocities.  Same values, same expressions.
-template <int K, bool U, bool W>
+template <int K, bool U, bool W, int TM = -1>
 __device__ __forceinline__ void edge_tend_hex2v(const MeshView &m, const ModelCoef &mc,
                                                 const StageArgs &a, const DiagOut &d, int eA,
                                                 int eB, bool hi, int k, const Vd<2> (&P)[hexu::kNV],
                                                 const Vd<2> (&Q)[hexu::kNV]) {
   constexpr int N = 2;
   const int L = m.L;
+  // TM >= 0: the term set is TM and both tendencies are on (compiled in)
+  const int terms = TM >= 0 ? TM : a.terms;
+  const bool do_vn = TM >= 0 ? true : a.do_vn != 0;
   const int e = hi ? eB : eA;
   const size_t idx = (size_t)e * L + k;
-  if (!a.do_vn) {
+  if (!do_vn) {
     if (a.epi == EPI_STORE) return;
     epilogue<N>(a, a.vn, idx, tend_base<N>(a, a.vn, idx));
     return;
@@ -777,7 +780,7 @@ __device__ __forceinline__ void edge_tend_hex2v(const MeshView &m, const ModelCo
   constexpr int o0 = hex::OWNV[K][0], o1 = hex::OWNV[K][1];
   constexpr int q0 = hex::OWNC[K][0], q1 = hex::OWNC[K][1];
   constexpr int v0 = hexu::OV[K][0], v1 = hexu::OV[K][1];
-  const bool pv = a.terms & T_PV;
+  const bool pv = terms & T_PV;
   Vd<N> hU[10], vJ[10];
 #pragma unroll
   for (int u = 0; u < 10; ++u) hU[u] = rowv<N, W>(h, uc[u], L, k);
@@ -807,21 +810,21 @@ __device__ __forceinline__ void edge_tend_hex2v(const MeshView &m, const ModelCo
       T[i] = T[i] + qt;
     }
   }
-  if (a.terms & T_KE) {
+  if (terms & T_KE) {
     const Vd<N> k0 = rowv<N, W>(d.ke_c, c0, L, k), k1 = rowv<N, W>(d.ke_c, c1, L, k);
 #pragma unroll
     for (int i = 0; i < N; ++i) T[i] = T[i] - (k1[i] - k0[i]) * idc;
   }
-  if (a.terms & T_SSH) {
+  if (terms & T_SSH) {
 #pragma unroll
     for (int i = 0; i < N; ++i) T[i] = T[i] - mc.grav * (hU[q1][i] - hU[q0][i]) * idc;
   }
-  if (a.terms & (T_VDEL2 | T_VDEL4)) {
+  if (terms & (T_VDEL2 | T_VDEL4)) {
     const int ml = cpick<U>(m.maxlev_e, eA, eB, hi);
     const double idv = cpick<U>(m.inv_dv, eA, eB, hi);
     const int b0 = pick2(rA, 10 + o0, (long)(rB - rA) + 10 + o0, hi);
     const int b1 = pick2(rA, 10 + o1, (long)(rB - rA) + 10 + o1, hi);
-    if (a.terms & T_VDEL2) {
+    if (terms & T_VDEL2) {
       const Vd<N> d0 = rowv<N, W>(d.div_c, c0, L, k), d1 = rowv<N, W>(d.div_c, c1, L, k);
       const Vd<N> r0 = rowv<N, W>(d.rvort_v, b0, L, k), r1 = rowv<N, W>(d.rvort_v, b1, L, k);
       const double cv = cpick<U>(mc.cv2, eA, eB, hi);
@@ -831,7 +834,7 @@ __device__ __forceinline__ void edge_tend_hex2v(const MeshView &m, const ModelCo
         T[i] = T[i] + ((k + i < ml) ? cv : 0.0) * del2u;
       }
     }
-    if (a.terms & T_VDEL4) {
+    if (terms & T_VDEL4) {
       const Vd<N> d0 = rowv<N, W>(d.vdel2_div_c, c0, L, k), d1 = rowv<N, W>(d.vdel2_div_c, c1, L, k);
       const Vd<N> r0 = rowv<N, W>(d.vdel2_rv_v, b0, L, k), r1 = rowv<N, W>(d.vdel2_rv_v, b1, L, k);
       const double cv = cpick<U>(mc.cv4, eA, eB, hi);
@@ -870,7 +873,7 @@ __device__ __forceinline__ void union_pv2(const MeshView &m, const ModelCoef &mc
   }
 }
 
-template <bool U, bool W>
+template <bool U, bool W, int TM = -1>
 __device__ __forceinline__ void cell_tend_hex2(const MeshView &m, const ModelCoef &mc,
                                                const StageArgs &a, const DiagOut &d, int cA,
                                                int cB, bool hi, int k) {
@@ -902,7 +905,7 @@ __device__ __forceinline__ void cell_tend_hex2(const MeshView &m, const ModelCoe
     for (int i = 0; i < N; ++i) fh[j][i] = 0.5 * (hN[j][i] + hC[i]);
   }
   const size_t hidx = (size_t)c * L + k;
-  if (a.do_h) {
+  if (TM >= 0 || a.do_h) {
     Vd<N> T = tend_base<N>(a, a.h, hidx);
 #pragma unroll
     for (int i = 0; i < N; ++i) {
@@ -917,7 +920,8 @@ __device__ __forceinline__ void cell_tend_hex2(const MeshView &m, const ModelCoe
   }
   const int ntr = a.ntr;
   if (ntr <= 0) return;
-  const bool td2 = a.terms & T_TDEL2, td4 = a.terms & T_TDEL4;
+  const int terms = TM >= 0 ? TM : a.terms;
+  const bool td2 = terms & T_TDEL2, td4 = terms & T_TDEL4;
   Vd<N> invC, invN[6];
   double idc[6], c2[6], s2[6], c4[6];
 #pragma unroll

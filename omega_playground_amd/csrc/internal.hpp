@@ -124,6 +124,9 @@ struct MeshView {
   // s4, w, cdiv, cke, crv, cth) are equal at every stencil slot, bit for
   // bit; off32: every [rows][L] array has < 2^24 rows and < 4 GiB
   int coef_uni, off32;
+  // the pair tendency kernel may use its compiled-in term sets (cleared by
+  // a model's kernel forms for tests)
+  int term_sets;
   const int *hexrec;            // [ne][30]
   const int *hexu;              // [nc][49] per-cell union stencil (hex_pattern.hpp)
   const int *coc;               // cells_on_cell [nc][6]
@@ -257,6 +260,8 @@ void launch_tend_linear(const MeshView &m, const Launch &lc, const LinCoef &lin,
                         const StageArgs &a, hipStream_t s);
 // Pattern-specialised kernels for the regular hexagonal mesh (kernels_hex.hip)
 bool hex_usable(const MeshView &m, const Launch &lc);
+// the pair tendency kernel has a compiled-in form for this term set
+bool hex_term_set(int terms);
 void launch_diag_hex(const MeshView &m, const Launch &lc, int terms, int ntr, const Real *h,
                      const Real *vn, const Real *tr, const DiagOut &d, hipStream_t s);
 void launch_tend_hex(const MeshView &m, const Launch &lc, const ModelCoef &mc, const StageArgs &a,

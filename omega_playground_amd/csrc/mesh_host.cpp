@@ -504,6 +504,7 @@ MeshView DeviceMesh::view() const {
   v.hexd_ok = hex_ok && hexd_ok ? 1 : 0;
   v.coef_uni = hex_ok && coef_uni ? 1 : 0;
   v.off32 = off32 ? 1 : 0;
+  v.term_sets = 1;
   v.hexrec = hexrec.p;
   v.hexu = hexu.p;
   v.coc = coc.p;

@@ -92,13 +92,15 @@ struct omega_model_s {
   // the mesh (ModelCoef::uniform)
   bool f_uniform = false, mix_uniform = false;
   // pair-kernel forms the model may use (bit 0 shared coefficients, bit 1
-  // 32-bit offsets); clearing bits forces the general forms (tests, tuning)
-  int forms = 3;
+  // 32-bit offsets, bit 2 compiled-in term sets); clearing bits forces the
+  // general forms (tests, tuning)
+  int forms = 7;
   // the mesh view this model's kernels get
   omega_hip::MeshView view() const {
     omega_hip::MeshView v = mesh->dm.view();
     if (!(forms & 1)) v.coef_uni = 0;
     if (!(forms & 2)) v.off32 = 0;
+    if (!(forms & 4)) v.term_sets = 0;
     return v;
   }
   ModelCoef coef() const {

@@ -561,14 +561,16 @@ class ShallowWaterModel:
     def kernel_flags(self) -> int:
         """bit 0: coefficient tables uniform per stencil slot (shared by a
         wavefront's two cells); bit 1: 32-bit row offsets; bit 2: the
-        tendency VJP's slot-uniform form (after the first adjoint call)."""
+        tendency VJP's slot-uniform form (after the first adjoint call);
+        bit 3: the pair tendency kernel compiled for the model's term set."""
         v = C.c_int()
         check(lib().omega_model_kernel_flags(self._h, C.byref(v)))
         return v.value
 
-    def set_kernel_forms(self, forms: int = 3):
-        """Allow (bits set) or forbid the shared-coefficient (bit 0) and
-        32-bit-offset (bit 1) forms of the pair kernels."""
+    def set_kernel_forms(self, forms: int = 7):
+        """Allow (bits set) or forbid the shared-coefficient (bit 0),
+        32-bit-offset (bit 1) and compiled-in term-set (bit 2) forms of the
+        pair kernels."""
         check(lib().omega_model_set_kernel_forms(self._h, int(forms)))
 
     def kernel_variant(self) -> int:

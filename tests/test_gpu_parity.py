@@ -418,3 +418,36 @@ def test_mesh_layer_and_tracer_counts_bitwise(nx, ny, L, ntr):
         ost.run(0.0, dt, 2, ref)
         for name, a, b in zip(("h", "vn", "tr"), gs.download(), ref):
             assert same_bits(a, b), (kind, name)
+
+
+@pytest.mark.parametrize("terms", [7, 15, 79, 127])
+def test_term_set_kernels_bitwise(terms):
+    """For the term sets compiled into the pair tendency kernel (kernel flag
+    bit 3) the terms are resolved at compile time; with that form forbidden
+    (forms bit 2) the same kernel reads them at run time.  Both give the
+    oracle's bits for the tendency, RK4 steps and LSRK5 steps."""
+    om, omodel, gm, gmodel, state = make_pair(12, 10, 64, 2, seed=terms)
+    omodel.terms = terms
+    gmodel.terms = terms
+    print("case", terms, flush=True)
+    tend_o = omodel.new_state()
+    omodel.compute_tendency(tend_o, state, 0.0)
+    for forms in (7, 3):
+        gmodel.set_kernel_forms(forms)
+        assert bool(gmodel.kernel_flags() & 8) == (forms == 7), gmodel.kernel_flags()
+        gt = swe.ShallowWaterState(gm, 2)
+        gmodel.compute_tendency(gt, upload(gm, 2, state), 0.0)
+        for name, a, b in zip(("h", "vn", "tr"), gt.download(), tend_o):
+            assert same_bits(a, b), (forms, name)
+        for kind in ("rk4", "lsrk"):
+            gs = upload(gm, 2, state)
+            ref = tuple(a.copy() for a in state)
+            if kind == "rk4":
+                gst, ost = swe.RK4Stepper(gmodel), po.Stepper(omodel, "rk4")
+                gst.set_graph(True)
+            else:
+                gst, ost = swe.LSRKStepper(gmodel, 5), po.Stepper(omodel, "lsrk", 5)
+            gst.run(0.0, 30.0, 2, gs)
+            ost.run(0.0, 30.0, 2, ref)
+            for name, a, b in zip(("h", "vn", "tr"), gs.download(), ref):
+                assert same_bits(a, b), (forms, kind, name)
