@@ -53,10 +53,11 @@ __device__ __forceinline__ int logical_block(int b, int G, int swz) {
 }
 
 // Stage epilogue: consumes the tendency T of one (entity, level-vector).
-template <int N>
+// E >= 0: the epilogue kind compiled in (it must equal a.epi).
+template <int N, int E = -1>
 __device__ __forceinline__ void epilogue(const StageArgs &a, const FieldIO &f, size_t idx,
                                          const Vd<N> &T) {
-  switch (a.epi) {
+  switch (E >= 0 ? E : a.epi) {
   case EPI_STORE:
     st<N>(f.tend + idx, T);
     break;

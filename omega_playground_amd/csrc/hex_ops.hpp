@@ -749,7 +749,7 @@ __device__ __forceinline__ void diag_union_p(const MeshView &m, int terms, int n
 // union vertices (hex_pattern.hpp, namespace hexu) are formed once per cell
 // by the caller and shared by its three edges; the rows per edge are then
 // 10 thicknesses and 10 velocities.  Same values, same expressions.
-template <int K, bool U, bool W, int TM = -1>
+template <int K, bool U, bool W, int TM = -1, int EK = -1>
 __device__ __forceinline__ void edge_tend_hex2v(const MeshView &m, const ModelCoef &mc,
                                                 const StageArgs &a, const DiagOut &d, int eA,
                                                 int eB, bool hi, int k, const Vd<2> (&P)[hexu::kNV],
@@ -763,7 +763,7 @@ __device__ __forceinline__ void edge_tend_hex2v(const MeshView &m, const ModelCo
   const size_t idx = (size_t)e * L + k;
   if (!do_vn) {
     if (a.epi == EPI_STORE) return;
-    epilogue<N>(a, a.vn, idx, tend_base<N>(a, a.vn, idx));
+    epilogue<N, EK>(a, a.vn, idx, tend_base<N>(a, a.vn, idx));
     return;
   }
   const double *__restrict__ h = a.h.in;
@@ -845,7 +845,7 @@ __device__ __forceinline__ void edge_tend_hex2v(const MeshView &m, const ModelCo
       }
     }
   }
-  epilogue<N>(a, a.vn, idx, T);
+  epilogue<N, EK>(a, a.vn, idx, T);
 }
 
 // PV products inv_thick*rel_vort and inv_thick*f of the 16 union vertices
@@ -873,7 +873,7 @@ __device__ __forceinline__ void union_pv2(const MeshView &m, const ModelCoef &mc
   }
 }
 
-template <bool U, bool W, int TM = -1>
+template <bool U, bool W, int TM = -1, int EK = -1>
 __device__ __forceinline__ void cell_tend_hex2(const MeshView &m, const ModelCoef &mc,
                                                const StageArgs &a, const DiagOut &d, int cA,
                                                int cB, bool hi, int k) {
@@ -914,9 +914,9 @@ __device__ __forceinline__ void cell_tend_hex2(const MeshView &m, const ModelCoe
       for (int j = 0; j < 6; ++j) acc = acc - cd[j] * fh[j][i] * vE[j][i];
       T[i] = T[i] + acc;
     }
-    epilogue<N>(a, a.h, hidx, T);
+    epilogue<N, EK>(a, a.h, hidx, T);
   } else if (a.epi != EPI_STORE) {
-    epilogue<N>(a, a.h, hidx, tend_base<N>(a, a.h, hidx));
+    epilogue<N, EK>(a, a.h, hidx, tend_base<N>(a, a.h, hidx));
   }
   const int ntr = a.ntr;
   if (ntr <= 0) return;
@@ -971,7 +971,7 @@ __device__ __forceinline__ void cell_tend_hex2(const MeshView &m, const ModelCoe
       if (td2) T[i] = T[i] + dif;
       if (td4) T[i] = T[i] + hyp;
     }
-    epilogue<N>(a, a.tr, tidx, T);
+    epilogue<N, EK>(a, a.tr, tidx, T);
   }
 }
 

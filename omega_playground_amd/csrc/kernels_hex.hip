@@ -107,8 +107,9 @@ __global__ void __launch_bounds__(256)
 // formed once and shared by its three edges (needs the per-cell union
 // records, hexu).  TM >= 0: the term set TM with both tendencies on,
 // resolved at compile time (no run-time term branches between the stencil
-// phases, so the compiler schedules across them); -1: read from a.
-template <bool U, bool W, int TM = -1>
+// phases, so the compiler schedules across them); E >= 0: the stage
+// epilogue likewise; -1: read from a.
+template <bool U, bool W, int TM = -1, int E = -1>
 __global__ void __launch_bounds__(256)
     k_tend_hex2v(MeshView m, ModelCoef mc, StageArgs a, DiagOut d, int swz) {
   const int s = logical_block(blockIdx.x, gridDim.x, swz);
@@ -124,22 +125,33 @@ __global__ void __launch_bounds__(256)
   // keeps the compiler from hoisting the edges' loads above the union PV
   // products, which measured 6 % slower (1.957 against 1.850 ms at nx=512)
   if (a.do_vn && (a.terms & T_PV)) union_pv2<U, W>(m, mc, d, cA, cB, hi, k, P, Q);
-  edge_tend_hex2v<0, U, W, TM>(m, mc, a, d, 3 * cA, 3 * cB, hi, k, P, Q);
-  edge_tend_hex2v<1, U, W, TM>(m, mc, a, d, 3 * cA + 1, 3 * cB + 1, hi, k, P, Q);
-  edge_tend_hex2v<2, U, W, TM>(m, mc, a, d, 3 * cA + 2, 3 * cB + 2, hi, k, P, Q);
-  cell_tend_hex2<U, W, TM>(m, mc, a, d, cA, cB, hi, k);
+  edge_tend_hex2v<0, U, W, TM, E>(m, mc, a, d, 3 * cA, 3 * cB, hi, k, P, Q);
+  edge_tend_hex2v<1, U, W, TM, E>(m, mc, a, d, 3 * cA + 1, 3 * cB + 1, hi, k, P, Q);
+  edge_tend_hex2v<2, U, W, TM, E>(m, mc, a, d, 3 * cA + 2, 3 * cB + 2, hi, k, P, Q);
+  cell_tend_hex2<U, W, TM, E>(m, mc, a, d, cA, cB, hi, k);
 }
 
 // Term sets compiled into the shared-coefficient, 32-bit-offset form:
 // momentum and thickness only (7), with del2 viscosity (15), with del2
-// viscosity and tracer del4 (79), everything (127)
+// viscosity and tracer del4 (79), everything (127); each also with the
+// epilogue of a time step's stages compiled in (RK4 first, middle and last
+// stage, low-storage RK stage)
 using TendFn = void (*)(MeshView, ModelCoef, StageArgs, DiagOut, int);
-TendFn tend_term_set(int terms) {
+template <int TM> TendFn tend_for_epilogue(int epi) {
+  switch (epi) {
+  case EPI_RK4_FIRST: return k_tend_hex2v<true, false, TM, EPI_RK4_FIRST>;
+  case EPI_RK4_MID: return k_tend_hex2v<true, false, TM, EPI_RK4_MID>;
+  case EPI_RK4_LAST: return k_tend_hex2v<true, false, TM, EPI_RK4_LAST>;
+  case EPI_LSRK: return k_tend_hex2v<true, false, TM, EPI_LSRK>;
+  default: return k_tend_hex2v<true, false, TM>;
+  }
+}
+TendFn tend_term_set(int terms, int epi) {
   switch (terms) {
-  case 7: return k_tend_hex2v<true, false, 7>;
-  case 15: return k_tend_hex2v<true, false, 15>;
-  case 79: return k_tend_hex2v<true, false, 79>;
-  case 127: return k_tend_hex2v<true, false, 127>;
+  case 7: return tend_for_epilogue<7>(epi);
+  case 15: return tend_for_epilogue<15>(epi);
+  case 79: return tend_for_epilogue<79>(epi);
+  case 127: return tend_for_epilogue<127>(epi);
   default: return nullptr;
   }
 }
@@ -166,7 +178,7 @@ int hex_n(const MeshView &m, const Launch &lc) { return lc.lpt == 1 ? 1 : 2; }
 
 } // namespace
 
-bool hex_term_set(int terms) { return tend_term_set(terms) != nullptr; }
+bool hex_term_set(int terms) { return tend_term_set(terms, -1) != nullptr; }
 
 bool hex_usable(const MeshView &m, const Launch &lc) {
   // 16 or 32 layers: several cells per wavefront, two levels per lane
@@ -203,7 +215,7 @@ void launch_tend_hex(const MeshView &m, const Launch &lc, const ModelCoef &mc, c
     auto kern = U ? (m.off32 ? k_tend_hex2v<true, false> : k_tend_hex2v<true, true>)
                   : (m.off32 ? k_tend_hex2v<false, false> : k_tend_hex2v<false, true>);
     if (U && m.off32 && m.term_sets && a.do_h && a.do_vn)
-      if (auto ks = tend_term_set(a.terms)) kern = ks;
+      if (auto ks = tend_term_set(a.terms, a.epi)) kern = ks;
     hipLaunchKernelGGL(kern, grid, block, 0, s, m, mc, a, d, lc.swizzle);
   } else if (hex_pair(m, N)) {
     auto kern = U ? (m.off32 ? k_tend_hex2<true, false> : k_tend_hex2<true, true>)
