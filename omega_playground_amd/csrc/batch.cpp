@@ -39,6 +39,7 @@ struct omega_batch_s {
   // one table over all members; member kinds share each launch
   int nblocks = 0;
   int kinds = 0; // bit 0 general-kernel members, bit 1 pattern-specialised
+  int tset = 0;  // term set compiled into the pair tendency launch (0: none)
   bool any_vdel4 = false;
   DBuf<int> blk;
   DBuf<BatchMember> desc; // [parity][stage][member]
@@ -135,6 +136,24 @@ void build_tables(omega_batch_s *B) {
   B->nblocks = (int)blk.size();
   B->kinds = 0;
   for (const BatchMember &g : geo) B->kinds |= g.hex ? 2 : 1;
+  // the pair members' term set, compiled into the tendency launch when they
+  // all share it (16/32-layer members keep reading theirs at run time)
+  B->tset = 0;
+  if (B->kinds == 2 && N == 2) {
+    int ts = 0;
+    bool ok = true;
+    for (int i = 0; i < nm && ok; ++i) {
+      omega_model_s *M = B->mem[i]->M;
+      if (M->mesh->dm.host.nlayers % 64 != 0) continue;
+      const MeshView v = M->view();
+      const ModelCoef c = M->coef();
+      ok = v.coef_uni && c.uniform && v.off32 && v.term_sets && v.hexu_ok &&
+           !M->params.disable_h_tendency && !M->params.disable_vn_tendency &&
+           term_set_batch(M->terms) && (ts == 0 || ts == M->terms);
+      ts = M->terms;
+    }
+    if (ok) B->tset = ts;
+  }
   std::vector<BatchMember> tab((size_t)2 * ns * nm);
   for (int p = 0; p < 2; ++p)
     for (int s = 0; s < ns; ++s)
@@ -235,7 +254,7 @@ void step_launches(omega_batch_s *B, int parity, hipStream_t s) {
     const int n = B->nblocks;
     launch_batch_diag(d, g, n, B->threads, B->N, pers, B->kinds, s);
     if (B->any_vdel4) launch_batch_vdel4(d, g, n, B->threads, B->N, pers, s);
-    launch_batch_tend(d, g, n, B->threads, B->N, pers, B->kinds, s);
+    launch_batch_tend(d, g, n, B->threads, B->N, pers, B->kinds, B->tset, s);
   }
 }
 

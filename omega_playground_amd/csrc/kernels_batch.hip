@@ -203,25 +203,27 @@ __device__ __forceinline__ Row row_of(const BatchMember &B, int lb) {
   return Row{seg * B.rows + (int)threadIdx.x / B.lx, (chunk * B.lx + (int)threadIdx.x % B.lx) * 2};
 }
 
-template <bool U, bool W>
+// TM >= 0: the member's term set, compiled in (both tendencies on, the
+// vertex-union records present)
+template <bool U, bool W, int TM = -1>
 __device__ __forceinline__ void batch_tend_pair(const BatchMember &B, const Pair &q) {
   const MeshView &m = B.m;
   const ModelCoef &mc = B.mc;
   const StageArgs &a = B.a;
   const DiagOut &d = B.d;
   const int eA = 3 * q.cA, eB = 3 * q.cB;
-  if (m.hexu_ok) {
+  if (TM >= 0 || m.hexu_ok) {
     Vd<2> P[hexu::kNV], Q[hexu::kNV];
     if (a.do_vn && (a.terms & T_PV)) hexops::union_pv2<U, W>(m, mc, d, q.cA, q.cB, q.hi, q.k, P, Q);
-    hexops::edge_tend_hex2v<0, U, W>(m, mc, a, d, eA, eB, q.hi, q.k, P, Q);
-    hexops::edge_tend_hex2v<1, U, W>(m, mc, a, d, eA + 1, eB + 1, q.hi, q.k, P, Q);
-    hexops::edge_tend_hex2v<2, U, W>(m, mc, a, d, eA + 2, eB + 2, q.hi, q.k, P, Q);
+    hexops::edge_tend_hex2v<0, U, W, TM>(m, mc, a, d, eA, eB, q.hi, q.k, P, Q);
+    hexops::edge_tend_hex2v<1, U, W, TM>(m, mc, a, d, eA + 1, eB + 1, q.hi, q.k, P, Q);
+    hexops::edge_tend_hex2v<2, U, W, TM>(m, mc, a, d, eA + 2, eB + 2, q.hi, q.k, P, Q);
   } else {
     hexops::edge_tend_hex2<0, U, W>(m, mc, a, d, eA, eB, q.hi, q.k);
     hexops::edge_tend_hex2<1, U, W>(m, mc, a, d, eA + 1, eB + 1, q.hi, q.k);
     hexops::edge_tend_hex2<2, U, W>(m, mc, a, d, eA + 2, eB + 2, q.hi, q.k);
   }
-  hexops::cell_tend_hex2<U, W>(m, mc, a, d, q.cA, q.cB, q.hi, q.k);
+  hexops::cell_tend_hex2<U, W, TM>(m, mc, a, d, q.cA, q.cB, q.hi, q.k);
 }
 
 __device__ __forceinline__ void hex2_diag(const BatchMember &B, int lb) {
@@ -242,7 +244,7 @@ __device__ __forceinline__ void hex2_diag(const BatchMember &B, int lb) {
   else batch_diag_pair<false, true>(B, q);
 }
 
-__device__ __forceinline__ void hex2_tend(const BatchMember &B, int lb) {
+template <int TM = -1> __device__ __forceinline__ void hex2_tend(const BatchMember &B, int lb) {
   if (B.m.L % 64 != 0) {
     const Row r = row_of(B, lb);
     if (r.c >= B.m.nc) return;
@@ -254,7 +256,8 @@ __device__ __forceinline__ void hex2_tend(const BatchMember &B, int lb) {
   }
   const Pair q = pair_of(B, lb);
   if (!q.ok) return;
-  if (B.m.coef_uni && B.mc.uniform && B.m.off32) batch_tend_pair<true, false>(B, q);
+  if constexpr (TM >= 0) batch_tend_pair<true, false, TM>(B, q);
+  else if (B.m.coef_uni && B.mc.uniform && B.m.off32) batch_tend_pair<true, false>(B, q);
   else batch_tend_pair<false, true>(B, q);
 }
 
@@ -280,7 +283,10 @@ __global__ void __launch_bounds__(256)
   });
 }
 
-template <int N, int KINDS, bool PERS>
+// TM >= 0 (pattern-specialised members only, N = 2): every 64k-layer member
+// has the shared-coefficient, 32-bit-offset forms and the term set TM with
+// both tendencies on, so their pair operators get it compiled in.
+template <int N, int KINDS, bool PERS, int TM = -1>
 __global__ void __launch_bounds__(256)
     k_batch_tend(const BatchMember *__restrict__ desc, const int *__restrict__ blk, int nblocks) {
   for_entries<PERS>(nblocks, [&](int b) {
@@ -288,7 +294,7 @@ __global__ void __launch_bounds__(256)
     const BatchMember &B = desc[id];
     const int lb = b - B.block0;
     if (KINDS == 2 || (KINDS == 3 && B.hex)) {
-      if constexpr ((KINDS & 2) && N == 2) hex2_tend(B, lb);
+      if constexpr ((KINDS & 2) && N == 2) hex2_tend<TM>(B, lb);
       else if constexpr (KINDS & 2) hex1_tend(B, lb);
     } else if constexpr (KINDS & 1) {
       gen_tend<N>(B, lb);
@@ -314,6 +320,23 @@ void launch_phase(int phase, const BatchMember *desc, const int *blk, int nblock
   else
     hipLaunchKernelGGL((k_batch_tend<N, KINDS, PERS>), dim3(grid), dim3(threads), 0, s, desc,
                        blk, nblocks);
+}
+
+// the tendency launch of a pattern-specialised ensemble with a compiled-in
+// term set (term_set_batch)
+template <bool PERS>
+bool launch_tend_set(int tset, const BatchMember *desc, const int *blk, int nblocks, int grid,
+                     int threads, hipStream_t s) {
+  void (*k)(const BatchMember *, const int *, int) = nullptr;
+  switch (tset) {
+  case 7: k = k_batch_tend<2, 2, PERS, 7>; break;
+  case 15: k = k_batch_tend<2, 2, PERS, 15>; break;
+  case 79: k = k_batch_tend<2, 2, PERS, 79>; break;
+  case 127: k = k_batch_tend<2, 2, PERS, 127>; break;
+  default: return false;
+  }
+  hipLaunchKernelGGL(k, dim3(grid), dim3(threads), 0, s, desc, blk, nblocks);
+  return true;
 }
 
 template <int KINDS, bool PERS>
@@ -374,9 +397,22 @@ void launch_batch_vdel4(const BatchMember *desc, const int *blk, int nblocks, in
 }
 
 void launch_batch_tend(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
-                       int persistent, int kinds, hipStream_t s) {
+                       int persistent, int kinds, int tset, hipStream_t s) {
   TimerScope ts("batch_tend", s);
+  if (tset > 0 && N == 2 && kinds == 2) {
+    int grid;
+    bool pers;
+    grid_of(nblocks, persistent, grid, pers);
+    const bool done = pers ? launch_tend_set<true>(tset, desc, blk, nblocks, grid, threads, s)
+                           : launch_tend_set<false>(tset, desc, blk, nblocks, grid, threads, s);
+    if (done) {
+      OMEGA_HIP_CHECK(hipGetLastError());
+      return;
+    }
+  }
   launch_kinds(1, desc, blk, nblocks, threads, N, persistent, kinds, s);
 }
+
+bool term_set_batch(int terms) { return terms == 7 || terms == 15 || terms == 79 || terms == 127; }
 
 } // namespace omega_hip

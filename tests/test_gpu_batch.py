@@ -235,3 +235,32 @@ def test_model_autotune_keeps_results():
     po.Stepper(omodel, "rk4").run(0.0, 20.0, 2, ref)
     for a, c in zip(gs.download(), ref):
         assert same_bits(a, c)
+
+
+@pytest.mark.parametrize("kind,nstages", [("lsrk", 5), ("rk4", 4)])
+@pytest.mark.parametrize("forms", [7, 3])
+def test_batch_compiled_term_set(kind, nstages, forms):
+    """Pattern-specialised members that share a term set (here del2
+    viscosity and tracer del4, the 64-layer members' pair operators get it
+    compiled into the ensemble's tendency launch; 16/32-layer members keep
+    reading theirs) end with the bits of their solo steps, as with that
+    form forbidden."""
+    over = dict(visc_del4=0.0, eddy_diff2=0.0)
+    members = make_members([(8, 8, 64, 2, 10e3, over, 12.0), (10, 8, 64, 1, 10e3, over, 15.0),
+                            (8, 6, 32, 3, 10e3, over, 14.0), (12, 10, 128, 2, 10e3, over, 9.0)])
+    for m in members:
+        m[1].set_kernel_forms(forms)
+        assert m[1].terms == 79
+    print("case", kind, forms, flush=True)
+    ref = solo(members, kind, nstages, 3)
+    b = swe.Batch(kind, nstages)
+    states = []
+    for mesh, model, ntr, y0, dt, _ in members:
+        s = state_of(mesh, ntr, y0)
+        b.add(model, s, dt)
+        states.append(s)
+    b.set_graph(True)
+    b.run(3)
+    for i, (s, r) in enumerate(zip(states, ref)):
+        for name, a, c in zip(("h", "vn", "tr"), s.download(), r):
+            assert same_bits(a, c), (i, name)
