@@ -253,13 +253,18 @@ __global__ void __launch_bounds__(kAB)
 // Per edge at the linearisation state: the PV sum S_e = Q_e + P_e (PV term
 // only) and the thickness flux F_e u_e, read by the PV pass for every edge
 // whose edges-on-edge list holds e, by the edge pass and by the cell pass.
-template <int N, int H, int U>
+template <int N, int H, int U, int TM = -1>
 __global__ void __launch_bounds__(kAB)
     k_adj_pre(MeshView m, ModelCoef mc, AdjArgs a, DiagOut d, AdjScratch s, int swz) {
+  // TM >= 0: the term set, compiled in, with both tendencies on
+  const int terms = TM >= 0 ? TM : a.terms;
+  const bool do_vn = TM >= 0 || a.do_vn, do_h = TM >= 0 || a.do_h;
+  (void)do_h;
+  (void)do_vn;
   const Seg g = segment<N>(m, swz);
   if (!g.active) return;
   const int L = m.L, k = g.k;
-  const bool pv = a.do_vn && (a.terms & T_PV);
+  const bool pv = do_vn && (terms & T_PV);
   for_edges<U>(m, g, [&](int e, int K) {
     if (pv) {
       const int2 vv = pair_at(m.voe, e);
@@ -334,14 +339,19 @@ __global__ void __launch_bounds__(kAB)
 // ---------------------------------------------------------------------------
 // Vertex and cell diagnostics cotangents
 // ---------------------------------------------------------------------------
-template <int N, int H, int U>
+template <int N, int H, int U, int TM = -1>
 __global__ void __launch_bounds__(kAB)
     k_adj_vc(MeshView m, AdjMeshView am, ModelCoef mc, AdjCoef ac, AdjArgs a, DiagOut d,
              AdjScratch s, int swz) {
+  // TM >= 0: the term set, compiled in, with both tendencies on
+  const int terms = TM >= 0 ? TM : a.terms;
+  const bool do_vn = TM >= 0 || a.do_vn, do_h = TM >= 0 || a.do_h;
+  (void)do_h;
+  (void)do_vn;
   const Seg g = segment<N>(m, swz);
   const int L = m.L, k = g.k, NC = m.nc;
-  const bool pv = a.do_vn && (a.terms & T_PV), vd2 = a.do_vn && (a.terms & T_VDEL2),
-             vd4 = a.do_vn && (a.terms & T_VDEL4), ke = a.do_vn && (a.terms & T_KE);
+  const bool pv = do_vn && (terms & T_PV), vd2 = do_vn && (terms & T_VDEL2),
+             vd4 = do_vn && (terms & T_VDEL4), ke = do_vn && (terms & T_KE);
   // gradient w.r.t. visc_del2: sum_e kbar_e s2_e [(D_c1 - D_c0)/dc - (Z_v1 - Z_v0)/dv]
   // regrouped by cell and vertex, sum_c D_c ds_c + sum_v Z_v zs_v, with ds and
   // zs formed beside the del2 cotangents (no edge rows re-read)
@@ -396,7 +406,7 @@ __global__ void __launch_bounds__(kAB)
       st<N>(s.tbar_v + (size_t)v * L + k, tb);
     });
   }
-  const bool td4 = (a.terms & T_TDEL4) && a.ntr > 0;
+  const bool td4 = (terms & T_TDEL4) && a.ntr > 0;
   for (int c = g.c0 + (int)threadIdx.y; g.active && c < g.c1; c += blockDim.y) {
     Vd<N> kb = zero<N>(), db = zero<N>(), ds = zero<N>();
     const int n = H ? 6 : m.noc[c];
@@ -476,16 +486,21 @@ __global__ void __launch_bounds__(kAB)
 // ---------------------------------------------------------------------------
 // Edge pass: velocity cotangent (accumulated) and edge-thickness cotangent
 // ---------------------------------------------------------------------------
-template <int N, int H, int X, int U>
+template <int N, int H, int X, int U, int TM = -1>
 // four waves per SIMD (at most 128 VGPRs; fits without spilling)
 __global__ void __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4)))
     k_adj_edge(MeshView m, AdjMeshView am, ModelCoef mc, AdjCoef ac, AdjArgs a, DiagOut d,
                AdjScratch s, int swz) {
+  // TM >= 0: the term set, compiled in, with both tendencies on
+  const int terms = TM >= 0 ? TM : a.terms;
+  const bool do_vn = TM >= 0 || a.do_vn, do_h = TM >= 0 || a.do_h;
+  (void)do_h;
+  (void)do_vn;
   const Seg g = segment<N>(m, swz);
   const int L = m.L, k = g.k, NC = m.nc;
-  const bool pv = a.do_vn && (a.terms & T_PV), vd2 = a.do_vn && (a.terms & T_VDEL2),
-             vd4 = a.do_vn && (a.terms & T_VDEL4), ke = a.do_vn && (a.terms & T_KE);
-  const bool td2 = X && (a.terms & T_TDEL2), td4 = X && (a.terms & T_TDEL4);
+  const bool pv = do_vn && (terms & T_PV), vd2 = do_vn && (terms & T_VDEL2),
+             vd4 = do_vn && (terms & T_VDEL4), ke = do_vn && (terms & T_KE);
+  const bool td2 = X && (terms & T_TDEL2), td4 = X && (terms & T_TDEL4);
   double p_v4 = 0.0;
   if (g.active) {
     for_edges<U>(m, g, [&](int e, int K) {
@@ -505,7 +520,7 @@ __global__ void __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4)))
           ih1[i] = 1.0 / h1[i];
         }
       }
-      if (a.do_h) {
+      if (do_h) {
         const Vd<N> g0 = rd<N>(a.c.h, c0, L, k), g1 = rd<N>(a.c.h, c1, L, k);
 #pragma unroll
         for (int i = 0; i < N; ++i) {
@@ -548,7 +563,7 @@ __global__ void __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4)))
           st<N>(s.xa_e + ((size_t)l * m.ne + e) * L + k, xa);
         }
       }
-      if (a.do_vn) {
+      if (do_vn) {
         if (pv) {
           // sum_t kbar_t w_te (S_t + S_e)/2 = (A_e + S_e B_e)/2 (k_adj_pre, k_adj_pv)
           const Vd<N> Se = rd<N>(s.se_e, e, L, k);
@@ -599,14 +614,19 @@ __global__ void __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4)))
 // ---------------------------------------------------------------------------
 // Cell pass: thickness and tracer cotangents (accumulated)
 // ---------------------------------------------------------------------------
-template <int N, int H, int X, int U>
+template <int N, int H, int X, int U, int TM = -1>
 __global__ void __launch_bounds__(kAB)
     k_adj_cell(MeshView m, AdjMeshView am, ModelCoef mc, AdjCoef ac, AdjArgs a, DiagOut d,
                AdjScratch s, int swz) {
+  // TM >= 0: the term set, compiled in, with both tendencies on
+  const int terms = TM >= 0 ? TM : a.terms;
+  const bool do_vn = TM >= 0 || a.do_vn, do_h = TM >= 0 || a.do_h;
+  (void)do_h;
+  (void)do_vn;
   const Seg g = segment<N>(m, swz);
   const int L = m.L, k = g.k, NC = m.nc, me = m.me;
-  const bool pv = a.do_vn && (a.terms & T_PV), ssh = a.do_vn && (a.terms & T_SSH);
-  const bool td2 = X && (a.terms & T_TDEL2), td4 = X && (a.terms & T_TDEL4);
+  const bool pv = do_vn && (terms & T_PV), ssh = do_vn && (terms & T_SSH);
+  const bool td2 = X && (terms & T_TDEL2), td4 = X && (terms & T_TDEL4);
   double p_t2 = 0.0, p_t4 = 0.0;
   if (g.active) {
     for (int c = g.c0 + (int)threadIdx.y; c < g.c1; c += blockDim.y) {
@@ -909,11 +929,24 @@ void launch_tendency_vjp(const MeshView &m, const AdjMeshView &am, const Launch 
   const bool vd4 = a.do_vn && (a.terms & T_VDEL4);
   const bool pv = a.do_vn && (a.terms & T_PV);
   // regular hexagonal meshes: constant neighbour counts, unrolled loops;
-  // with slot-uniform tables and coefficients also the per-slot reads (U)
+  // with slot-uniform tables and coefficients also the per-slot reads (U),
+  // and for the common term sets with both tendencies on the terms compiled
+  // in (tm)
   const bool U = g.N == 2 && am.fixed && am.uni && m.coef_uni && mc.uniform && ac.uniform;
+  const bool tset = U && m.term_sets && a.do_h && a.do_vn &&
+                    (a.terms == 7 || a.terms == 15 || a.terms == 79 || a.terms == 127);
+  const int tm = tset ? a.terms : -1;
 #define OMEGA_ADJ_LAUNCH(K, ...)                                                                \
   do {                                                                                         \
-    if (U)                                                                                     \
+    if (tm == 7)                                                                               \
+      hipLaunchKernelGGL((K<2, 1, 1, 7>), g.grid, g.block, 0, st, __VA_ARGS__);                 \
+    else if (tm == 15)                                                                         \
+      hipLaunchKernelGGL((K<2, 1, 1, 15>), g.grid, g.block, 0, st, __VA_ARGS__);                \
+    else if (tm == 79)                                                                         \
+      hipLaunchKernelGGL((K<2, 1, 1, 79>), g.grid, g.block, 0, st, __VA_ARGS__);                \
+    else if (tm == 127)                                                                        \
+      hipLaunchKernelGGL((K<2, 1, 1, 127>), g.grid, g.block, 0, st, __VA_ARGS__);               \
+    else if (U)                                                                                \
       hipLaunchKernelGGL((K<2, 1, 1>), g.grid, g.block, 0, st, __VA_ARGS__);                    \
     else if (g.N == 2 && am.fixed)                                                             \
       hipLaunchKernelGGL((K<2, 1, 0>), g.grid, g.block, 0, st, __VA_ARGS__);                    \
@@ -943,9 +976,15 @@ void launch_tendency_vjp(const MeshView &m, const AdjMeshView &am, const Launch 
   if (pv || a.ntr > 0) OMEGA_ADJ_LAUNCH(k_adj_pre, m, mc, a, d, s, swz);
   if (pv) OMEGA_ADJ_LAUNCH2(k_adj_pv, m, am, mc, a, d, s, swz);
   OMEGA_ADJ_LAUNCH(k_adj_vc, m, am, mc, ac, a, d, s, swz);
-#define OMEGA_ADJ_LAUNCH_X(K, X, ...)                                                           \
+  // X = 0: no tracer diffusion term (edge and cell passes without its code);
+  // the compiled term sets 7 and 15 have none, 79 and 127 have it
+#define OMEGA_ADJ_LAUNCH_X(K, X, TA, TB, ...)                                                   \
   do {                                                                                         \
-    if (U)                                                                                     \
+    if (tm == TA)                                                                              \
+      hipLaunchKernelGGL((K<2, 1, X, 1, TA>), g.grid, g.block, 0, st, __VA_ARGS__);             \
+    else if (tm == TB)                                                                         \
+      hipLaunchKernelGGL((K<2, 1, X, 1, TB>), g.grid, g.block, 0, st, __VA_ARGS__);             \
+    else if (U)                                                                                \
       hipLaunchKernelGGL((K<2, 1, X, 1>), g.grid, g.block, 0, st, __VA_ARGS__);                 \
     else if (g.N == 2 && am.fixed)                                                             \
       hipLaunchKernelGGL((K<2, 1, X, 0>), g.grid, g.block, 0, st, __VA_ARGS__);                 \
@@ -956,13 +995,12 @@ void launch_tendency_vjp(const MeshView &m, const AdjMeshView &am, const Launch 
     else                                                                                       \
       hipLaunchKernelGGL((K<1, 0, X, 0>), g.grid, g.block, 0, st, __VA_ARGS__);                 \
   } while (0)
-  // X = 0: no tracer diffusion term (edge and cell passes without its code)
   if (a.terms & (T_TDEL2 | T_TDEL4)) {
-    OMEGA_ADJ_LAUNCH_X(k_adj_edge, 1, m, am, mc, ac, a, d, s, swz);
-    OMEGA_ADJ_LAUNCH_X(k_adj_cell, 1, m, am, mc, ac, a, d, s, swz);
+    OMEGA_ADJ_LAUNCH_X(k_adj_edge, 1, 79, 127, m, am, mc, ac, a, d, s, swz);
+    OMEGA_ADJ_LAUNCH_X(k_adj_cell, 1, 79, 127, m, am, mc, ac, a, d, s, swz);
   } else {
-    OMEGA_ADJ_LAUNCH_X(k_adj_edge, 0, m, am, mc, ac, a, d, s, swz);
-    OMEGA_ADJ_LAUNCH_X(k_adj_cell, 0, m, am, mc, ac, a, d, s, swz);
+    OMEGA_ADJ_LAUNCH_X(k_adj_edge, 0, 7, 15, m, am, mc, ac, a, d, s, swz);
+    OMEGA_ADJ_LAUNCH_X(k_adj_cell, 0, 7, 15, m, am, mc, ac, a, d, s, swz);
   }
 #undef OMEGA_ADJ_LAUNCH_X
 #undef OMEGA_ADJ_LAUNCH2

@@ -573,30 +573,32 @@ def test_forward_unchanged_after_step_vjp():
 
 
 @pytest.mark.parametrize("terms,ntr", [(ALL_TERMS, 2), (1 | 2 | 4 | 8, 0), (1 | 2 | 4 | 8 | 32, 3),
-                                       (ALL_TERMS, 0)])
+                                       (ALL_TERMS, 0), (1 | 2 | 4 | 8 | 64, 2), (1 | 2 | 4, 1)])
 @pytest.mark.parametrize("L", [6, 64])
 def test_tendency_vjp_slot_uniform_form_same_bits(terms, ntr, L):
     """On the regular hexagonal mesh the VJP reads its per-slot coefficients
-    once per wavefront (kernel flag bit 2); forbidding the shared
-    coefficient form runs the general gathers.  Same expressions and the
-    same entities per thread, so the same bits, coefficient gradients
-    included."""
+    once per wavefront (kernel flag bit 2), and for the term sets 7, 15, 79
+    and 127 has the terms compiled in (forms bit 2); forbidding those forms
+    runs the run-time-terms and then the general gathers.  Same expressions
+    and the same entities per thread, so the same bits, coefficient
+    gradients included."""
     print("case", terms, ntr, L, flush=True)
     gm, gmodel, pd, state = setup(12, 10, L, ntr, terms=terms)
     cot = random_like(state, 21)
     gs, gc = upload(gm, ntr, state), upload(gm, ntr, cot)
     base = random_like(state, 22)
     outs, gps, flags = [], [], []
-    for forms in (3, 2):
+    for forms in (7, 3, 2):
         gmodel.set_kernel_forms(forms)
         out = upload(gm, ntr, base)
         gps.append(swe.tendency_vjp(gmodel, gs, gc, out))
         outs.append(out.download())
         flags.append(gmodel.kernel_flags())
-    assert flags[0] & 4 and not flags[1] & 4, flags
-    for name, a, b in zip(("h", "vn", "tr"), outs[0], outs[1]):
-        assert same_bits(a, b), name
-    assert same_bits(gps[0], gps[1])
+    assert flags[0] & 4 and flags[1] & 4 and not flags[2] & 4, flags
+    for o in outs[1:]:
+        for name, a, b in zip(("h", "vn", "tr"), outs[0], o):
+            assert same_bits(a, b), name
+    assert same_bits(gps[0], gps[1]) and same_bits(gps[0], gps[2])
     # and the slot-uniform form against the torch reference
     gmodel.set_kernel_forms(3)
     tm = ts.TorchMesh(gm, L)
