@@ -340,8 +340,9 @@ int omega_model_kernel_variant(omega_model_t model, int *in_use);
  * the tendency VJP runs its slot-uniform form (regular hexagonal
  * numbering, an even layer count, every adjoint table bit-identical at
  * each stencil slot; known after the model's first adjoint call); bit 3
- * set when the pair tendency kernel runs a form compiled for the model's
- * term set (7, 15, 79 or 127, both tendencies on).  No reference
+ * set when the pair tendency kernel (64k layers) runs a form compiled for
+ * the model's term set (7, 15, 79 or 127, both tendencies on).  No
+ * reference
  * counterpart (introspection for tests and tuning; the forward results are
  * the same bits either way, and so are the VJP's state cotangents). */
 int omega_model_kernel_flags(omega_model_t model, int *flags);

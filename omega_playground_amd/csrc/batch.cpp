@@ -136,18 +136,19 @@ void build_tables(omega_batch_s *B) {
   B->nblocks = (int)blk.size();
   B->kinds = 0;
   for (const BatchMember &g : geo) B->kinds |= g.hex ? 2 : 1;
-  // the pair members' term set, compiled into the tendency launch when they
-  // all share it (16/32-layer members keep reading theirs at run time)
+  // the members' term set, compiled into the tendency launch when they all
+  // share it (the 64k-layer members' pair operators also need the
+  // shared-coefficient, 32-bit-offset forms and the vertex-union records)
   B->tset = 0;
   if (B->kinds == 2 && N == 2) {
     int ts = 0;
     bool ok = true;
     for (int i = 0; i < nm && ok; ++i) {
       omega_model_s *M = B->mem[i]->M;
-      if (M->mesh->dm.host.nlayers % 64 != 0) continue;
       const MeshView v = M->view();
       const ModelCoef c = M->coef();
-      ok = v.coef_uni && c.uniform && v.off32 && v.term_sets && v.hexu_ok &&
+      const bool pair = M->mesh->dm.host.nlayers % 64 == 0;
+      ok = v.term_sets && (!pair || (v.coef_uni && c.uniform && v.off32 && v.hexu_ok)) &&
            !M->params.disable_h_tendency && !M->params.disable_vn_tendency &&
            term_set_batch(M->terms) && (ts == 0 || ts == M->terms);
       ts = M->terms;

@@ -31,9 +31,9 @@ void launch_batch_diag(const BatchMember *desc, const int *blk, int nblocks, int
                        int persistent, int kinds, hipStream_t s);
 void launch_batch_vdel4(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
                         int persistent, hipStream_t s);
-// tset > 0: every pattern-specialised member of a pair launch (N = 2, kinds
-// == 2) has the shared-coefficient and 32-bit-offset forms, both tendencies
-// and the term set tset, one of term_set_batch's
+// tset > 0: every member of a pattern-specialised launch (N = 2, kinds ==
+// 2) has both tendencies and the term set tset, one of term_set_batch's, and
+// every 64k-layer member the shared-coefficient and 32-bit-offset forms
 void launch_batch_tend(const BatchMember *desc, const int *blk, int nblocks, int threads, int N,
                        int persistent, int kinds, int tset, hipStream_t s);
 bool term_set_batch(int terms);

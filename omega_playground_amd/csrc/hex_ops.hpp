@@ -153,12 +153,17 @@ __device__ __forceinline__ void cell_diag_u(const MeshView &m, int terms, int nt
 // compute_vn_tendency for an edge of class K (shallow_water_model.cpp:186-217;
 // pv_flux_edge.hpp:56-84, ke_grad_edge.hpp:37-51, ssh_grad_edge.hpp:37-50,
 // vel_diff_edge.hpp:61-86, vel_hyperdiff_edge.hpp:212-237)
-template <int N, int K>
+template <int N, int K, int TM = -1>
 __device__ __forceinline__ void edge_tend_hex(const MeshView &m, const ModelCoef &mc,
                                               const StageArgs &a, const DiagOut &d, int e, int k) {
+  // TM >= 0: the term set, compiled in, with both tendencies on
+  const int terms = TM >= 0 ? TM : a.terms;
+  const bool do_vn = TM >= 0 || a.do_vn, do_h = TM >= 0 || a.do_h;
+  (void)do_vn;
+  (void)do_h;
   const int L = m.L;
   const size_t idx = (size_t)e * L + k;
-  if (!a.do_vn) {
+  if (!do_vn) {
     if (a.epi == EPI_STORE) return;
     epilogue<N>(a, a.vn, idx, tend_base<N>(a, a.vn, idx));
     return;
@@ -175,7 +180,7 @@ __device__ __forceinline__ void edge_tend_hex(const MeshView &m, const ModelCoef
   }
   constexpr int o0 = hex::OWNV[K][0], o1 = hex::OWNV[K][1];
   constexpr int q0 = hex::OWNC[K][0], q1 = hex::OWNC[K][1];
-  const bool pv = a.terms & T_PV;
+  const bool pv = terms & T_PV;
   Vd<N> hU[10], rvU[10], itU[10], vJ[10];
 #pragma unroll
   for (int u = 0; u < 10; ++u) hU[u] = rowv<N>(h, uc[u], L, k);
@@ -187,7 +192,7 @@ __device__ __forceinline__ void edge_tend_hex(const MeshView &m, const ModelCoef
       rvU[u] = rowv<N>(d.rvort_v, uv[u], L, k);
       itU[u] = rowv<N>(d.inv_thick_v, uv[u], L, k);
     }
-  } else if (a.terms & T_VDEL2) {
+  } else if (terms & T_VDEL2) {
     rvU[o0] = rowv<N>(d.rvort_v, uv[o0], L, k);
     rvU[o1] = rowv<N>(d.rvort_v, uv[o1], L, k);
   }
@@ -223,19 +228,19 @@ __device__ __forceinline__ void edge_tend_hex(const MeshView &m, const ModelCoef
       T[i] = T[i] + qt;
     }
   }
-  if (a.terms & T_KE) {
+  if (terms & T_KE) {
     const Vd<N> k0 = rowv<N>(d.ke_c, c0, L, k), k1 = rowv<N>(d.ke_c, c1, L, k);
 #pragma unroll
     for (int i = 0; i < N; ++i) T[i] = T[i] - (k1[i] - k0[i]) * idc;
   }
-  if (a.terms & T_SSH) {
+  if (terms & T_SSH) {
 #pragma unroll
     for (int i = 0; i < N; ++i) T[i] = T[i] - mc.grav * (hU[q1][i] - hU[q0][i]) * idc;
   }
-  if (a.terms & (T_VDEL2 | T_VDEL4)) {
+  if (terms & (T_VDEL2 | T_VDEL4)) {
     const int ml = m.maxlev_e[e];
     const double idv = m.inv_dv[e];
-    if (a.terms & T_VDEL2) {
+    if (terms & T_VDEL2) {
       const Vd<N> d0 = rowv<N>(d.div_c, c0, L, k), d1 = rowv<N>(d.div_c, c1, L, k);
       const double cv = mc.cv2[e];
 #pragma unroll
@@ -244,7 +249,7 @@ __device__ __forceinline__ void edge_tend_hex(const MeshView &m, const ModelCoef
         T[i] = T[i] + ((k + i < ml) ? cv : 0.0) * del2u;
       }
     }
-    if (a.terms & T_VDEL4) {
+    if (terms & T_VDEL4) {
       const int b0 = uv[o0], b1 = uv[o1];
       const Vd<N> d0 = rowv<N>(d.vdel2_div_c, c0, L, k), d1 = rowv<N>(d.vdel2_div_c, c1, L, k);
       const Vd<N> r0 = rowv<N>(d.vdel2_rv_v, b0, L, k), r1 = rowv<N>(d.vdel2_rv_v, b1, L, k);
@@ -262,9 +267,14 @@ __device__ __forceinline__ void edge_tend_hex(const MeshView &m, const ModelCoef
 // compute_h_tendency / compute_tr_tendency for one cell
 // (thickness_hadv_cell.hpp:48-72, tracer_hadv_cell.hpp:60-90,
 // tracer_diff_cell.hpp:61-92, tracer_hyperdiff_cell.hpp:131-161)
-template <int N>
+template <int N, int TM = -1>
 __device__ __forceinline__ void cell_tend_hex(const MeshView &m, const ModelCoef &mc,
                                               const StageArgs &a, const DiagOut &d, int c, int k) {
+  // TM >= 0: the term set, compiled in, with both tendencies on
+  const int terms = TM >= 0 ? TM : a.terms;
+  const bool do_vn = TM >= 0 || a.do_vn, do_h = TM >= 0 || a.do_h;
+  (void)do_vn;
+  (void)do_h;
   const int L = m.L, me = m.me, NC = m.nc;
   const double *__restrict__ h = a.h.in;
   const double *__restrict__ vn = a.vn.in;
@@ -289,7 +299,7 @@ __device__ __forceinline__ void cell_tend_hex(const MeshView &m, const ModelCoef
     for (int i = 0; i < N; ++i) fh[j][i] = 0.5 * (hN[j][i] + hC[i]);
   }
   const size_t hidx = (size_t)c * L + k;
-  if (a.do_h) {
+  if (do_h) {
     Vd<N> T = tend_base<N>(a, a.h, hidx);
 #pragma unroll
     for (int i = 0; i < N; ++i) {
@@ -304,7 +314,7 @@ __device__ __forceinline__ void cell_tend_hex(const MeshView &m, const ModelCoef
   }
   const int ntr = a.ntr;
   if (ntr <= 0) return;
-  const bool td2 = a.terms & T_TDEL2, td4 = a.terms & T_TDEL4;
+  const bool td2 = terms & T_TDEL2, td4 = terms & T_TDEL4;
   Vd<N> invC, invN[6];
   double idc[6], c2[6], s2[6], c4[6];
 #pragma unroll

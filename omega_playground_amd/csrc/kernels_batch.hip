@@ -248,10 +248,10 @@ template <int TM = -1> __device__ __forceinline__ void hex2_tend(const BatchMemb
   if (B.m.L % 64 != 0) {
     const Row r = row_of(B, lb);
     if (r.c >= B.m.nc) return;
-    hexops::edge_tend_hex<2, 0>(B.m, B.mc, B.a, B.d, 3 * r.c, r.k);
-    hexops::edge_tend_hex<2, 1>(B.m, B.mc, B.a, B.d, 3 * r.c + 1, r.k);
-    hexops::edge_tend_hex<2, 2>(B.m, B.mc, B.a, B.d, 3 * r.c + 2, r.k);
-    hexops::cell_tend_hex<2>(B.m, B.mc, B.a, B.d, r.c, r.k);
+    hexops::edge_tend_hex<2, 0, TM>(B.m, B.mc, B.a, B.d, 3 * r.c, r.k);
+    hexops::edge_tend_hex<2, 1, TM>(B.m, B.mc, B.a, B.d, 3 * r.c + 1, r.k);
+    hexops::edge_tend_hex<2, 2, TM>(B.m, B.mc, B.a, B.d, 3 * r.c + 2, r.k);
+    hexops::cell_tend_hex<2, TM>(B.m, B.mc, B.a, B.d, r.c, r.k);
     return;
   }
   const Pair q = pair_of(B, lb);
@@ -283,9 +283,10 @@ __global__ void __launch_bounds__(256)
   });
 }
 
-// TM >= 0 (pattern-specialised members only, N = 2): every 64k-layer member
-// has the shared-coefficient, 32-bit-offset forms and the term set TM with
-// both tendencies on, so their pair operators get it compiled in.
+// TM >= 0 (pattern-specialised members only, N = 2): every member has the
+// term set TM with both tendencies on, and every 64k-layer member the
+// shared-coefficient, 32-bit-offset forms, so their operators get it
+// compiled in.
 template <int N, int KINDS, bool PERS, int TM = -1>
 __global__ void __launch_bounds__(256)
     k_batch_tend(const BatchMember *__restrict__ desc, const int *__restrict__ blk, int nblocks) {

@@ -221,10 +221,13 @@ void launch_tend_hex(const MeshView &m, const Launch &lc, const ModelCoef &mc, c
     auto kern = U ? (m.off32 ? k_tend_hex2<true, false> : k_tend_hex2<true, true>)
                   : (m.off32 ? k_tend_hex2<false, false> : k_tend_hex2<false, true>);
     hipLaunchKernelGGL(kern, grid, block, 0, s, m, mc, a, d, lc.swizzle);
-  } else if (N == 2)
+  } else if (N == 2) {
+    // (compiling the term set into this form measured slower: 1.34 against
+    // 1.18 ms per RK4 step at nx=256, 32 layers)
     hipLaunchKernelGGL(k_tend_hex<2>, grid, block, 0, s, m, mc, a, d, lc.swizzle);
-  else
+  } else {
     hipLaunchKernelGGL(k_tend_hex<1>, grid, block, 0, s, m, mc, a, d, lc.swizzle);
+  }
   OMEGA_HIP_CHECK(hipGetLastError());
 }
 
