@@ -514,7 +514,7 @@ def main():
     stepper.set_graph(not args.no_graph)
     tune_ms = None
     if not args.launch and not args.no_tune:
-        tune_ms = model.autotune(state, reps=3)
+        tune_ms = model.autotune(state, reps=10)
         log(f"[bench] autotuned launch {model.get_launch()} ({tune_ms:.3f} ms per tendency)")
     log(f"[bench] {args.config}: nx={cfg['nx']} cells={nc} L={L} ntr={ntr} dt={dt:.4f}s "
         f"launch={model.get_launch()} kernels={model.kernel_variant()} graph={not args.no_graph}")
