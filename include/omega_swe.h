@@ -315,12 +315,15 @@ int omega_model_get_launch(omega_model_t model, int *rows_per_block, int *levels
                            int *xcd_swizzle, int *levels_per_block);
 /* Times reps tendency evaluations of state for each launch shape (cells per
  * block: the default fill, 4 or 2; levels per thread 2 or 1; XCD swizzle on
- * or off), keeps the fastest; *best_ms = its time per evaluation.  With the
- * environment variable OMEGA_TUNE_CACHE naming a file, a choice recorded
- * there for the same model shape (cells, layers, tracers, terms, kernel
- * family) is applied without timing, and a new choice is appended.  No
- * reference counterpart (Kokkos picks its tiles, src/common.hpp:78-104);
- * results are the same bits for every shape. */
+ * or off), keeps the fastest; *best_ms = its time per evaluation; then
+ * times the diagnostics launch alone for its own cells per block (as the
+ * tendency launch, the default fill, 4 or 2).  With the environment
+ * variable OMEGA_TUNE_CACHE naming a file, a choice recorded there for the
+ * same model shape (cells, layers, tracers, terms, kernel family) is
+ * applied without timing, and a new choice is appended.
+ * omega_model_set_launch makes the diagnostics follow the tendency launch
+ * again.  No reference counterpart (Kokkos picks its tiles,
+ * src/common.hpp:78-104); results are the same bits for every shape. */
 int omega_model_autotune(omega_model_t model, omega_state_t state, int reps, double *best_ms,
                          omega_stream_t stream);
 

@@ -688,6 +688,7 @@ int omega_model_set_launch(omega_model_t M, int rows, int lpt, int swizzle, int 
     }
     if (swizzle >= 0) M->lc.swizzle = swizzle ? 1 : 0;
     if (chunk >= 0) M->lc.chunk = chunk;
+    M->lc.drows = -1; // the diagnostics launch follows
     M->touch();
   });
 }
@@ -718,13 +719,15 @@ int omega_model_autotune(omega_model_t M, omega_state_t state, int reps, double 
       while (std::getline(in, line)) {
         std::istringstream ls(line);
         std::string key;
-        int r = 0, l = 0, w = 0;
+        int r = 0, l = 0, w = 0, dr = -1;
         double ms = -1;
         if (!(ls >> key >> r >> l >> w >> ms) || key != sig) continue;
+        if (!(ls >> dr) || dr < -1 || dr > 256) dr = -1;
         if (r >= 0 && r <= 256 && (l == 1 || (l == 2 && hm.nlayers % 2 == 0))) {
           M->lc.rows = r;
           M->lc.lpt = l;
           M->lc.swizzle = w ? 1 : 0;
+          M->lc.drows = dr;
           M->touch();
           if (best_ms) *best_ms = ms;
           return;
@@ -746,6 +749,7 @@ int omega_model_autotune(omega_model_t M, omega_state_t state, int reps, double 
           M->lc.rows = rows;
           M->lc.lpt = lpt;
           M->lc.swizzle = swz;
+          M->lc.drows = -1;
           compute_tendency_ptrs(M, t0.p, t1.p, t2.p, state->ptr(0), state->ptr(1),
                                 state->ptr(2), 0.0, OMEGA_ADD_REPLACE, s, 7);
           OMEGA_HIP_CHECK(hipEventRecord(a, s));
@@ -763,6 +767,28 @@ int omega_model_autotune(omega_model_t M, omega_state_t state, int reps, double 
           }
         }
     }
+    // then the diagnostics launch's own block height, timed alone
+    {
+      double dbest = -1;
+      int dpick = -1;
+      for (int dr : {-1, 0, 4, 2}) {
+        if (dr >= 0 && dr == pick.rows) continue;
+        M->lc = pick;
+        M->lc.drows = dr;
+        run_diag(M, state->ptr(0), state->ptr(1), state->ptr(2), s);
+        OMEGA_HIP_CHECK(hipEventRecord(a, s));
+        for (int i = 0; i < reps; ++i) run_diag(M, state->ptr(0), state->ptr(1), state->ptr(2), s);
+        OMEGA_HIP_CHECK(hipEventRecord(b, s));
+        OMEGA_HIP_CHECK(hipEventSynchronize(b));
+        float ms = 0;
+        OMEGA_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+        if (dbest < 0 || ms < dbest) {
+          dbest = ms;
+          dpick = dr;
+        }
+      }
+      pick.drows = dpick;
+    }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     M->lc = pick;
@@ -771,7 +797,7 @@ int omega_model_autotune(omega_model_t M, omega_state_t state, int reps, double 
       std::ofstream out(cache, std::ios::app);
       if (out)
         out << sig << " " << pick.rows << " " << pick.lpt << " " << pick.swizzle << " " << best
-            << "\n";
+            << " " << pick.drows << "\n";
     }
     if (best_ms) *best_ms = best;
   });

@@ -177,6 +177,7 @@ struct Launch {
   int swizzle = 1;   // XCD-contiguous block order
   int chunk = 0;     // levels per block (0: all levels, up to 64 lanes)
   int variant = 0;   // 0: pattern-specialised kernels when the mesh allows, 1: general
+  int drows = -1;    // entities per block of the diagnostics launch (-1: rows)
 };
 
 // Enabled-term bits (same meaning as omega_model_set_terms).

@@ -119,9 +119,11 @@ void run_diag_into(omega_model_s *M, const Real *h, const Real *vn, const Real *
   const int ntr = M->params.ntracers;
   const bool do_vn = !M->params.disable_vn_tendency;
   const int dterms = do_vn ? M->terms : (M->terms & T_TDEL4);
+  Launch ld = M->lc; // the diagnostics launch may have its own block height
+  if (ld.drows >= 0) ld.rows = ld.drows;
   if (hex_usable(mv, M->lc) && !general_only(M))
-    launch_diag_hex(mv, M->lc, dterms, ntr, h, vn, tr, d, s);
-  else launch_diag(mv, M->lc, M->coef(), dterms, ntr, h, vn, tr, d, s);
+    launch_diag_hex(mv, ld, dterms, ntr, h, vn, tr, d, s);
+  else launch_diag(mv, ld, M->coef(), dterms, ntr, h, vn, tr, d, s);
   if (do_vn && (M->terms & T_VDEL4)) launch_vdel4(mv, M->lc, d, s);
 }
 

@@ -264,3 +264,31 @@ def test_batch_compiled_term_set(kind, nstages, forms):
     for i, (s, r) in enumerate(zip(states, ref)):
         for name, a, c in zip(("h", "vn", "tr"), s.download(), r):
             assert same_bits(a, c), (i, name)
+
+
+@pytest.mark.parametrize("rows,drows", [(4, 2), (2, 0), (0, 4)])
+def test_model_tuned_launch_from_cache_keeps_results(tmp_path, monkeypatch, rows, drows):
+    """A recorded model choice (OMEGA_TUNE_CACHE: rows per block, levels per
+    thread, swizzle and the diagnostics launch's own rows) is applied without
+    timing; tendencies and steps under it stay the oracle's bits."""
+    nx, L, ntr, dc = 12, 64, 2, 10e3
+    pd = params_dict(dc, ntr)
+    om = po.Mesh(nx, nx, dc, L)
+    omodel = po.Model(om, **pd)
+    gm = swe.PlanarHexagonalMesh(nx, nx, dc, L)
+    gmodel = swe.ShallowWaterModel(gm, swe.ShallowWaterParams(**pd))
+    cache = tmp_path / "tune.txt"
+    cache.write_text(f"model;{gm.ncells}:{L}:{ntr}:{gmodel.terms}:h {rows} 2 1 0.5 {drows}\n")
+    monkeypatch.setenv("OMEGA_TUNE_CACHE", str(cache))
+    y0 = rich_state(om, L, ntr, 4)
+    gs = state_of(gm, ntr, y0)
+    assert gmodel.autotune(gs, reps=1) == pytest.approx(0.5)
+    assert gmodel.get_launch()[:3] == (rows, 2, 1)
+    print("case", rows, drows, flush=True)
+    st = swe.RK4Stepper(gmodel)
+    st.set_graph(True)
+    st.run(0.0, 20.0, 2, gs)
+    ref = tuple(a.copy() for a in y0)
+    po.Stepper(omodel, "rk4").run(0.0, 20.0, 2, ref)
+    for a, c in zip(gs.download(), ref):
+        assert same_bits(a, c)
