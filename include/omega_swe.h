@@ -344,14 +344,19 @@ int omega_model_kernel_variant(omega_model_t model, int *in_use);
  * numbering, an even layer count, every adjoint table bit-identical at
  * each stencil slot; known after the model's first adjoint call); bit 3
  * set when the pair tendency kernel (64k layers) runs a form compiled for
- * the model's term set (7, 15, 79 or 127, both tendencies on).  No
- * reference
- * counterpart (introspection for tests and tuning; the forward results are
- * the same bits either way, and so are the VJP's state cotangents). */
+ * the model's term set (7, 15, 79 or 127, both tendencies on); bit 4 set
+ * when a step adjoint's stage recomputation also writes the rows its
+ * tendency VJPs linearise about (edge PV sums, thickness fluxes and their
+ * edges-on-edge sums; the one-cell-per-row kernels at 16 or 32 layers), so
+ * those VJPs skip their pre-pass.  No reference counterpart (introspection
+ * for tests and tuning; the forward results are the same bits either way,
+ * and so are the VJP's state cotangents). */
 int omega_model_kernel_flags(omega_model_t model, int *flags);
 /* Which of those forms the model may use (bit 0 shared coefficients, bit 1
- * 32-bit offsets, bit 2 the compiled-in term sets; default 7); clearing a
- * bit forces the per-cell, 64-bit or run-time-terms form (tests, tuning). */
+ * 32-bit offsets, bit 2 the compiled-in term sets, bit 3 the VJP rows
+ * written by the stage recomputation; default 15); clearing a bit forces
+ * the per-cell, 64-bit, run-time-terms or separate pre-pass form (tests,
+ * tuning). */
 int omega_model_set_kernel_forms(omega_model_t model, int forms);
 /* ----------------------------------------------------------------------
  * Time steppers (src/time_integration/time_stepper.hpp:8-38)

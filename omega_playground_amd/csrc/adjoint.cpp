@@ -197,9 +197,10 @@ AdjScratch scratch(omega_model_s *M) {
 // out (+)= J(y)^T k for the tendency at state y (pointers in device numbering;
 // epi says how the result is combined);
 // the mixing-coefficient gradient accumulates in M->a_pgrad.
+// pre: the rows the stage kernel wrote at y (the pre-pass is skipped)
 void tendency_vjp_ptrs(omega_model_s *M, const Real *const y[3], const Real *const k[3],
                        Real *const out[3], hipStream_t s, const AdjEpi &epi = AdjEpi{},
-                       bool have_diag = false) {
+                       bool have_diag = false, const omega_model_s::PreSet *pre = nullptr) {
   if (!have_diag) run_diag(M, y[0], y[1], y[2], s);
   AdjArgs a{};
   a.terms = M->terms;
@@ -214,8 +215,14 @@ void tendency_vjp_ptrs(omega_model_s *M, const Real *const y[3], const Real *con
   a.nparts = M->a_nparts;
   a.epi = epi;
   const AdjCoef ac{M->ect2.p, M->ect4.p, M->ect_uniform ? 1 : 0};
-  launch_tendency_vjp(M->view(), M->mesh->adj.view(), M->lc, M->coef(), ac, a,
-                      M->diag(), scratch(M), s);
+  AdjScratch sc = scratch(M);
+  if (pre) {
+    sc.se_e = pre->se.p;
+    sc.fu_e = pre->fu.p;
+    sc.own_e = pre->own.p;
+  }
+  launch_tendency_vjp(M->view(), M->mesh->adj.view(), M->lc, M->coef(), ac, a, M->diag(), sc,
+                      s, pre != nullptr);
 }
 
 void zero_pgrad(omega_model_s *M, hipStream_t s) { launch_fill(M->a_pgrad.p, 0.0, 4, s); }
@@ -262,6 +269,25 @@ void ensure_diag_sets(omega_model_s *M, int nsets) {
   }
 }
 
+// The tendency VJP's rows at stage inputs 0..nsets-1 when the stage kernels
+// write them; nullptr entries otherwise.
+std::array<const omega_model_s::PreSet *, 4> ensure_pre_sets(omega_model_s *M, int nsets) {
+  std::array<const omega_model_s::PreSet *, 4> out{};
+  if (!stage_writes_pre(M)) return out;
+  const size_t n = (size_t)M->mesh->dm.host.nedges * M->mesh->dm.host.nlayers;
+  for (int i = 0; i < nsets; ++i) {
+    omega_model_s::PreSet &p = M->pre_sets[i];
+    for (DBuf<Real> *b : {&p.se, &p.fu, &p.own})
+      if (b->n != n) b->alloc(n);
+    out[i] = &p;
+  }
+  return out;
+}
+
+PreOut pre_out(const omega_model_s::PreSet *p) {
+  return p ? PreOut{p->se.p, p->fu.p, p->own.p} : PreOut{};
+}
+
 // Selects the diagnostic set for the duration of a step adjoint; every
 // other caller always sees set 0.
 struct DiagSlotScope {
@@ -303,6 +329,8 @@ void step_vjp_ptrs(omega_stepper_s *St, const Real *const y[3], Real *const cot[
   DiagSlotScope slots(M);
   if (St->kind == OMEGA_STEPPER_RK4) {
     ensure_diag_sets(M, 3);
+    // the stage kernels at P0..P2 also write the VJP's rows there, where they can
+    const auto pre = ensure_pre_sets(M, 3);
     // stage inputs P1..P3 (P_{i+1} = y + dt*a_i*k_i), as rk4.cpp:168-189
     for (int st = 0; st < 3; ++st) {
       slots.use(st);
@@ -314,7 +342,8 @@ void step_vjp_ptrs(omega_stepper_s *St, const Real *const y[3], Real *const cot[
       a.h = fio(in[0], nullptr, nullptr, y[0], out[0]);
       a.vn = fio(in[1], nullptr, nullptr, y[1], out[1]);
       a.tr = fio(in[2], nullptr, nullptr, y[2], out[2]);
-      run_stage(M, a, s);
+      const PreOut po = pre_out(pre[st]);
+      run_stage(M, a, s, 7, true, pre[st] ? &po : nullptr);
     }
     // reverse sweep with gy = the incoming cotangent (left in place until the
     // last VJP): acc = gy + sum_s J(P_s)^T kbar_s and, in the same kernels,
@@ -339,7 +368,8 @@ void step_vjp_ptrs(omega_stepper_s *St, const Real *const y[3], Real *const cot[
         if (st == 3) epi.base[f] = gy[f];
       }
       slots.use(st);
-      tendency_vjp_ptrs(M, ps.data(), kc.data(), acc.data(), s, epi, st < 3);
+      tendency_vjp_ptrs(M, ps.data(), kc.data(), acc.data(), s, epi, st < 3,
+                        st < 3 ? pre[st] : nullptr);
       std::swap(kin, kout);
     }
     // cot = acc + J(y)^T kbar_0
@@ -347,7 +377,7 @@ void step_vjp_ptrs(omega_stepper_s *St, const Real *const y[3], Real *const cot[
     AdjEpi last{};
     for (int f = 0; f < 3; ++f) last.base[f] = acc[f];
     slots.use(0);
-    tendency_vjp_ptrs(M, Y.data(), kc.data(), C.data(), s, last, true);
+    tendency_vjp_ptrs(M, Y.data(), kc.data(), C.data(), s, last, true, pre[0]);
   } else {
     // stage inputs Y_1..Y_{S-1} (lsrk.cpp:36-89: tend = a_s*tend + F(Y_s),
     // Y_{s+1} = Y_s + dt*b_s*tend)
@@ -355,6 +385,7 @@ void step_vjp_ptrs(omega_stepper_s *St, const Real *const y[3], Real *const cot[
     OMEGA_REQUIRE(S >= 1 && S <= 5, "step adjoint: unsupported number of stages");
     const auto tb = P(St->adj_t);
     ensure_diag_sets(M, S - 1);
+    const auto pre = ensure_pre_sets(M, S - 1);
     for (int st = 0; st + 1 < S; ++st) {
       slots.use(st);
       StageArgs a{};
@@ -367,7 +398,8 @@ void step_vjp_ptrs(omega_stepper_s *St, const Real *const y[3], Real *const cot[
       a.h = fio(in[0], tb[0], nullptr, nullptr, out[0]);
       a.vn = fio(in[1], tb[1], nullptr, nullptr, out[1]);
       a.tr = fio(in[2], tb[2], nullptr, nullptr, out[2]);
-      run_stage(M, a, s);
+      const PreOut po = pre_out(pre[st]);
+      run_stage(M, a, s, 7, true, pre[st] ? &po : nullptr);
     }
     // Tbar_s = dt*b_s*Ybar_{s+1} + a_{s+1}*Tbar_{s+1};  Ybar_s = Ybar_{s+1} + J(Y_s)^T Tbar_s.
     // The VJP at Y_s writes Ybar_s and, in the same kernels, Tbar_{s-1} =
@@ -396,7 +428,8 @@ void step_vjp_ptrs(omega_stepper_s *St, const Real *const y[3], Real *const cot[
       }
       Real *const *out = (last && !first) ? C.data() : acc.data();
       slots.use(st);
-      tendency_vjp_ptrs(M, ys.data(), kc.data(), out, s, epi, st + 1 < S);
+      tendency_vjp_ptrs(M, ys.data(), kc.data(), out, s, epi, st + 1 < S,
+                        st + 1 < S ? pre[st] : nullptr);
       std::swap(kin, kout);
     }
     if (S == 1) copy_fields(C.data(), K(St->adj_gy).data(), n, s);

@@ -84,6 +84,8 @@ struct AdjScratch {
   Real *pb_e;     // [ne][L] sum over t listing e of w_te kbar_t
   Real *xa_e;     // [ntr][ne][L] tracer advection share of each cell of the edge,
                   // -(cd0 b0 + cd1 b1) F_e u_e / 2 (no tracer diffusion only)
+  Real *own_e;    // [ne][L] sum_j w_ej F_j u_j over the edge's edges-on-edge (given
+                  // with pre rows only; otherwise the PV pass forms it)
 };
 
 // What the VJP does with its result v = vs * J^T k for each state element:
@@ -110,9 +112,13 @@ struct AdjArgs {
   AdjEpi epi;
 };
 
+// pre: se_e, fu_e and own_e of s already hold the rows at the linearisation
+// state (written by the stage kernel there), so the pre-pass is skipped and
+// the PV pass reads own_e
 void launch_tendency_vjp(const MeshView &m, const AdjMeshView &am, const Launch &lc,
                          const ModelCoef &mc, const AdjCoef &ac, const AdjArgs &a,
-                         const DiagOut &d, const AdjScratch &s, hipStream_t st);
+                         const DiagOut &d, const AdjScratch &s, hipStream_t st,
+                         bool pre = false);
 int adj_partial_slots(const MeshView &m, const Launch &lc);
 
 // y[i] = v

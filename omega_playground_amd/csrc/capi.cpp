@@ -823,7 +823,7 @@ int omega_model_kernel_variant(omega_model_t M, int *in_use) {
 int omega_model_set_kernel_forms(omega_model_t M, int forms) {
   return guarded([&] {
     OMEGA_REQUIRE(M, "null model");
-    OMEGA_REQUIRE(forms >= 0 && forms <= 7, "forms must be 0..7");
+    OMEGA_REQUIRE(forms >= 0 && forms <= 15, "forms must be 0..15");
     M->forms = forms;
     M->touch();
   });
@@ -844,7 +844,7 @@ int omega_model_kernel_flags(omega_model_t M, int *flags) {
                       M->lc.lpt != 1 && mv.L % 64 == 0 && hex_term_set(M->terms) &&
                       !M->params.disable_h_tendency && !M->params.disable_vn_tendency;
     *flags = (mv.coef_uni && mc.uniform ? 1 : 0) | (mv.off32 ? 2 : 0) | (adj_uni ? 4 : 0) |
-             (tset ? 8 : 0);
+             (tset ? 8 : 0) | (stage_writes_pre(M) ? 16 : 0);
   });
 }
 

@@ -153,9 +153,13 @@ __device__ __forceinline__ void cell_diag_u(const MeshView &m, int terms, int nt
 // compute_vn_tendency for an edge of class K (shallow_water_model.cpp:186-217;
 // pv_flux_edge.hpp:56-84, ke_grad_edge.hpp:37-51, ssh_grad_edge.hpp:37-50,
 // vel_diff_edge.hpp:61-86, vel_hyperdiff_edge.hpp:212-237)
-template <int N, int K, int TM = -1>
+// EM: also write the edge's PreOut rows (its PV sum and flux-weighted
+// edges-on-edge sum with the PV term on, its thickness flux always), formed
+// by the same expressions as the tendency VJP's own pre-pass
+template <int N, int K, int TM = -1, bool EM = false>
 __device__ __forceinline__ void edge_tend_hex(const MeshView &m, const ModelCoef &mc,
-                                              const StageArgs &a, const DiagOut &d, int e, int k) {
+                                              const StageArgs &a, const DiagOut &d, int e, int k,
+                                              const PreOut &po = PreOut{}) {
   // TM >= 0: the term set, compiled in, with both tendencies on
   const int terms = TM >= 0 ? TM : a.terms;
   const bool do_vn = TM >= 0 || a.do_vn, do_h = TM >= 0 || a.do_h;
@@ -213,10 +217,11 @@ __device__ __forceinline__ void edge_tend_hex(const MeshView &m, const ModelCoef
     double w[10];
 #pragma unroll
     for (int j = 0; j < 10; ++j) w[j] = m.w[e * 10 + j];
+    Vd<N> S, O;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       const double qs = 0.5 * (nrvU[o0][i] + nrvU[o1][i]) + 0.5 * (npvU[o0][i] + npvU[o1][i]);
-      double qt = 0.0;
+      double qt = 0.0, ow = 0.0;
 #pragma unroll
       for (int j = 0; j < 10; ++j) {
         const double fh = 0.5 * (hU[hex::LC0[K][j]][i] + hU[hex::LC1[K][j]][i]);
@@ -224,9 +229,23 @@ __device__ __forceinline__ void edge_tend_hex(const MeshView &m, const ModelCoef
         const double npv = 0.5 * (npvU[hex::LV0[K][j]][i] + npvU[hex::LV1[K][j]][i]);
         const double nv = (qs + nrv + npv) * 0.5;
         qt = qt + w[j] * fh * vJ[j][i] * nv;
+        if constexpr (EM) ow = ow + w[j] * (fh * vJ[j][i]);
       }
       T[i] = T[i] + qt;
+      S[i] = qs;
+      O[i] = ow;
     }
+    if constexpr (EM) {
+      st<N>(po.se + idx, S);
+      st<N>(po.own + idx, O);
+    }
+  }
+  if constexpr (EM) {
+    const Vd<N> ue = rowv<N>(vn, e, L, k);
+    Vd<N> FU;
+#pragma unroll
+    for (int i = 0; i < N; ++i) FU[i] = 0.5 * (hU[q0][i] + hU[q1][i]) * ue[i];
+    st<N>(po.fu + idx, FU);
   }
   if (terms & T_KE) {
     const Vd<N> k0 = rowv<N>(d.ke_c, c0, L, k), k1 = rowv<N>(d.ke_c, c1, L, k);

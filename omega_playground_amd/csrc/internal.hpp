@@ -259,14 +259,24 @@ struct LinCoef {
 };
 void launch_tend_linear(const MeshView &m, const Launch &lc, const LinCoef &lin,
                         const StageArgs &a, hipStream_t s);
+// Rows a tendency VJP reads at its linearisation state ([ne][L] each): the
+// edge PV sum Q_e + P_e, the edge thickness flux F_e u_e and the flux-weighted
+// edges-on-edge sum sum_j w_ej F_j u_j.  A stage kernel that evaluates the
+// tendency at that state can write them (the adjoint's stage recomputation).
+struct PreOut {
+  Real *se, *fu, *own;
+};
 // Pattern-specialised kernels for the regular hexagonal mesh (kernels_hex.hip)
 bool hex_usable(const MeshView &m, const Launch &lc);
+// launch_tend_hex writes PreOut rows for this mesh and launch shape (the
+// one-cell-per-row tendency kernel; its vn tendency must be on)
+bool hex_writes_pre(const MeshView &m, const Launch &lc);
 // the pair tendency kernel has a compiled-in form for this term set
 bool hex_term_set(int terms);
 void launch_diag_hex(const MeshView &m, const Launch &lc, int terms, int ntr, const Real *h,
                      const Real *vn, const Real *tr, const DiagOut &d, hipStream_t s);
 void launch_tend_hex(const MeshView &m, const Launch &lc, const ModelCoef &mc, const StageArgs &a,
-                     const DiagOut &d, hipStream_t s);
+                     const DiagOut &d, hipStream_t s, const PreOut *pre = nullptr);
 // y += alpha*x elementwise
 void launch_axpy(Real *y, Real alpha, const Real *x, size_t n, hipStream_t s);
 // y = x + alpha*z elementwise

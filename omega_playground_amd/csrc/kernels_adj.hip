@@ -292,8 +292,10 @@ __global__ void __launch_bounds__(kAB)
 // needs: A_e = sum_t w_te kbar_t S_t, B_e = sum_t w_te kbar_t over the edges
 // t whose edges-on-edge list holds e (kbar = velocity-tendency cotangent).
 // This pass keeps its per-edge tables on every mesh: a slot-uniform form
-// (weights and lists read per class) measured slower on MI355X.
-template <int N, int H>
+// (weights and lists read per class) measured slower on MI355X.  O: the
+// flux-weighted edges-on-edge sum is given (s.own_e, written by the stage
+// kernel at the linearisation state) instead of gathered here.
+template <int N, int H, int O = 0>
 __global__ void __launch_bounds__(kAB)
     k_adj_pv(MeshView m, AdjMeshView am, ModelCoef mc, AdjArgs a, DiagOut d, AdjScratch s,
              int swz) {
@@ -304,14 +306,18 @@ __global__ void __launch_bounds__(kAB)
     const Vd<N> fue = rd<N>(s.fu_e, e, L, k);
     const Vd<N> ute = rd<N>(a.c.vn, e, L, k);
     Vd<N> own = zero<N>(), pa = zero<N>(), pb = zero<N>();
-    const int ne2 = H ? 10 : m.noe[e];
+    if constexpr (O) {
+      own = rd<N>(s.own_e, e, L, k);
+    } else {
+      const int ne2 = H ? 10 : m.noe[e];
 #pragma unroll
-    for (int j = 0; j < ne2; ++j) {
-      const int ej = m.eoe[e * m.me2 + j];
-      const double w = m.w[e * m.me2 + j];
-      const Vd<N> fu = rd<N>(s.fu_e, ej, L, k);
+      for (int j = 0; j < ne2; ++j) {
+        const int ej = m.eoe[e * m.me2 + j];
+        const double w = m.w[e * m.me2 + j];
+        const Vd<N> fu = rd<N>(s.fu_e, ej, L, k);
 #pragma unroll
-      for (int i = 0; i < N; ++i) own[i] += w * fu[i];
+        for (int i = 0; i < N; ++i) own[i] += w * fu[i];
+      }
     }
     const int q0 = am.tofs[e], nq = H ? 10 : am.tofs[e + 1] - q0;
 #pragma unroll
@@ -922,7 +928,7 @@ void launch_sum(const Real *partial, int n, Real *out, int accumulate, hipStream
 
 void launch_tendency_vjp(const MeshView &m, const AdjMeshView &am, const Launch &lc,
                          const ModelCoef &mc, const AdjCoef &ac, const AdjArgs &a,
-                         const DiagOut &d, const AdjScratch &s, hipStream_t st) {
+                         const DiagOut &d, const AdjScratch &s, hipStream_t st, bool pre) {
   TimerScope ts("adjoint", st);
   const AGeo g = adj_geometry(m, lc);
   const int swz = lc.swizzle;
@@ -972,9 +978,22 @@ void launch_tendency_vjp(const MeshView &m, const AdjMeshView &am, const Launch 
     OMEGA_ADJ_LAUNCH2(k_adj_v4a, m, mc, a, s, swz);
     OMEGA_ADJ_LAUNCH2(k_adj_v4b, m, am, s, swz);
   }
-  // edge PV sums (PV term) and thickness fluxes (PV term, tracer advection)
-  if (pv || a.ntr > 0) OMEGA_ADJ_LAUNCH(k_adj_pre, m, mc, a, d, s, swz);
-  if (pv) OMEGA_ADJ_LAUNCH2(k_adj_pv, m, am, mc, a, d, s, swz);
+  // edge PV sums (PV term) and thickness fluxes (PV term, tracer advection),
+  // unless the stage kernel at this state wrote them
+  if (!pre && (pv || a.ntr > 0)) OMEGA_ADJ_LAUNCH(k_adj_pre, m, mc, a, d, s, swz);
+  if (pv && pre) {
+    OMEGA_REQUIRE(s.own_e != nullptr, "tendency VJP: pre rows without the edges-on-edge sum");
+    if (g.N == 2 && am.fixed)
+      hipLaunchKernelGGL((k_adj_pv<2, 1, 1>), g.grid, g.block, 0, st, m, am, mc, a, d, s, swz);
+    else if (g.N == 2)
+      hipLaunchKernelGGL((k_adj_pv<2, 0, 1>), g.grid, g.block, 0, st, m, am, mc, a, d, s, swz);
+    else if (am.fixed)
+      hipLaunchKernelGGL((k_adj_pv<1, 1, 1>), g.grid, g.block, 0, st, m, am, mc, a, d, s, swz);
+    else
+      hipLaunchKernelGGL((k_adj_pv<1, 0, 1>), g.grid, g.block, 0, st, m, am, mc, a, d, s, swz);
+  } else if (pv) {
+    OMEGA_ADJ_LAUNCH2(k_adj_pv, m, am, mc, a, d, s, swz);
+  }
   OMEGA_ADJ_LAUNCH(k_adj_vc, m, am, mc, ac, a, d, s, swz);
   // X = 0: no tracer diffusion term (edge and cell passes without its code);
   // the compiled term sets 7 and 15 have none, 79 and 127 have it

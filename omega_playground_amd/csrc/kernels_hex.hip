@@ -68,18 +68,19 @@ __global__ void __launch_bounds__(256)
   cell_diag_p<U, W>(m, terms, ntr, cA, cB, hi, k, h, vn, tr, d);
 }
 
-template <int N>
+// EM: the edges also write their PreOut rows (po)
+template <int N, bool EM = false>
 __global__ void __launch_bounds__(256)
-    k_tend_hex(MeshView m, ModelCoef mc, StageArgs a, DiagOut d, int swz) {
+    k_tend_hex(MeshView m, ModelCoef mc, StageArgs a, DiagOut d, int swz, PreOut po) {
   const int s = logical_block(blockIdx.x, gridDim.x, swz);
   const int R = blockDim.y;
   const int c = ent<N>(s * R + (int)threadIdx.y);
   if (c >= m.nc) return;
   const int k = (blockIdx.y * blockDim.x + threadIdx.x) * N;
   const int e0 = 3 * c;
-  edge_tend_hex<N, 0>(m, mc, a, d, e0, k);
-  edge_tend_hex<N, 1>(m, mc, a, d, e0 + 1, k);
-  edge_tend_hex<N, 2>(m, mc, a, d, e0 + 2, k);
+  edge_tend_hex<N, 0, -1, EM>(m, mc, a, d, e0, k, po);
+  edge_tend_hex<N, 1, -1, EM>(m, mc, a, d, e0 + 1, k, po);
+  edge_tend_hex<N, 2, -1, EM>(m, mc, a, d, e0 + 2, k, po);
   cell_tend_hex<N>(m, mc, a, d, c, k);
 }
 
@@ -180,6 +181,8 @@ int hex_n(const MeshView &m, const Launch &lc) { return lc.lpt == 1 ? 1 : 2; }
 
 bool hex_term_set(int terms) { return tend_term_set(terms, -1) != nullptr; }
 
+bool hex_writes_pre(const MeshView &m, const Launch &lc) { return !hex_pair(m, hex_n(m, lc)); }
+
 bool hex_usable(const MeshView &m, const Launch &lc) {
   // 16 or 32 layers: several cells per wavefront, two levels per lane
   const bool rows_ok = m.L % 64 == 0 || ((m.L == 16 || m.L == 32) && lc.lpt != 1);
@@ -205,12 +208,15 @@ void launch_diag_hex(const MeshView &m, const Launch &lc, int terms, int ntr, co
 }
 
 void launch_tend_hex(const MeshView &m, const Launch &lc, const ModelCoef &mc, const StageArgs &a,
-                     const DiagOut &d, hipStream_t s) {
+                     const DiagOut &d, hipStream_t s, const PreOut *pre) {
   TimerScope ts("tend", s);
   dim3 block;
   const int N = hex_n(m, lc);
   const dim3 grid = hex_grid(m, lc, N, block);
   const bool U = m.coef_uni && mc.uniform;
+  OMEGA_REQUIRE(!pre || (!hex_pair(m, N) && a.do_vn),
+                "tendency launch: this kernel form does not write the VJP rows");
+  const PreOut po = pre ? *pre : PreOut{};
   if (hex_pair(m, N) && m.hexu_ok) {
     auto kern = U ? (m.off32 ? k_tend_hex2v<true, false> : k_tend_hex2v<true, true>)
                   : (m.off32 ? k_tend_hex2v<false, false> : k_tend_hex2v<false, true>);
@@ -224,9 +230,11 @@ void launch_tend_hex(const MeshView &m, const Launch &lc, const ModelCoef &mc, c
   } else if (N == 2) {
     // (compiling the term set into this form measured slower: 1.34 against
     // 1.18 ms per RK4 step at nx=256, 32 layers)
-    hipLaunchKernelGGL(k_tend_hex<2>, grid, block, 0, s, m, mc, a, d, lc.swizzle);
+    auto kern = pre ? k_tend_hex<2, true> : k_tend_hex<2, false>;
+    hipLaunchKernelGGL(kern, grid, block, 0, s, m, mc, a, d, lc.swizzle, po);
   } else {
-    hipLaunchKernelGGL(k_tend_hex<1>, grid, block, 0, s, m, mc, a, d, lc.swizzle);
+    auto kern = pre ? k_tend_hex<1, true> : k_tend_hex<1, false>;
+    hipLaunchKernelGGL(kern, grid, block, 0, s, m, mc, a, d, lc.swizzle, po);
   }
   OMEGA_HIP_CHECK(hipGetLastError());
 }

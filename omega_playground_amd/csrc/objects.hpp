@@ -91,10 +91,11 @@ struct omega_model_s {
   // f_v, and cv2/cv4/ct2/ct4 at every stencil slot, bit-identical across
   // the mesh (ModelCoef::uniform)
   bool f_uniform = false, mix_uniform = false;
-  // pair-kernel forms the model may use (bit 0 shared coefficients, bit 1
-  // 32-bit offsets, bit 2 compiled-in term sets); clearing bits forces the
-  // general forms (tests, tuning)
-  int forms = 7;
+  // kernel forms the model may use (bit 0 shared coefficients, bit 1
+  // 32-bit offsets, bit 2 compiled-in term sets, bit 3 VJP rows written by
+  // a step adjoint's stage recomputation); clearing bits forces the general
+  // forms (tests, tuning)
+  int forms = 15;
   // the mesh view this model's kernels get
   omega_hip::MeshView view() const {
     omega_hip::MeshView v = mesh->dm.view();
@@ -115,6 +116,12 @@ struct omega_model_s {
   };
   DiagSet diag_sets[4];
   int diag_slot = 0;
+  // step adjoint: the tendency VJP's rows at stage inputs 0..3, written by
+  // the stage recomputation where its kernel can (stage_writes_pre)
+  struct PreSet {
+    DBuf<Real> se, fu, own;
+  };
+  PreSet pre_sets[4];
   DiagOut diag() {
     if (diag_slot > 0) {
       DiagSet &d = diag_sets[diag_slot - 1];
@@ -195,8 +202,11 @@ void model_build_linear(omega_model_s *M);
 void run_diag(omega_model_s *M, const Real *h, const Real *vn, const Real *tr, hipStream_t s);
 // fields: bit 0 h, bit 1 vn, bit 2 tracers (all by default); with_diag =
 // false reuses the diagnostics of the last run_diag
+// pre: also write the tendency VJP's rows at the stage input (only where
+// stage_writes_pre says the model's stage kernel can)
 void run_stage(omega_model_s *M, StageArgs a, hipStream_t s, int fields = 7,
-               bool with_diag = true);
+               bool with_diag = true, const omega_hip::PreOut *pre = nullptr);
+bool stage_writes_pre(omega_model_s *M);
 // true when the model's terms need the general kernels (bottom drag)
 bool general_only(const omega_model_s *M);
 // true when the enabled terms read diagnostics (vn tendency, tracer del4)

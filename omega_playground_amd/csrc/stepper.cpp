@@ -161,7 +161,14 @@ void run_stage_into(omega_model_s *M, StageArgs a, hipStream_t s, int fields,
 // One tendency evaluation (+ optional fused stage update).  with_diag =
 // false reads the diagnostics an earlier run_diag left in the model's
 // buffers (the per-field tendency calls after compute_aux).
-void run_stage(omega_model_s *M, StageArgs a, hipStream_t s, int fields, bool with_diag) {
+bool stage_writes_pre(omega_model_s *M) {
+  const MeshView mv = M->view();
+  return (M->forms & 8) && !M->linear && !general_only(M) &&
+         !M->params.disable_vn_tendency && hex_usable(mv, M->lc) && hex_writes_pre(mv, M->lc);
+}
+
+void run_stage(omega_model_s *M, StageArgs a, hipStream_t s, int fields, bool with_diag,
+               const PreOut *pre) {
   const MeshView mv = M->view();
   const ModelCoef mc = M->coef();
   a.terms = M->terms;
@@ -174,11 +181,13 @@ void run_stage(omega_model_s *M, StageArgs a, hipStream_t s, int fields, bool wi
   }
   if (with_diag) run_diag(M, a.h.in, a.vn.in, a.tr.in, s);
   const DiagOut d = M->diag();
+  OMEGA_REQUIRE(!pre || (fields == 7 && stage_writes_pre(M)),
+                "stage: this model's stage kernel does not write the VJP rows");
   if (general_only(M)) {
     launch_tend(mv, M->lc, mc, a, d, s);
     return;
   }
-  if (hex_usable(mv, M->lc)) launch_tend_hex(mv, M->lc, mc, a, d, s);
+  if (hex_usable(mv, M->lc)) launch_tend_hex(mv, M->lc, mc, a, d, s, pre);
   else launch_tend(mv, M->lc, mc, a, d, s);
 }
 

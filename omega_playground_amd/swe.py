@@ -562,15 +562,18 @@ class ShallowWaterModel:
         """bit 0: coefficient tables uniform per stencil slot (shared by a
         wavefront's two cells); bit 1: 32-bit row offsets; bit 2: the
         tendency VJP's slot-uniform form (after the first adjoint call);
-        bit 3: the pair tendency kernel compiled for the model's term set."""
+        bit 3: the pair tendency kernel compiled for the model's term set;
+        bit 4: a step adjoint's stage recomputation writes the rows its
+        tendency VJPs linearise about."""
         v = C.c_int()
         check(lib().omega_model_kernel_flags(self._h, C.byref(v)))
         return v.value
 
-    def set_kernel_forms(self, forms: int = 7):
+    def set_kernel_forms(self, forms: int = 15):
         """Allow (bits set) or forbid the shared-coefficient (bit 0),
         32-bit-offset (bit 1) and compiled-in term-set (bit 2) forms of the
-        pair kernels."""
+        pair kernels, and a step adjoint's stage recomputation writing the
+        rows its tendency VJPs linearise about (bit 3)."""
         check(lib().omega_model_set_kernel_forms(self._h, int(forms)))
 
     def kernel_variant(self) -> int:

@@ -608,3 +608,74 @@ def test_tendency_vjp_slot_uniform_form_same_bits(terms, ntr, L):
         if b.size:
             assert rel_err(a - b0, b) <= TOL_REF, name
     check_pgrad(gps[0], refp, TOL_REF)
+
+
+@pytest.mark.parametrize("terms,ntr", [(ALL_TERMS, 2), (1 | 2 | 4 | 8, 0),
+                                       (1 | 2 | 4 | 8 | 64, 2), (ADVECTION_ONLY, 1),
+                                       (2 | 4 | 8 | 32, 1)])
+@pytest.mark.parametrize("L,lpt", [(16, 2), (32, 2), (64, 1)])
+@pytest.mark.parametrize("kind,nstages", [("rk4", 4), ("lsrk", 5), ("lsrk", 2)])
+def test_step_vjp_stage_written_rows_same_bits(kind, nstages, L, lpt, terms, ntr):
+    """Where the one-cell-per-row tendency kernel runs (16 or 32 layers, or
+    one level per lane) a step adjoint's stage recomputation also writes the
+    rows its tendency VJPs linearise about (kernel flag bit 4: edge PV sums,
+    thickness fluxes and their edges-on-edge sums), and those VJPs skip their
+    pre-pass.  With that form forbidden (forms bit 3) the pre-pass forms the
+    same rows: same bits for the state cotangents and the coefficient
+    gradients."""
+    print("case", kind, nstages, L, lpt, terms, ntr, flush=True)
+    gm, gmodel, pd, state = setup(12, 10, L, ntr, seed=terms % 7, terms=terms)
+    gmodel.set_launch(0, lpt)
+    dt = 30.0
+    cot = random_like(state, 51)
+    outs, gps, flags = [], [], []
+    for forms in (15, 7):
+        gmodel.set_kernel_forms(forms)
+        st = make_stepper(gmodel, kind, nstages)
+        gc = upload(gm, ntr, cot)
+        gps.append(swe.step_vjp(st, 0.0, dt, upload(gm, ntr, state), gc))
+        outs.append(gc.download())
+        flags.append(gmodel.kernel_flags())
+    assert flags[0] & 16 and not flags[1] & 16, flags
+    for name, a, b in zip(("h", "vn", "tr"), *outs):
+        assert same_bits(a, b), name
+    assert same_bits(gps[0], gps[1])
+
+
+def test_stage_written_rows_off_where_not_written():
+    """The rows are written only by the one-cell-per-row kernel with the vn
+    tendency on: not by the 64-layer pair kernel, not with the vn tendency
+    disabled, not on the general kernels."""
+    gm, gmodel, pd, state = setup(8, 8, 64, 1)
+    assert not gmodel.kernel_flags() & 16
+    gmodel.set_launch(0, 1)
+    assert gmodel.kernel_flags() & 16
+    gm, gmodel, pd, state = setup(8, 8, 32, 1)
+    assert gmodel.kernel_flags() & 16
+    gmodel.set_variant(1)
+    assert not gmodel.kernel_flags() & 16
+    pd = params_dict(DC, 1)
+    pd["disable_vn_tendency"] = True
+    gmodel = swe.ShallowWaterModel(gm, swe.ShallowWaterParams(**pd))
+    assert not gmodel.kernel_flags() & 16
+
+
+@pytest.mark.parametrize("kind,nstages", [("rk4", 4), ("lsrk", 5)])
+def test_checkpointed_gradient_stage_written_rows_same_bits(kind, nstages):
+    """A checkpointed 6-step gradient at 32 layers gives the same J, state
+    gradient and coefficient gradient bits with the stage-written VJP rows
+    as with the pre-pass."""
+    gm, gmodel, pd, state = setup(12, 10, 32, 2, seed=9)
+    obs = random_like(state, 61)
+    res = []
+    for forms in (15, 7):
+        gmodel.set_kernel_forms(forms)
+        st = make_stepper(gmodel, kind, nstages)
+        grad = swe.ShallowWaterState(gm, 2)
+        J, gp = swe.adjoint_run(st, 0.0, 30.0, 6, upload(gm, 2, state), upload(gm, 2, obs), grad,
+                                (1.0, 0.5, 1.0), n_snapshots=3)
+        res.append((J, gp, grad.download()))
+    assert res[0][0] == res[1][0]
+    assert same_bits(res[0][1], res[1][1])
+    for name, a, b in zip(("h", "vn", "tr"), res[0][2], res[1][2]):
+        assert same_bits(a, b), name
