@@ -423,7 +423,9 @@ int omega_model_tendency_vjp(omega_model_t model, omega_state_t state, omega_sta
                              omega_stream_t stream);
 
 /* cot <- (d y_{n+1} / d y_n)^T cot for one do_step(t, dt) from y_n = state.
- * The stage states are recomputed from state with the forward kernels. */
+ * The stage states are recomputed from state with the forward kernels.
+ * param_grad[0..3] += the mixing-coefficient gradient (the call waits for
+ * the stream to read it); param_grad = NULL: not read back, no wait. */
 int omega_stepper_step_vjp(omega_stepper_t stepper, double t, double dt, omega_state_t state,
                            omega_state_t cot, double *param_grad, omega_stream_t stream);
 

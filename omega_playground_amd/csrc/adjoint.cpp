@@ -197,10 +197,13 @@ AdjScratch scratch(omega_model_s *M) {
 // out (+)= J(y)^T k for the tendency at state y (pointers in device numbering;
 // epi says how the result is combined);
 // the mixing-coefficient gradient accumulates in M->a_pgrad.
-// pre: the rows the stage kernel wrote at y (the pre-pass is skipped)
+// pre: the rows the stage kernel wrote at y (the pre-pass is skipped);
+// part: bit 0 adds the coefficient-gradient partials to those of an earlier
+// VJP, bit 1 leaves them for a later VJP to reduce (AdjArgs::part_add/keep)
 void tendency_vjp_ptrs(omega_model_s *M, const Real *const y[3], const Real *const k[3],
                        Real *const out[3], hipStream_t s, const AdjEpi &epi = AdjEpi{},
-                       bool have_diag = false, const omega_model_s::PreSet *pre = nullptr) {
+                       bool have_diag = false, const omega_model_s::PreSet *pre = nullptr,
+                       int part = 0) {
   if (!have_diag) run_diag(M, y[0], y[1], y[2], s);
   AdjArgs a{};
   a.terms = M->terms;
@@ -214,6 +217,8 @@ void tendency_vjp_ptrs(omega_model_s *M, const Real *const y[3], const Real *con
   a.c = CotIO{k[0], k[1], k[2], out[0], out[1], out[2]};
   a.nparts = M->a_nparts;
   a.epi = epi;
+  a.part_add = part & 1;
+  a.part_keep = (part >> 1) & 1;
   const AdjCoef ac{M->ect2.p, M->ect4.p, M->ect_uniform ? 1 : 0};
   AdjScratch sc = scratch(M);
   if (pre) {
@@ -368,8 +373,9 @@ void step_vjp_ptrs(omega_stepper_s *St, const Real *const y[3], Real *const cot[
         if (st == 3) epi.base[f] = gy[f];
       }
       slots.use(st);
+      // the four VJPs' coefficient-gradient partials are reduced once, by the last
       tendency_vjp_ptrs(M, ps.data(), kc.data(), acc.data(), s, epi, st < 3,
-                        st < 3 ? pre[st] : nullptr);
+                        st < 3 ? pre[st] : nullptr, st == 3 ? 2 : 3);
       std::swap(kin, kout);
     }
     // cot = acc + J(y)^T kbar_0
@@ -377,7 +383,7 @@ void step_vjp_ptrs(omega_stepper_s *St, const Real *const y[3], Real *const cot[
     AdjEpi last{};
     for (int f = 0; f < 3; ++f) last.base[f] = acc[f];
     slots.use(0);
-    tendency_vjp_ptrs(M, Y.data(), kc.data(), C.data(), s, last, true, pre[0]);
+    tendency_vjp_ptrs(M, Y.data(), kc.data(), C.data(), s, last, true, pre[0], 1);
   } else {
     // stage inputs Y_1..Y_{S-1} (lsrk.cpp:36-89: tend = a_s*tend + F(Y_s),
     // Y_{s+1} = Y_s + dt*b_s*tend)
@@ -428,8 +434,9 @@ void step_vjp_ptrs(omega_stepper_s *St, const Real *const y[3], Real *const cot[
       }
       Real *const *out = (last && !first) ? C.data() : acc.data();
       slots.use(st);
+      // the S VJPs' coefficient-gradient partials are reduced once, by the last
       tendency_vjp_ptrs(M, ys.data(), kc.data(), out, s, epi, st + 1 < S,
-                        st + 1 < S ? pre[st] : nullptr);
+                        st + 1 < S ? pre[st] : nullptr, (first ? 0 : 1) | (last ? 0 : 2));
       std::swap(kin, kout);
     }
     if (S == 1) copy_fields(C.data(), K(St->adj_gy).data(), n, s);

@@ -110,6 +110,12 @@ struct AdjArgs {
   CotIO c;
   int nparts;               // partial slots available per parameter
   AdjEpi epi;
+  // coefficient-gradient partials shared by the VJPs of one step adjoint:
+  // part_add adds vs * this VJP's block sums to the slots an earlier VJP
+  // filled instead of storing them, part_keep leaves them for a later VJP
+  // to reduce (no reduction launch); with either set the block sums carry
+  // the vs scaling and the reduction adds them unscaled
+  int part_add, part_keep;
 };
 
 // pre: se_e, fu_e and own_e of s already hold the rows at the linearisation

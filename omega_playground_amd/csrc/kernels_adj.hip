@@ -43,8 +43,17 @@ __device__ __forceinline__ Vd<N> rd(const double *__restrict__ p, int e, int L, 
   return ld<N>(p + (size_t)e * L + k);
 }
 
+// A block's sum into its partial slot: stored, or scaled and stored, or
+// scaled and added (AdjArgs::part_add / part_keep)
+__device__ __forceinline__ void put_partial(const AdjArgs &g, double x, double *out) {
+  if (!(g.part_add || g.part_keep)) *out = x;
+  else if (g.part_add) *out = *out + g.epi.vs * x;
+  else *out = g.epi.vs * x;
+}
+
 // Block-wide sum of two values into two partial slots (all threads call it).
-__device__ void block_partials2(double a, double b, double *out_a, double *out_b) {
+__device__ void block_partials2(const AdjArgs &g, double a, double b, double *out_a,
+                                double *out_b) {
   __shared__ double sa[kAB], sb[kAB];
   const int t = threadIdx.y * blockDim.x + threadIdx.x;
   const int n = blockDim.x * blockDim.y;
@@ -59,13 +68,13 @@ __device__ void block_partials2(double a, double b, double *out_a, double *out_b
     __syncthreads();
   }
   if (t == 0) {
-    *out_a = sa[0];
-    *out_b = sb[0];
+    put_partial(g, sa[0], out_a);
+    put_partial(g, sb[0], out_b);
   }
 }
 
 // Block-wide sum of one value into one partial slot (all threads call it).
-__device__ void block_partial(double a, double *out) {
+__device__ void block_partial(const AdjArgs &g, double a, double *out) {
   __shared__ double sa[kAB];
   const int t = threadIdx.y * blockDim.x + threadIdx.x;
   const int n = blockDim.x * blockDim.y;
@@ -75,7 +84,7 @@ __device__ void block_partial(double a, double *out) {
     if (t < s && t + s < n) sa[t] = sa[t] + sa[t + s];
     __syncthreads();
   }
-  if (t == 0) *out = sa[0];
+  if (t == 0) put_partial(g, sa[0], out);
 }
 
 struct Seg {
@@ -486,7 +495,7 @@ __global__ void __launch_bounds__(kAB)
     }
   }
   const int b = blockIdx.y * gridDim.x + blockIdx.x;
-  block_partial(p_v2, s.partial + b);
+  block_partial(a, p_v2, s.partial + b);
 }
 
 // ---------------------------------------------------------------------------
@@ -614,7 +623,7 @@ __global__ void __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4)))
     });
   }
   const int b = blockIdx.y * gridDim.x + blockIdx.x;
-  block_partial(p_v4, s.partial + a.nparts + b);
+  block_partial(a, p_v4, s.partial + a.nparts + b);
 }
 
 // ---------------------------------------------------------------------------
@@ -784,7 +793,7 @@ __global__ void __launch_bounds__(kAB)
     }
   }
   const int b = blockIdx.y * gridDim.x + blockIdx.x;
-  block_partials2(p_t2, p_t4, s.partial + 2 * a.nparts + b, s.partial + 3 * a.nparts + b);
+  block_partials2(a, p_t2, p_t4, s.partial + 2 * a.nparts + b, s.partial + 3 * a.nparts + b);
 }
 
 // pgrad[p] += sum of the block partials of parameter p, in a fixed order
@@ -1024,8 +1033,9 @@ void launch_tendency_vjp(const MeshView &m, const AdjMeshView &am, const Launch 
 #undef OMEGA_ADJ_LAUNCH_X
 #undef OMEGA_ADJ_LAUNCH2
 #undef OMEGA_ADJ_LAUNCH
-  hipLaunchKernelGGL(k_adj_param_reduce, dim3(4), dim3(kAB), 0, st, s.partial, a.nparts,
-                     (int)(g.grid.x * g.grid.y), a.epi.vs, s.pgrad);
+  if (!a.part_keep)
+    hipLaunchKernelGGL(k_adj_param_reduce, dim3(4), dim3(kAB), 0, st, s.partial, a.nparts,
+                       (int)(g.grid.x * g.grid.y), a.part_add ? 1.0 : a.epi.vs, s.pgrad);
   OMEGA_HIP_CHECK(hipGetLastError());
 }
 

@@ -684,9 +684,15 @@ def tendency_vjp(model: ShallowWaterModel, state: ShallowWaterState, tend_cot: S
 
 
 def step_vjp(stepper: "_Stepper", t: float, dt: float, state: ShallowWaterState,
-             cot: ShallowWaterState, stream=None) -> np.ndarray:
+             cot: ShallowWaterState, stream=None, pgrad: bool = True):
     """cot <- (dy_{n+1}/dy_n)^T cot for one step from state; returns the
-    coefficient gradient contribution."""
+    coefficient gradient contribution (read back to the host, which waits
+    for the stream), or None with pgrad=False (the call then returns
+    without waiting)."""
+    if not pgrad:
+        check(lib().omega_stepper_step_vjp(stepper._h, float(t), float(dt), state._h, cot._h,
+                                           None, _stream(stream)))
+        return None
     buf, ptr = _pgrad_out([0.0] * 4)
     check(lib().omega_stepper_step_vjp(stepper._h, float(t), float(dt), state._h, cot._h, ptr,
                                        _stream(stream)))

@@ -679,3 +679,20 @@ def test_checkpointed_gradient_stage_written_rows_same_bits(kind, nstages):
     assert same_bits(res[0][1], res[1][1])
     for name, a, b in zip(("h", "vn", "tr"), res[0][2], res[1][2]):
         assert same_bits(a, b), name
+
+
+def test_step_vjp_without_coefficient_read_back():
+    """step_vjp with pgrad=False does not read the coefficient gradient back
+    (no wait for the stream); the state cotangent has the same bits."""
+    gm, gmodel, pd, state = setup(8, 8, 32, 2, seed=3)
+    st = swe.RK4Stepper(gmodel)
+    cot = random_like(state, 71)
+    outs = []
+    for pg in (True, False):
+        gc = upload(gm, 2, cot)
+        r = swe.step_vjp(st, 0.0, 30.0, upload(gm, 2, state), gc, pgrad=pg)
+        assert (r is None) == (not pg)
+        swe.synchronize()
+        outs.append(gc.download())
+    for name, a, b in zip(("h", "vn", "tr"), *outs):
+        assert same_bits(a, b), name
