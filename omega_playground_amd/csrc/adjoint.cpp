@@ -167,8 +167,7 @@ void ensure_adjoint(omega_model_s *M) {
   if (!M->a_sbar.p) {
     M->a_se.alloc(NE * L);
     M->a_fu.alloc(NE * L);
-    M->a_pa.alloc(NE * L);
-    M->a_pb.alloc(NE * L);
+    M->a_px.alloc(NE * L);
     if (ntr) M->a_xa.alloc(ntr * NE * L);
     M->a_sbar.alloc(NE * L);
     M->a_fbar.alloc(NE * L);
@@ -191,7 +190,7 @@ AdjScratch scratch(omega_model_s *M) {
   return AdjScratch{M->a_sbar.p, M->a_zbar.p,  M->a_tbar.p,  M->a_kbar.p,
                     M->a_dbar.p, M->a_gbar.p,  M->a_fbar.p,  M->a_d2bar.p,
                     M->a_z2bar.p, M->a_e2bar.p, M->a_partial.p, M->a_pgrad.p,
-                    M->a_se.p, M->a_fu.p, M->a_pa.p, M->a_pb.p, M->a_xa.p};
+                    M->a_se.p, M->a_fu.p, M->a_px.p, M->a_xa.p};
 }
 
 // out (+)= J(y)^T k for the tendency at state y (pointers in device numbering;

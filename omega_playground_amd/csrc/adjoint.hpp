@@ -80,8 +80,8 @@ struct AdjScratch {
   Real *pgrad;    // [4] accumulated parameter gradient
   Real *se_e;     // [ne][L] edge PV sum Q_e + P_e at the linearisation state
   Real *fu_e;     // [ne][L] edge thickness flux F_e u_e at the linearisation state
-  Real *pa_e;     // [ne][L] sum over t listing e of w_te kbar_t S_t
-  Real *pb_e;     // [ne][L] sum over t listing e of w_te kbar_t
+  Real *px_e;     // [ne][L] (A_e + S_e B_e)/2 with A_e = sum_t w_te kbar_t S_t and
+                  // B_e = sum_t w_te kbar_t over the edges t listing e
   Real *xa_e;     // [ntr][ne][L] tracer advection share of each cell of the edge,
                   // -(cd0 b0 + cd1 b1) F_e u_e / 2 (no tracer diffusion only)
   Real *own_e;    // [ne][L] sum_j w_ej F_j u_j over the edge's edges-on-edge (given

@@ -297,9 +297,10 @@ __global__ void __launch_bounds__(kAB)
   });
 }
 
-// PV-sum cotangent of each edge, and the two transposed sums the edge pass
-// needs: A_e = sum_t w_te kbar_t S_t, B_e = sum_t w_te kbar_t over the edges
-// t whose edges-on-edge list holds e (kbar = velocity-tendency cotangent).
+// PV-sum cotangent of each edge, and the edge pass's PV share
+// (A_e + S_e B_e)/2 from the two transposed sums A_e = sum_t w_te kbar_t S_t,
+// B_e = sum_t w_te kbar_t over the edges t whose edges-on-edge list holds e
+// (kbar = velocity-tendency cotangent).
 // This pass keeps its per-edge tables on every mesh: a slot-uniform form
 // (weights and lists read per class) measured slower on MI355X.  O: the
 // flux-weighted edges-on-edge sum is given (s.own_e, written by the stage
@@ -342,12 +343,15 @@ __global__ void __launch_bounds__(kAB)
         pa[i] += wu * St[i];
       }
     }
-    Vd<N> out;
+    const Vd<N> Se = rd<N>(s.se_e, e, L, k);
+    Vd<N> out, px;
 #pragma unroll
-    for (int i = 0; i < N; ++i) out[i] = 0.5 * (ute[i] * own[i] + pb[i] * fue[i]);
+    for (int i = 0; i < N; ++i) {
+      out[i] = 0.5 * (ute[i] * own[i] + pb[i] * fue[i]);
+      px[i] = 0.5 * (pa[i] + Se[i] * pb[i]);
+    }
     st<N>(s.sbar_e + (size_t)e * L + k, out);
-    st<N>(s.pa_e + (size_t)e * L + k, pa);
-    st<N>(s.pb_e + (size_t)e * L + k, pb);
+    st<N>(s.px_e + (size_t)e * L + k, px);
   }
 }
 
@@ -580,14 +584,12 @@ __global__ void __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4)))
       }
       if (do_vn) {
         if (pv) {
-          // sum_t kbar_t w_te (S_t + S_e)/2 = (A_e + S_e B_e)/2 (k_adj_pre, k_adj_pv)
-          const Vd<N> Se = rd<N>(s.se_e, e, L, k);
-          const Vd<N> pa = rd<N>(s.pa_e, e, L, k), pb = rd<N>(s.pb_e, e, L, k);
+          // sum_t kbar_t w_te (S_t + S_e)/2 = (A_e + S_e B_e)/2 (k_adj_pv)
+          const Vd<N> px = rd<N>(s.px_e, e, L, k);
 #pragma unroll
           for (int i = 0; i < N; ++i) {
-            const double x = 0.5 * (pa[i] + Se[i] * pb[i]);
-            ub[i] += x * F[i];
-            fb[i] += x * ue[i];
+            ub[i] += px[i] * F[i];
+            fb[i] += px[i] * ue[i];
           }
         }
         if (pv || vd2 || vd4) {

@@ -80,7 +80,7 @@ struct omega_model_s {
   // first use
   DBuf<Real> ect2, ect4;
   bool ect_uniform = false;
-  DBuf<Real> a_se, a_fu, a_pa, a_pb, a_xa;
+  DBuf<Real> a_se, a_fu, a_px, a_xa;
   DBuf<Real> a_sbar, a_zbar, a_tbar, a_kbar, a_dbar, a_gbar, a_fbar, a_d2bar, a_z2bar, a_e2bar,
       a_partial, a_pgrad, a_mpart;
   int a_nparts = 0;
