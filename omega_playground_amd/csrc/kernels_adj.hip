@@ -92,12 +92,27 @@ struct Seg {
   bool active;
 };
 
+// The grid is one-dimensional over (cell block, level chunk) pairs, mapped
+// as in the forward kernels: with a multiple of 8 chunks the chunk runs
+// fastest (XCD x handles chunks x, x + 8, ... of every cell block),
+// otherwise slowest with the optional swizzle over cell blocks.
 template <int N> __device__ __forceinline__ Seg segment(const MeshView &m, int swz) {
   Seg g;
-  const int s = logical_block(blockIdx.x, gridDim.x, swz);
+  const int span = (int)blockDim.x * N;
+  const int nch = (m.L + span - 1) / span;
+  int s, ch;
+  if (nch % 8 == 0) {
+    s = __builtin_amdgcn_readfirstlane((int)blockIdx.x / nch);
+    ch = (int)blockIdx.x - s * nch;
+  } else {
+    const int ncb = (int)gridDim.x / nch;
+    const int lb = logical_block(blockIdx.x, gridDim.x, swz);
+    ch = __builtin_amdgcn_readfirstlane(lb / ncb);
+    s = lb - ch * ncb;
+  }
   g.c0 = s * (int)blockDim.y;
   g.c1 = min(g.c0 + (int)blockDim.y, m.nc);
-  g.k = (blockIdx.y * blockDim.x + threadIdx.x) * N;
+  g.k = (ch * (int)blockDim.x + (int)threadIdx.x) * N;
   g.active = g.c0 < m.nc && g.k < m.L;
   return g;
 }
@@ -827,7 +842,7 @@ AGeo adj_geometry(const MeshView &m, const Launch &lc) {
   while (lx < lanes) lx <<= 1;
   const int rows = kAB / lx;
   const int chunks = (m.L + lx * N - 1) / (lx * N);
-  return AGeo{dim3((m.nc + rows - 1) / rows, chunks), dim3(lx, rows), N};
+  return AGeo{dim3((m.nc + rows - 1) / rows * chunks), dim3(lx, rows), N};
 }
 
 } // namespace
