@@ -347,8 +347,8 @@ int omega_model_kernel_variant(omega_model_t model, int *in_use);
  * the model's term set (7, 15, 79 or 127, both tendencies on); bit 4 set
  * when a step adjoint's stage recomputation also writes the rows its
  * tendency VJPs linearise about (edge PV sums, thickness fluxes and their
- * edges-on-edge sums; the one-cell-per-row kernels at 16 or 32 layers), so
- * those VJPs skip their pre-pass.  No reference counterpart (introspection
+ * edges-on-edge sums; the one-cell-per-row kernels at 16 or 32 layers, and
+ * the pair kernel with bits 0 and 1), so those VJPs skip their pre-pass.  No reference counterpart (introspection
  * for tests and tuning; the forward results are the same bits either way,
  * and so are the VJP's state cotangents). */
 int omega_model_kernel_flags(omega_model_t model, int *flags);

@@ -777,12 +777,14 @@ __device__ __forceinline__ void diag_union_p(const MeshView &m, int terms, int n
 // Vertex-union form of edge_tend_hex2: the PV products of the cell's 16
 // union vertices (hex_pattern.hpp, namespace hexu) are formed once per cell
 // by the caller and shared by its three edges; the rows per edge are then
-// 10 thicknesses and 10 velocities.  Same values, same expressions.
-template <int K, bool U, bool W, int TM = -1, int EK = -1>
+// 10 thicknesses and 10 velocities.  Same values, same expressions.  EM: also
+// write the edge's PreOut rows, as edge_tend_hex does.
+template <int K, bool U, bool W, int TM = -1, int EK = -1, bool EM = false>
 __device__ __forceinline__ void edge_tend_hex2v(const MeshView &m, const ModelCoef &mc,
                                                 const StageArgs &a, const DiagOut &d, int eA,
                                                 int eB, bool hi, int k, const Vd<2> (&P)[hexu::kNV],
-                                                const Vd<2> (&Q)[hexu::kNV]) {
+                                                const Vd<2> (&Q)[hexu::kNV],
+                                                const PreOut &po = PreOut{}) {
   constexpr int N = 2;
   const int L = m.L;
   // TM >= 0: the term set is TM and both tendencies are on (compiled in)
@@ -824,10 +826,11 @@ __device__ __forceinline__ void edge_tend_hex2v(const MeshView &m, const ModelCo
     double w[10];
 #pragma unroll
     for (int j = 0; j < 10; ++j) w[j] = cpick<U>(m.w, (long)eA * 10 + j, (long)eB * 10 + j, hi);
+    Vd<N> S, O;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       const double qs = 0.5 * (P[v0][i] + P[v1][i]) + 0.5 * (Q[v0][i] + Q[v1][i]);
-      double qt = 0.0;
+      double qt = 0.0, ow = 0.0;
 #pragma unroll
       for (int j = 0; j < 10; ++j) {
         const double fh = 0.5 * (hU[hex::LC0[K][j]][i] + hU[hex::LC1[K][j]][i]);
@@ -835,9 +838,23 @@ __device__ __forceinline__ void edge_tend_hex2v(const MeshView &m, const ModelCo
         const double npv = 0.5 * (Q[hexu::TV0[K][j]][i] + Q[hexu::TV1[K][j]][i]);
         const double nv = (qs + nrv + npv) * 0.5;
         qt = qt + w[j] * fh * vJ[j][i] * nv;
+        if constexpr (EM) ow = ow + w[j] * (fh * vJ[j][i]);
       }
       T[i] = T[i] + qt;
+      S[i] = qs;
+      O[i] = ow;
     }
+    if constexpr (EM) {
+      st<N>(po.se + idx, S);
+      st<N>(po.own + idx, O);
+    }
+  }
+  if constexpr (EM) {
+    const Vd<N> ue = rowv<N, W>(vn, e, L, k);
+    Vd<N> FU;
+#pragma unroll
+    for (int i = 0; i < N; ++i) FU[i] = 0.5 * (hU[q0][i] + hU[q1][i]) * ue[i];
+    st<N>(po.fu + idx, FU);
   }
   if (terms & T_KE) {
     const Vd<N> k0 = rowv<N, W>(d.ke_c, c0, L, k), k1 = rowv<N, W>(d.ke_c, c1, L, k);

@@ -268,9 +268,10 @@ struct PreOut {
 };
 // Pattern-specialised kernels for the regular hexagonal mesh (kernels_hex.hip)
 bool hex_usable(const MeshView &m, const Launch &lc);
-// launch_tend_hex writes PreOut rows for this mesh and launch shape (the
-// one-cell-per-row tendency kernel; its vn tendency must be on)
-bool hex_writes_pre(const MeshView &m, const Launch &lc);
+// launch_tend_hex writes PreOut rows for this mesh, launch shape and
+// coefficients (the one-cell-per-row tendency kernel, or the pair kernel's
+// shared-coefficient 32-bit vertex-union form; its vn tendency must be on)
+bool hex_writes_pre(const MeshView &m, const Launch &lc, const ModelCoef &mc);
 // the pair tendency kernel has a compiled-in form for this term set
 bool hex_term_set(int terms);
 void launch_diag_hex(const MeshView &m, const Launch &lc, int terms, int ntr, const Real *h,

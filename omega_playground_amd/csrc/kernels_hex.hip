@@ -133,9 +133,10 @@ __global__ void __launch_bounds__(256)
 // resolved at compile time (no run-time term branches between the stencil
 // phases, so the compiler schedules across them); E >= 0: the stage
 // epilogue likewise; -1: read from a.
-template <bool U, bool W, int TM = -1, int E = -1>
-__global__ void __launch_bounds__(256)
-    k_tend_hex2v(MeshView m, ModelCoef mc, StageArgs a, DiagOut d, int swz) {
+template <bool U, bool W, int TM, int E, bool EM>
+__device__ __forceinline__ void tend_hex2v(const MeshView &m, const ModelCoef &mc,
+                                           const StageArgs &a, const DiagOut &d, int swz,
+                                           const PreOut &po) {
   int s;
   const int k = block_rows(m, swz, 2, s);
   const int R = blockDim.y;
@@ -149,10 +150,25 @@ __global__ void __launch_bounds__(256)
   // keeps the compiler from hoisting the edges' loads above the union PV
   // products, which measured 6 % slower (1.957 against 1.850 ms at nx=512)
   if (a.do_vn && (a.terms & T_PV)) union_pv2<U, W>(m, mc, d, cA, cB, hi, k, P, Q);
-  edge_tend_hex2v<0, U, W, TM, E>(m, mc, a, d, 3 * cA, 3 * cB, hi, k, P, Q);
-  edge_tend_hex2v<1, U, W, TM, E>(m, mc, a, d, 3 * cA + 1, 3 * cB + 1, hi, k, P, Q);
-  edge_tend_hex2v<2, U, W, TM, E>(m, mc, a, d, 3 * cA + 2, 3 * cB + 2, hi, k, P, Q);
+  edge_tend_hex2v<0, U, W, TM, E, EM>(m, mc, a, d, 3 * cA, 3 * cB, hi, k, P, Q, po);
+  edge_tend_hex2v<1, U, W, TM, E, EM>(m, mc, a, d, 3 * cA + 1, 3 * cB + 1, hi, k, P, Q, po);
+  edge_tend_hex2v<2, U, W, TM, E, EM>(m, mc, a, d, 3 * cA + 2, 3 * cB + 2, hi, k, P, Q, po);
   cell_tend_hex2<U, W, TM, E>(m, mc, a, d, cA, cB, hi, k);
+}
+
+template <bool U, bool W, int TM = -1, int E = -1>
+__global__ void __launch_bounds__(256)
+    k_tend_hex2v(MeshView m, ModelCoef mc, StageArgs a, DiagOut d, int swz, PreOut po) {
+  tend_hex2v<U, W, TM, E, false>(m, mc, a, d, swz, po);
+}
+
+// The same with the VJP rows written (the adjoint's stage recomputation,
+// run-time epilogue); held at two waves per SIMD, where the stores' extra
+// registers would otherwise cost half the occupancy
+template <int TM>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+    k_tend_hex2v_pre(MeshView m, ModelCoef mc, StageArgs a, DiagOut d, int swz, PreOut po) {
+  tend_hex2v<true, false, TM, -1, true>(m, mc, a, d, swz, po);
 }
 
 // Term sets compiled into the shared-coefficient, 32-bit-offset form:
@@ -160,7 +176,7 @@ __global__ void __launch_bounds__(256)
 // viscosity and tracer del4 (79), everything (127); each also with the
 // epilogue of a time step's stages compiled in (RK4 first, middle and last
 // stage, low-storage RK stage)
-using TendFn = void (*)(MeshView, ModelCoef, StageArgs, DiagOut, int);
+using TendFn = void (*)(MeshView, ModelCoef, StageArgs, DiagOut, int, PreOut);
 template <int TM> TendFn tend_for_epilogue(int epi) {
   switch (epi) {
   case EPI_RK4_FIRST: return k_tend_hex2v<true, false, TM, EPI_RK4_FIRST>;
@@ -168,6 +184,17 @@ template <int TM> TendFn tend_for_epilogue(int epi) {
   case EPI_RK4_LAST: return k_tend_hex2v<true, false, TM, EPI_RK4_LAST>;
   case EPI_LSRK: return k_tend_hex2v<true, false, TM, EPI_LSRK>;
   default: return k_tend_hex2v<true, false, TM>;
+  }
+}
+// the forms that also write the VJP rows (run-time epilogue: the adjoint's
+// stage recomputation)
+TendFn tend_writing_pre(int terms, bool term_sets) {
+  switch (term_sets ? terms : -1) {
+  case 7: return k_tend_hex2v_pre<7>;
+  case 15: return k_tend_hex2v_pre<15>;
+  case 79: return k_tend_hex2v_pre<79>;
+  case 127: return k_tend_hex2v_pre<127>;
+  default: return k_tend_hex2v_pre<-1>;
   }
 }
 TendFn tend_term_set(int terms, int epi) {
@@ -204,7 +231,11 @@ int hex_n(const MeshView &m, const Launch &lc) { return lc.lpt == 1 ? 1 : 2; }
 
 bool hex_term_set(int terms) { return tend_term_set(terms, -1) != nullptr; }
 
-bool hex_writes_pre(const MeshView &m, const Launch &lc) { return !hex_pair(m, hex_n(m, lc)); }
+bool hex_writes_pre(const MeshView &m, const Launch &lc, const ModelCoef &mc) {
+  // the one-cell-per-row kernels, and the pair kernel's vertex-union form with
+  // shared coefficients and 32-bit offsets
+  return !hex_pair(m, hex_n(m, lc)) || (m.hexu_ok && m.coef_uni && mc.uniform && m.off32);
+}
 
 bool hex_usable(const MeshView &m, const Launch &lc) {
   // 16 or 32 layers: several cells per wavefront, two levels per lane
@@ -237,7 +268,7 @@ void launch_tend_hex(const MeshView &m, const Launch &lc, const ModelCoef &mc, c
   const int N = hex_n(m, lc);
   const dim3 grid = hex_grid(m, lc, N, block);
   const bool U = m.coef_uni && mc.uniform;
-  OMEGA_REQUIRE(!pre || (!hex_pair(m, N) && a.do_vn),
+  OMEGA_REQUIRE(!pre || (hex_writes_pre(m, lc, mc) && a.do_vn),
                 "tendency launch: this kernel form does not write the VJP rows");
   const PreOut po = pre ? *pre : PreOut{};
   if (hex_pair(m, N) && m.hexu_ok) {
@@ -245,7 +276,8 @@ void launch_tend_hex(const MeshView &m, const Launch &lc, const ModelCoef &mc, c
                   : (m.off32 ? k_tend_hex2v<false, false> : k_tend_hex2v<false, true>);
     if (U && m.off32 && m.term_sets && a.do_h && a.do_vn)
       if (auto ks = tend_term_set(a.terms, a.epi)) kern = ks;
-    hipLaunchKernelGGL(kern, grid, block, 0, s, m, mc, a, d, lc.swizzle);
+    if (pre) kern = tend_writing_pre(a.terms, m.term_sets && a.do_h && a.do_vn);
+    hipLaunchKernelGGL(kern, grid, block, 0, s, m, mc, a, d, lc.swizzle, po);
   } else if (hex_pair(m, N)) {
     auto kern = U ? (m.off32 ? k_tend_hex2<true, false> : k_tend_hex2<true, true>)
                   : (m.off32 ? k_tend_hex2<false, false> : k_tend_hex2<false, true>);

@@ -164,7 +164,8 @@ void run_stage_into(omega_model_s *M, StageArgs a, hipStream_t s, int fields,
 bool stage_writes_pre(omega_model_s *M) {
   const MeshView mv = M->view();
   return (M->forms & 8) && !M->linear && !general_only(M) &&
-         !M->params.disable_vn_tendency && hex_usable(mv, M->lc) && hex_writes_pre(mv, M->lc);
+         !M->params.disable_vn_tendency && hex_usable(mv, M->lc) &&
+         hex_writes_pre(mv, M->lc, M->coef());
 }
 
 void run_stage(omega_model_s *M, StageArgs a, hipStream_t s, int fields, bool with_diag,

@@ -613,13 +613,14 @@ def test_tendency_vjp_slot_uniform_form_same_bits(terms, ntr, L):
 @pytest.mark.parametrize("terms,ntr", [(ALL_TERMS, 2), (1 | 2 | 4 | 8, 0),
                                        (1 | 2 | 4 | 8 | 64, 2), (ADVECTION_ONLY, 1),
                                        (2 | 4 | 8 | 32, 1)])
-@pytest.mark.parametrize("L,lpt", [(16, 2), (32, 2), (64, 1)])
+@pytest.mark.parametrize("L,lpt", [(16, 2), (32, 2), (64, 1), (64, 2), (128, 2)])
 @pytest.mark.parametrize("kind,nstages", [("rk4", 4), ("lsrk", 5), ("lsrk", 2)])
 def test_step_vjp_stage_written_rows_same_bits(kind, nstages, L, lpt, terms, ntr):
-    """Where the one-cell-per-row tendency kernel runs (16 or 32 layers, or
-    one level per lane) a step adjoint's stage recomputation also writes the
-    rows its tendency VJPs linearise about (kernel flag bit 4: edge PV sums,
-    thickness fluxes and their edges-on-edge sums), and those VJPs skip their
+    """A step adjoint's stage recomputation also writes the rows its tendency
+    VJPs linearise about (kernel flag bit 4: edge PV sums, thickness fluxes
+    and their edges-on-edge sums) with the one-cell-per-row tendency kernel
+    (16 or 32 layers, one level per lane) and with the pair kernel (64k
+    layers, its compiled term sets included), and those VJPs skip their
     pre-pass.  With that form forbidden (forms bit 3) the pre-pass forms the
     same rows: same bits for the state cotangents and the coefficient
     gradients."""
@@ -643,13 +644,19 @@ def test_step_vjp_stage_written_rows_same_bits(kind, nstages, L, lpt, terms, ntr
 
 
 def test_stage_written_rows_off_where_not_written():
-    """The rows are written only by the one-cell-per-row kernel with the vn
-    tendency on: not by the 64-layer pair kernel, not with the vn tendency
+    """The rows are written by the one-cell-per-row kernel and by the pair
+    kernel's shared-coefficient 32-bit form, with the vn tendency on: not by
+    the pair kernel's per-cell or 64-bit forms, not with the vn tendency
     disabled, not on the general kernels."""
     gm, gmodel, pd, state = setup(8, 8, 64, 1)
-    assert not gmodel.kernel_flags() & 16
-    gmodel.set_launch(0, 1)
     assert gmodel.kernel_flags() & 16
+    for forms in (14, 13):
+        gmodel.set_kernel_forms(forms)
+        assert not gmodel.kernel_flags() & 16
+        gmodel.set_launch(0, 1)
+        assert gmodel.kernel_flags() & 16
+        gmodel.set_launch(0, 2)
+    gmodel.set_kernel_forms(15)
     gm, gmodel, pd, state = setup(8, 8, 32, 1)
     assert gmodel.kernel_flags() & 16
     gmodel.set_variant(1)
