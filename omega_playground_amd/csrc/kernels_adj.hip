@@ -701,6 +701,38 @@ __global__ void __launch_bounds__(kAB)
       Vd<N> ihc;
 #pragma unroll
       for (int i = 0; i < N; ++i) ihc[i] = 1.0 / hc[i];
+      // HO (six edges, term set compiled in): the diffusion terms'
+      // tracer-independent rows of the six edges (the thickness flux, the mean
+      // thickness and 1/h across the edge) are loaded once for all tracers
+      // (with run-time terms their registers would cost occupancy)
+      constexpr bool HO = H && TM >= 0;
+      Vd<N> fuH[6], FH[6], ihoH[6];
+      if constexpr (HO) {
+        if (mix && a.ntr > 0) {
+#pragma unroll
+          for (int j = 0; j < 6; ++j) {
+            const int e = U ? ec[j] : m.eoc[c * me + j];
+            bool second;
+            int o;
+            if constexpr (U) {
+              second = am.ufirst[j] == 0;
+              o = m.coc[c * 6 + j];
+            } else {
+              const int a0 = m.coe[2 * e], a1 = m.coe[2 * e + 1];
+              second = a1 == c;
+              o = second ? a0 : a1;
+            }
+            fuH[j] = rd<N>(s.fu_e, e, L, k);
+            const Vd<N> ho = rd<N>(a.h, o, L, k);
+#pragma unroll
+            for (int i = 0; i < N; ++i) {
+              const double h0 = second ? ho[i] : hc[i], h1 = second ? hc[i] : ho[i];
+              FH[j][i] = 0.5 * (h0 + h1);
+              ihoH[j][i] = td2 ? 1.0 / (second ? h0 : h1) : 0.0;
+            }
+          }
+        }
+      }
       for (int l = 0; l < a.ntr; ++l) {
         const size_t off = (size_t)l * NC * L;
         const Vd<N> tcl = rd<N>(a.tr + off, c, L, k);
@@ -737,31 +769,34 @@ __global__ void __launch_bounds__(kAB)
                        idc = ecoef<U>(m.inv_dc, e);
           const double c20 = eslot<U>(ac.ect2, e, Kj, 0), c21 = eslot<U>(ac.ect2, e, Kj, 1);
           const double s2 = ecoef<U>(m.s2, e), s4 = ecoef<U>(m.s4, e);
-          const Vd<N> fu = rd<N>(s.fu_e, e, L, k);
+          const Vd<N> fu = HO ? fuH[j] : rd<N>(s.fu_e, e, L, k);
           const Vd<N> bo = rd<N>(a.c.tr + off, o, L, k);
-          Vd<N> b0, b1, h0 = zero<N>(), h1 = zero<N>(), tt0 = zero<N>(), tt1 = zero<N>(),
-                        g0 = zero<N>(), g1 = zero<N>(), G0 = zero<N>(), G1 = zero<N>(),
-                        ih0 = zero<N>(), ih1 = zero<N>();
+          Vd<N> b0, b1, F, tt0 = zero<N>(), tt1 = zero<N>(), g0 = zero<N>(), g1 = zero<N>(),
+                           G0 = zero<N>(), G1 = zero<N>(), ih0 = zero<N>(), ih1 = zero<N>();
 #pragma unroll
           for (int i = 0; i < N; ++i) {
             b0[i] = second ? bo[i] : bcl[i];
             b1[i] = second ? bcl[i] : bo[i];
           }
-          if (mix) {
+          Vd<N> iho;
+          if constexpr (HO) {
+            F = FH[j];
+            iho = ihoH[j];
+          } else {
             const Vd<N> ho = rd<N>(a.h, o, L, k);
 #pragma unroll
             for (int i = 0; i < N; ++i) {
-              h0[i] = second ? ho[i] : hc[i];
-              h1[i] = second ? hc[i] : ho[i];
+              const double h0 = second ? ho[i] : hc[i], h1 = second ? hc[i] : ho[i];
+              F[i] = 0.5 * (h0 + h1);
+              iho[i] = td2 ? 1.0 / (second ? h0 : h1) : 0.0;
             }
           }
           if (td2) {
             const Vd<N> to = rd<N>(a.tr + off, o, L, k);
 #pragma unroll
             for (int i = 0; i < N; ++i) {
-              const double iho = 1.0 / (second ? h0[i] : h1[i]);
-              ih0[i] = second ? iho : ihc[i];
-              ih1[i] = second ? ihc[i] : iho;
+              ih0[i] = second ? iho[i] : ihc[i];
+              ih1[i] = second ? ihc[i] : iho[i];
               tt0[i] = second ? to[i] : tcl[i];
               tt1[i] = second ? tcl[i] : to[i];
             }
@@ -779,18 +814,17 @@ __global__ void __launch_bounds__(kAB)
           const double cdc = rslot<U>(m.cdiv, c, me, j);
 #pragma unroll
           for (int i = 0; i < N; ++i) {
-            const double F = 0.5 * (h0[i] + h1[i]);
             const double w = cd0 * b0[i] + cd1 * b1[i];
             nb[i] += 0.5 * (-w * fu[i]);
             if (td2) {
               const double grad = (tt1[i] * ih1[i] - tt0[i] * ih0[i]) * idc;
               const double w2 = c20 * b0[i] + c21 * b1[i];
-              nb[i] += sg * w2 * F * s2 * idc;
-              t2[i] += cdc * F * s2 * grad;
+              nb[i] += sg * w2 * F[i] * s2 * idc;
+              t2[i] += cdc * F[i] * s2 * grad;
             }
             if (td4) {
               const double wg = cd0 * g0[i] + cd1 * g1[i];
-              nb[i] += sg * wg * F * idc;
+              nb[i] += sg * wg * F[i] * idc;
               t4[i] -= cdc * s4 * (G1[i] - G0[i]) * idc;
             }
           }
