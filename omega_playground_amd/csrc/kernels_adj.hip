@@ -101,7 +101,10 @@ template <int N> __device__ __forceinline__ Seg segment(const MeshView &m, int s
   const int span = (int)blockDim.x * N;
   const int nch = (m.L + span - 1) / span;
   int s, ch;
-  if (nch % 8 == 0) {
+  if (nch == 1) {
+    s = logical_block(blockIdx.x, gridDim.x, swz);
+    ch = 0;
+  } else if (nch % 8 == 0) {
     s = __builtin_amdgcn_readfirstlane((int)blockIdx.x / nch);
     ch = (int)blockIdx.x - s * nch;
   } else {
