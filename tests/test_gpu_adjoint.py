@@ -613,7 +613,7 @@ def test_tendency_vjp_slot_uniform_form_same_bits(terms, ntr, L):
 @pytest.mark.parametrize("terms,ntr", [(ALL_TERMS, 2), (1 | 2 | 4 | 8, 0),
                                        (1 | 2 | 4 | 8 | 64, 2), (ADVECTION_ONLY, 1),
                                        (2 | 4 | 8 | 32, 1)])
-@pytest.mark.parametrize("L,lpt", [(16, 2), (32, 2), (64, 1), (64, 2), (128, 2)])
+@pytest.mark.parametrize("L,lpt", [(16, 2), (32, 2), (64, 1), (64, 2), (128, 2), (512, 2)])
 @pytest.mark.parametrize("kind,nstages", [("rk4", 4), ("lsrk", 5), ("lsrk", 2)])
 def test_step_vjp_stage_written_rows_same_bits(kind, nstages, L, lpt, terms, ntr):
     """A step adjoint's stage recomputation also writes the rows its tendency

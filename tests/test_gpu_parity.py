@@ -388,13 +388,15 @@ def test_general_mpas_mesh_masks_and_scalings_bitwise(ordering):
 
 
 @pytest.mark.parametrize("nx,ny,L,ntr", [(4, 4, 1, 0), (5, 4, 3, 1), (7, 6, 48, 2), (8, 6, 65, 0),
-                                         (6, 6, 100, 9), (8, 6, 128, 2), (6, 4, 192, 1)])
+                                         (6, 6, 100, 9), (8, 6, 128, 2), (6, 4, 192, 1),
+                                         (6, 4, 512, 1), (4, 4, 1024, 2)])
 def test_mesh_layer_and_tracer_counts_bitwise(nx, ny, L, ntr):
     """Edge shapes: the smallest periodic meshes, odd mesh sizes, one layer,
     layer counts that are not a multiple of the 64-level row (partial level
     chunks, general kernels) and several rows of 64 (pattern kernels over
-    level chunks), no tracers and many.  Tendency, one RK4 step replayed
-    from a graph and one LSRK5 step, each bit for bit."""
+    level chunks, with 8 and 16 chunks the chunk fastest in the grid), no
+    tracers and many.  Tendency, one RK4 step replayed from a graph and one
+    LSRK5 step, each bit for bit."""
     print("case", nx, ny, L, ntr, flush=True)
     om, omodel, gm, gmodel, state = make_pair(nx, ny, L, ntr, seed=L + ntr)
     tend_o = omodel.new_state()
