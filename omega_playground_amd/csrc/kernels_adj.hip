@@ -516,7 +516,7 @@ __global__ void __launch_bounds__(kAB)
       }
     }
   }
-  const int b = blockIdx.y * gridDim.x + blockIdx.x;
+  const int b = blockIdx.x;
   block_partial(a, p_v2, s.partial + b);
 }
 
@@ -642,7 +642,7 @@ __global__ void __launch_bounds__(kAB) __attribute__((amdgpu_waves_per_eu(4)))
       st<N>(s.fbar_e + (size_t)e * L + k, fb);
     });
   }
-  const int b = blockIdx.y * gridDim.x + blockIdx.x;
+  const int b = blockIdx.x;
   block_partial(a, p_v4, s.partial + a.nparts + b);
 }
 
@@ -846,7 +846,7 @@ __global__ void __launch_bounds__(kAB)
       adj_out<N>(a, 0, a.c.oh, (size_t)c * L + k, hb);
     }
   }
-  const int b = blockIdx.y * gridDim.x + blockIdx.x;
+  const int b = blockIdx.x;
   block_partials2(a, p_t2, p_t4, s.partial + 2 * a.nparts + b, s.partial + 3 * a.nparts + b);
 }
 

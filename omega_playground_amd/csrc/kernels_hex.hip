@@ -212,8 +212,8 @@ TendFn tend_term_set(int terms, int epi) {
   }
 }
 
-// lanes per row = 64 / N (one row covers 64 levels); blockIdx.y walks the
-// 64-level chunks
+// lanes per row = 64 / N (one row covers 64 levels); the one-dimensional
+// grid walks (cell block, 64-level chunk) pairs (block_rows)
 // the two-cells-per-wavefront kernels need rows of 64 levels
 bool hex_pair(const MeshView &m, int N) { return N == 2 && m.L % 64 == 0; }
 
