@@ -652,6 +652,76 @@ __device__ __forceinline__ void cell_diag_p(const MeshView &m, int terms, int nt
   }
 }
 
+// Momentum del4 pre-pass in the pair layout (vel_hyperdiff_edge.hpp:140-210;
+// the general kernels' vdel4_edge / vdel4_vertex / vdel4_cell): del2 of the
+// velocity on an owned edge, then its curl at a vertex and divergence at a
+// cell, with the tables read at wavefront-uniform addresses.  Same
+// expressions and summation order, so the same bits.
+template <bool U, bool W>
+__device__ __forceinline__ void vdel4_edge_p(const MeshView &m, const DiagOut &d, int eA, int eB,
+                                             bool hi, int k) {
+  constexpr int N = 2;
+  const int L = m.L;
+  const int e = hi ? eB : eA;
+  const int a0 = pick2(m.coe, 2L * eA, 2L * eB, hi), a1 = pick2(m.coe, 2L * eA + 1, 2L * eB + 1, hi);
+  const int b0 = pick2(m.voe, 2L * eA, 2L * eB, hi), b1 = pick2(m.voe, 2L * eA + 1, 2L * eB + 1, hi);
+  const double idc = cpick<U>(m.inv_dc, eA, eB, hi), idv = cpick<U>(m.inv_dv4, eA, eB, hi);
+  const Vd<N> d0 = rowv<N, W>(d.div_c, a0, L, k), d1 = rowv<N, W>(d.div_c, a1, L, k);
+  const Vd<N> r0 = rowv<N, W>(d.rvort_v, b0, L, k), r1 = rowv<N, W>(d.rvort_v, b1, L, k);
+  Vd<N> out;
+#pragma unroll
+  for (int i = 0; i < N; ++i) out[i] = (d1[i] - d0[i]) * idc - (r1[i] - r0[i]) * idv;
+  st<N>(d.vdel2_e + (size_t)e * L + k, out);
+}
+
+template <bool U, bool W>
+__device__ __forceinline__ void vdel4_vertex_p(const MeshView &m, const DiagOut &d, int vA, int vB,
+                                               bool hi, int k) {
+  constexpr int N = 2;
+  const int L = m.L;
+  const int v = hi ? vB : vA;
+  int e[3];
+  double crv[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    e[j] = pick2(m.eov, 3L * vA + j, 3L * vB + j, hi);
+    crv[j] = cpick<U>(m.crv, 3L * vA + j, 3L * vB + j, hi);
+  }
+  Vd<N> x[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) x[j] = rowv<N, W>(d.vdel2_e, e[j], L, k);
+  Vd<N> rv = zero<N>();
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < N; ++i) rv[i] = rv[i] + crv[j] * x[j][i];
+  st<N>(d.vdel2_rv_v + (size_t)v * L + k, rv);
+}
+
+template <bool U, bool W>
+__device__ __forceinline__ void vdel4_cell_p(const MeshView &m, const DiagOut &d, int cA, int cB,
+                                             bool hi, int k) {
+  constexpr int N = 2;
+  const int L = m.L, me = m.me;
+  const int c = hi ? cB : cA;
+  int e[6];
+  double cd[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    e[j] = pick2(m.eoc, (long)cA * me + j, (long)cB * me + j, hi);
+    cd[j] = cpick<U>(m.cdiv, (long)cA * me + j, (long)cB * me + j, hi);
+  }
+  Vd<N> x[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) x[j] = rowv<N, W>(d.vdel2_e, e[j], L, k);
+  Vd<N> acc = zero<N>();
+#pragma unroll
+  for (int j = 0; j < 6; ++j)
+#pragma unroll
+    for (int i = 0; i < N; ++i) acc[i] = acc[i] + cd[j] * x[j][i];
+  st<N>(d.vdel2_div_c + (size_t)c * L + k, acc);
+}
+
 // Union form of vertex_diag_p x 2 + cell_diag_p: the rows of the cell's
 // six edges and seven cells (itself and its neighbours) plus the two other
 // edges of its owned vertices are loaded once and shared by the vertex,

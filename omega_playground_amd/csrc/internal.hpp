@@ -274,6 +274,10 @@ bool hex_usable(const MeshView &m, const Launch &lc);
 bool hex_writes_pre(const MeshView &m, const Launch &lc, const ModelCoef &mc);
 // the pair tendency kernel has a compiled-in form for this term set
 bool hex_term_set(int terms);
+// momentum del4 pre-pass in the pair layout (rows of 64 levels, two levels
+// per lane on the pattern-specialised mesh)
+bool hex_vdel4(const MeshView &m, const Launch &lc);
+void launch_vdel4_hex(const MeshView &m, const Launch &lc, const DiagOut &d, hipStream_t s);
 void launch_diag_hex(const MeshView &m, const Launch &lc, int terms, int ntr, const Real *h,
                      const Real *vn, const Real *tr, const DiagOut &d, hipStream_t s);
 void launch_tend_hex(const MeshView &m, const Launch &lc, const ModelCoef &mc, const StageArgs &a,

@@ -176,6 +176,39 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
   tend_hex2v<true, false, TM, -1, true>(m, mc, a, d, swz, po);
 }
 
+// Momentum del4 pre-pass, two cells per wavefront: del2 of the velocity on
+// each cell's three owned edges, then (after the whole grid) its curl at the
+// cell's two vertices and its divergence at the cell
+template <bool U, bool W>
+__global__ void __launch_bounds__(256) k_vdel4_edge_hex2(MeshView m, DiagOut d, int swz) {
+  int s;
+  const int k = block_rows(m, swz, 2, s);
+  const int R = blockDim.y;
+  const int cA = uniform(s * R + ((int)threadIdx.y & ~1));
+  if (cA >= m.nc) return;
+  const int cB = min(cA + 1, m.nc - 1);
+  const bool hi = (threadIdx.y & 1) != 0;
+  if (s * R + (int)threadIdx.y >= m.nc) return;
+  vdel4_edge_p<U, W>(m, d, 3 * cA, 3 * cB, hi, k);
+  vdel4_edge_p<U, W>(m, d, 3 * cA + 1, 3 * cB + 1, hi, k);
+  vdel4_edge_p<U, W>(m, d, 3 * cA + 2, 3 * cB + 2, hi, k);
+}
+
+template <bool U, bool W>
+__global__ void __launch_bounds__(256) k_vdel4_vc_hex2(MeshView m, DiagOut d, int swz) {
+  int s;
+  const int k = block_rows(m, swz, 2, s);
+  const int R = blockDim.y;
+  const int cA = uniform(s * R + ((int)threadIdx.y & ~1));
+  if (cA >= m.nc) return;
+  const int cB = min(cA + 1, m.nc - 1);
+  const bool hi = (threadIdx.y & 1) != 0;
+  if (s * R + (int)threadIdx.y >= m.nc) return;
+  vdel4_vertex_p<U, W>(m, d, 2 * cA, 2 * cB, hi, k);
+  vdel4_vertex_p<U, W>(m, d, 2 * cA + 1, 2 * cB + 1, hi, k);
+  vdel4_cell_p<U, W>(m, d, cA, cB, hi, k);
+}
+
 // Term sets compiled into the shared-coefficient, 32-bit-offset form:
 // momentum and thickness only (7), with del2 viscosity (15), with del2
 // viscosity and tracer del4 (79), everything (127); each also with the
@@ -263,6 +296,24 @@ void launch_diag_hex(const MeshView &m, const Launch &lc, int terms, int ntr, co
     hipLaunchKernelGGL(k_diag_hex<2>, grid, block, 0, s, m, terms, ntr, h, vn, tr, d, lc.swizzle);
   else
     hipLaunchKernelGGL(k_diag_hex<1>, grid, block, 0, s, m, terms, ntr, h, vn, tr, d, lc.swizzle);
+  OMEGA_HIP_CHECK(hipGetLastError());
+}
+
+bool hex_vdel4(const MeshView &m, const Launch &lc) {
+  return hex_usable(m, lc) && hex_pair(m, hex_n(m, lc));
+}
+
+void launch_vdel4_hex(const MeshView &m, const Launch &lc, const DiagOut &d, hipStream_t s) {
+  TimerScope ts("vdel4", s);
+  OMEGA_REQUIRE(hex_vdel4(m, lc), "del4 pre-pass: the pair layout needs rows of 64 levels");
+  dim3 block;
+  const dim3 grid = hex_grid(m, lc, 2, block);
+  auto ke = m.coef_uni ? (m.off32 ? k_vdel4_edge_hex2<true, false> : k_vdel4_edge_hex2<true, true>)
+                       : (m.off32 ? k_vdel4_edge_hex2<false, false> : k_vdel4_edge_hex2<false, true>);
+  auto kv = m.coef_uni ? (m.off32 ? k_vdel4_vc_hex2<true, false> : k_vdel4_vc_hex2<true, true>)
+                       : (m.off32 ? k_vdel4_vc_hex2<false, false> : k_vdel4_vc_hex2<false, true>);
+  hipLaunchKernelGGL(ke, grid, block, 0, s, m, d, lc.swizzle);
+  hipLaunchKernelGGL(kv, grid, block, 0, s, m, d, lc.swizzle);
   OMEGA_HIP_CHECK(hipGetLastError());
 }
 

@@ -425,6 +425,7 @@ void DeviceMesh::build_device(int ord) {
   // tables the pair kernels may read from one cell for both (bit-identical
   // at every stencil slot: e = 3c+K, v = 2c+x on the hexagonal numbering)
   coef_uni = hex_ok && rows_equal(inv_dc_h, 1) && rows_equal(inv_dv_h, 1) &&
+             rows_equal(inv_dv4_h, 1) &&
              rows_equal(maxlev_h, 1) && rows_equal(s2_h, 1) && rows_equal(s4_h, 1) &&
              rows_equal(w_h, 3 * (size_t)me2) && rows_equal(cdiv_h, ME) &&
              rows_equal(cke_h, ME) && rows_equal(crv_h, 6) && rows_equal(cth_h, 6);

@@ -124,7 +124,10 @@ void run_diag_into(omega_model_s *M, const Real *h, const Real *vn, const Real *
   if (hex_usable(mv, M->lc) && !general_only(M))
     launch_diag_hex(mv, ld, dterms, ntr, h, vn, tr, d, s);
   else launch_diag(mv, ld, M->coef(), dterms, ntr, h, vn, tr, d, s);
-  if (do_vn && (M->terms & T_VDEL4)) launch_vdel4(mv, M->lc, d, s);
+  if (do_vn && (M->terms & T_VDEL4)) {
+    if (hex_vdel4(mv, M->lc) && !general_only(M)) launch_vdel4_hex(mv, M->lc, d, s);
+    else launch_vdel4(mv, M->lc, d, s);
+  }
 }
 
 void run_diag(omega_model_s *M, const Real *h, const Real *vn, const Real *tr, hipStream_t s) {
