@@ -167,13 +167,14 @@ __global__ void __launch_bounds__(256)
   tend_hex2v<U, W, TM, E, false>(m, mc, a, d, swz, po);
 }
 
-// The same with the VJP rows written (the adjoint's stage recomputation,
-// run-time epilogue); held at two waves per SIMD, where the stores' extra
-// registers would otherwise cost half the occupancy
-template <int TM>
+// The same with the VJP rows written (the adjoint's stage recomputation:
+// the RK4 provisional-stage or low-storage epilogue, compiled in with the
+// term set); held at two waves per SIMD, where the stores' extra registers
+// would otherwise cost half the occupancy
+template <int TM, int E>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     k_tend_hex2v_pre(MeshView m, ModelCoef mc, StageArgs a, DiagOut d, int swz, PreOut po) {
-  tend_hex2v<true, false, TM, -1, true>(m, mc, a, d, swz, po);
+  tend_hex2v<true, false, TM, E, true>(m, mc, a, d, swz, po);
 }
 
 // Momentum del4 pre-pass, two cells per wavefront: del2 of the velocity on
@@ -224,15 +225,21 @@ template <int TM> TendFn tend_for_epilogue(int epi) {
   default: return k_tend_hex2v<true, false, TM>;
   }
 }
-// the forms that also write the VJP rows (run-time epilogue: the adjoint's
-// stage recomputation)
-TendFn tend_writing_pre(int terms, bool term_sets) {
+// the forms that also write the VJP rows (the adjoint's stage recomputation)
+template <int TM> TendFn tend_writing_pre_epi(int epi) {
+  switch (epi) {
+  case EPI_PROVIS: return k_tend_hex2v_pre<TM, EPI_PROVIS>;
+  case EPI_LSRK: return k_tend_hex2v_pre<TM, EPI_LSRK>;
+  default: return k_tend_hex2v_pre<-1, -1>;
+  }
+}
+TendFn tend_writing_pre(int terms, bool term_sets, int epi) {
   switch (term_sets ? terms : -1) {
-  case 7: return k_tend_hex2v_pre<7>;
-  case 15: return k_tend_hex2v_pre<15>;
-  case 79: return k_tend_hex2v_pre<79>;
-  case 127: return k_tend_hex2v_pre<127>;
-  default: return k_tend_hex2v_pre<-1>;
+  case 7: return tend_writing_pre_epi<7>(epi);
+  case 15: return tend_writing_pre_epi<15>(epi);
+  case 79: return tend_writing_pre_epi<79>(epi);
+  case 127: return tend_writing_pre_epi<127>(epi);
+  default: return k_tend_hex2v_pre<-1, -1>;
   }
 }
 TendFn tend_term_set(int terms, int epi) {
@@ -332,7 +339,7 @@ void launch_tend_hex(const MeshView &m, const Launch &lc, const ModelCoef &mc, c
                   : (m.off32 ? k_tend_hex2v<false, false> : k_tend_hex2v<false, true>);
     if (U && m.off32 && m.term_sets && a.do_h && a.do_vn)
       if (auto ks = tend_term_set(a.terms, a.epi)) kern = ks;
-    if (pre) kern = tend_writing_pre(a.terms, m.term_sets && a.do_h && a.do_vn);
+    if (pre) kern = tend_writing_pre(a.terms, m.term_sets && a.do_h && a.do_vn, a.epi);
     hipLaunchKernelGGL(kern, grid, block, 0, s, m, mc, a, d, lc.swizzle, po);
   } else if (hex_pair(m, N)) {
     auto kern = U ? (m.off32 ? k_tend_hex2<true, false> : k_tend_hex2<true, true>)
