@@ -177,13 +177,16 @@ def reference_benchmark(nx: int = 512, nlayers: int = 64, ntracers: int = 1, nst
     }
 
 
-def adjoint_bench(n_snapshots: int = 10, nsteps: int | None = None, graph: bool = True):
+def adjoint_bench(n_snapshots: int = 10, nsteps: int | None = None, graph: bool = True,
+                  config: str = "c3"):
     """The adjoint workload: RK4 on nx=256, 32 layers, 2 tracers, del2
     viscosity on; misfit of h and tracers after the trajectory against
     observations from a run whose initial h was perturbed by a correlated
     Gaussian random field; gradient with respect to the initial state and
-    the mixing coefficients with binomial checkpointing."""
-    cfg = dict(wl.CONFIGS["c3"])
+    the mixing coefficients with binomial checkpointing.  config "c2" runs
+    the same on the forward workload's shape (nx=512, 64 layers, 4 tracers,
+    del2 viscosity and tracer del4), for profiles of the 64-layer kernels."""
+    cfg = dict(wl.CONFIGS[config])
     nsteps = nsteps or cfg["nsteps"]
     mesh = swe.PlanarHexagonalMesh(cfg["nx"], cfg["nx"], cfg["dc"], cfg["nlayers"])
     params = swe.ShallowWaterParams(f0=cfg["f0"], grav=cfg["grav"], visc_del2=cfg["visc_del2"],
@@ -237,8 +240,10 @@ def adjoint_bench(n_snapshots: int = 10, nsteps: int | None = None, graph: bool 
     recompute = swe.revolve_cost(nsteps, n_snapshots)
     return {
         "workload": f"RK4 adjoint, planar hex nx={cfg['nx']} ({nc} cells), {L} layers, {ntr} "
-                    f"tracers, del2 visc {cfg['visc_del2']:g}, {nsteps} steps, {n_snapshots} "
-                    f"snapshots, misfit of h and tracers vs perturbed-h0 observations",
+                    f"tracers, del2 visc {cfg['visc_del2']:g}"
+                    + (f", tracer del4 {cfg['eddy_diff4']:g}" if cfg["eddy_diff4"] else "")
+                    + f", {nsteps} steps, {n_snapshots} snapshots, misfit of h and tracers vs "
+                    f"perturbed-h0 observations",
         "fwd_ms_per_step": 1e3 * t_fwd,
         "step_adjoint_ms": 1e3 * t_adj,
         "ratio_per_step": t_adj / t_fwd,
@@ -482,6 +487,9 @@ def main():
                     help="steps of the ensemble adjoint leg (the workload's length)")
     ap.add_argument("--skip-reference-shape", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=50)
+    ap.add_argument("--adjoint-config", choices=("c3", "c2"), default="c3",
+                    help="adjoint leg on its own workload (c3) or on the forward workload's "
+                         "shape (c2: nx=512, 64 layers, 4 tracers; for profiles)")
     ap.add_argument("--adjoint-steps", type=int, default=None,
                     help="trajectory length of the adjoint leg (default: the workload's 200)")
     ap.add_argument("--cpu-runs", type=int, default=3)
@@ -567,7 +575,8 @@ def main():
     adj = None
     if not args.skip_adjoint:
         log("[bench] adjoint workload")
-        adj = adjoint_bench(nsteps=args.adjoint_steps, graph=not args.no_graph)
+        adj = adjoint_bench(nsteps=args.adjoint_steps, graph=not args.no_graph,
+                            config=args.adjoint_config)
         log(f"[bench] adjoint: step ratio {adj['ratio_per_step']:.2f}, trajectory ratio "
             f"{adj['ratio_total']:.2f} ({adj['adjoint_run_s']:.3f} s)")
 
