@@ -360,7 +360,7 @@ def batched_bench(nmembers: int = 48, nsteps: int = 50, specs=None, label=None, 
     }
 
 
-def ensemble_adjoint_bench(nsteps: int | None = None, n_snapshots: int = 10):
+def ensemble_adjoint_bench(nsteps: int | None = None, n_snapshots: int = 10, overrides=None):
     """The ensemble adjoint workload: 16 members on one nx=128 mesh, 64
     layers, 2 tracers, RK4, shared del2 viscosity 1e3 and tracer del4 1e9;
     per-member seeded initial states and observations (forward runs from
@@ -370,6 +370,7 @@ def ensemble_adjoint_bench(nsteps: int | None = None, n_snapshots: int = 10):
     sweep for the ensemble against a loop of per-member adjoint runs with
     the same snapshot count (the one the cap leaves the ensemble) and cap."""
     cfg = dict(wl.CONFIGS["c5"])
+    cfg.update(overrides or {})
     nsteps = nsteps or cfg["nsteps"]
     M, ntr, L = cfg["nmembers"], cfg["ntracers"], cfg["nlayers"]
     mesh = swe.PlanarHexagonalMesh(cfg["nx"], cfg["nx"], cfg["dc"], L)
@@ -608,6 +609,12 @@ def main():
         eadj = ensemble_adjoint_bench(args.c5_steps)
         log(f"[bench] ensemble adjoint: batched {eadj['batched_s']:.3f} s vs looped "
             f"{eadj['looped_s']:.3f} s = {eadj['speedup']:.2f}x, {eadj['snapshots']} snapshots")
+        # where stacking pays: 16 small members (nx=32, 16 layers), whose
+        # per-member runs cannot fill the GPU
+        sm = ensemble_adjoint_bench(100, overrides=dict(nx=32, nlayers=16))
+        eadj["small_ensemble_adjoint"] = sm
+        log(f"[bench] small ensemble adjoint: batched {sm['batched_s']:.4f} s vs looped "
+            f"{sm['looped_s']:.4f} s = {sm['speedup']:.2f}x, {sm['snapshots']} snapshots")
 
     refb = None
     if not args.skip_reference_shape:
